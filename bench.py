@@ -1,0 +1,179 @@
+"""BM25 full-Search benchmark on one MI355X.
+
+Workload (default): the 10,000,000-doc synthetic corpus (vocabulary 1,000,000,
+Zipf 1.07, lognormal lengths of mean 56, k1 1.5, b 0.75, seed 0), Q=8 queries of
+3 distinct df-weighted terms, K=10. One step is one full Search of one Q8 batch:
+host term IDs in, host top-K (doc IDs + FP32 scores) out, index resident in HBM.
+Steps alternate between two fixed batches A and B, like the reference's A/B
+protocol (FR/test/ascendhost/bm25/benchmark_common.h:51-107).
+
+Prints one JSON line: QPS (value), ms per Search, the scoring kernel's HBM
+roofline fraction, the CPU oracle timed on this host's cores, and parity.
+"""
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT), str(ROOT / 'bm25-npu_amd')]
+
+import bm25mi  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def nearest_rank(values, p):
+    v = sorted(values)
+    return v[max(0, int(np.ceil(p * len(v))) - 1)]
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in Path('/proc/cpuinfo').read_text().splitlines():
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def algorithmic_bytes(session, offsets, terms, k, stats):
+    """Bytes one Search must move: 8 B per scanned posting ({doc, impact}), 8 B of
+    directory per (long term, block) pair, the query packet and the top-K output."""
+    nq = len(offsets) - 1
+    return 8 * stats['postings_scanned'] + 8 * (nq + 1) + 4 * len(terms) + 8 * nq * k
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--corpus', default='synthetic10m')
+    ap.add_argument('--queries', type=int, default=8)
+    ap.add_argument('--terms', type=int, default=3)
+    ap.add_argument('--k', type=int, default=10)
+    ap.add_argument('--block-docs', type=int, default=8192)
+    ap.add_argument('--cpu-calls', type=int, default=10, help='timed CPU oracle calls (A/B alternating)')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: no CPU leg')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)',
+                          'n_gpus': args.gpus}))
+        sys.exit(2)
+
+    t0 = time.time()
+    corpus = bm25mi.generate(args.corpus)
+    index = corpus.index()
+    log(f'corpus {args.corpus}: N={index.num_docs} V={index.vocab} M={index.postings} ({time.time() - t0:.1f}s)')
+    qa = bm25mi.draw_queries(corpus.offsets, args.queries, args.terms, seed=1)
+    qb = bm25mi.draw_queries(corpus.offsets, args.queries, args.terms, seed=2)
+    packed = [bm25mi.pack(qa), bm25mi.pack(qb)]
+    del corpus
+
+    t0 = time.time()
+    session = bm25mi.Session(index, max_queries=args.queries, max_words=args.queries * args.terms, max_k=args.k,
+                             block_docs=args.block_docs)
+    log(f'session create {time.time() - t0:.1f}s')
+
+    results = {}
+    for step in range(args.warmup):
+        session.search(packed[step % 2], args.k)
+    # Timed region: K end-to-end Searches (host IDs -> host top-K), graph replay.
+    samples = []
+    t_start = time.perf_counter()
+    for step in range(args.steps):
+        t = time.perf_counter()
+        s, i, _ = session.search(packed[step % 2], args.k)
+        samples.append((time.perf_counter() - t) * 1e3)
+        results[step % 2] = (s, i)
+    total_s = time.perf_counter() - t_start
+    log(f'timed {args.steps} Searches: median {statistics.median(samples):.4f} ms')
+
+    # Scoring-kernel time with HIP events on the launch stream (direct launches).
+    kernel_ms, bytes_per = [], []
+    for step in range(max(20, min(args.steps, 100))):
+        offs, terms = packed[step % 2]
+        session.search(packed[step % 2], args.k, timed=True)
+        st = session.stats()
+        kernel_ms.append(st['score_kernel_ms'])
+        bytes_per.append(algorithmic_bytes(session, offs, terms, args.k, st))
+    k_ms = statistics.mean(kernel_ms)
+    b_avg = statistics.mean(bytes_per)
+    achieved = b_avg / (k_ms * 1e-3) / 1e9
+    st = session.stats()
+
+    record = {
+        'metric': 'Q8/K10 full-Search ms @10M docs; QPS; % HBM roofline; learned vs plain',
+        'value': args.queries * args.steps / total_s,
+        'unit': 'queries/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': 1e3 * total_s / args.steps,
+        'ms_median': statistics.median(samples),
+        'ms_p95': nearest_rank(samples, 0.95),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': f'{args.corpus} full Search, Q{args.queries}x{args.terms} terms, K{args.k}',
+                   'docs': index.num_docs, 'vocab': index.vocab, 'postings': index.postings,
+                   'queries': args.queries, 'terms_per_query': args.terms, 'k': args.k,
+                   'block_docs': args.block_docs, 'k1': index.k1, 'b': index.b},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'KScore', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg},
+        'postings_per_search': st['postings_total'],
+    }
+
+    if not args.profile and not args.no_cpu:
+        from oracle import oracle
+        cores = host_cores()
+        threads = min(32, cores)
+        mismatches = 0
+        cpu_ms = []
+        for call in range(args.cpu_calls + 2):
+            offs, terms = packed[call % 2]
+            t = time.perf_counter()
+            want_s, want_i = oracle.search(index, offs, terms, args.k, shards=threads, threads=threads,
+                                           chunk=args.queries)
+            dt = (time.perf_counter() - t) * 1e3
+            if call >= 2:
+                cpu_ms.append(dt)
+            got_s, got_i = results[call % 2]
+            mismatches += int((got_i != want_i).sum()) + int((got_s.view(np.uint32) != want_s.view(np.uint32)).sum())
+        record['cpu_baseline'] = {
+            'value': args.queries * 1e3 / statistics.mean(cpu_ms), 'unit': 'queries/s', 'cores': threads,
+            'kind': 'port', 'ms_median': statistics.median(cpu_ms),
+            'sample': f'{args.cpu_calls} A/B-alternating Q{args.queries} Searches of the same workload after 2 '
+                      f'warm-up calls; oracle/bm25_oracle.c (restated reference CPU Search), '
+                      f'{threads} OpenMP threads = shards, {cpu_model()}'}
+        record['parity'] = {'mismatches_vs_oracle': mismatches, 'checked_batches': 2,
+                            'rule': 'doc IDs and FP32 score bits equal'}
+    print(json.dumps(record), flush=True)
+    session.close()
+
+
+if __name__ == '__main__':
+    main()

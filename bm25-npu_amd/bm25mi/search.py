@@ -1,0 +1,272 @@
+"""Search surfaces over the C ABI.
+
+* Session            - the resident-index session (bm25mi_create / search / destroy).
+* SearchSession      - Python Search(query term IDs, K) -> (scores, ids) with the
+                       request/response checks of the reference's raw-ID wrapper
+                       (standalone/run_bm25s_index_search.py:15-76).
+* AscendIndexBm25    - Init / Search / Close / GetLastError with the facade's
+                       status codes, output atomicity and state machine
+                       (FR/ascendhost/include/index/AscendIndexBm25.h:17-57,
+                       FR/ascendhost/src/bm25/AscendIndexBm25.cpp:33-84).
+"""
+
+import ctypes as C
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .corpus import pack
+
+LIMIT = 2**31 - 1
+
+
+def _is_int(value, low, high):
+    return type(value) is int and low <= value <= high
+
+
+class Session:
+    """A resident index on one GPU. All searches go through libbm25mi."""
+
+    def __init__(self, index, max_queries=8, max_words=24, max_k=10, block_docs=8192, k1=None, device=0,
+                 use_graph=True, grid_blocks=0):
+        self._lib = _lib.load()
+        cfg = _lib.Config()
+        self._lib.bm25mi_config_default(C.byref(cfg))
+        cfg.device_id = device
+        cfg.max_queries = max_queries
+        cfg.max_words = max_words
+        cfg.max_k = max_k
+        cfg.block_docs = block_docs
+        cfg.k1 = index.k1 if k1 is None else k1
+        cfg.use_graph = 1 if use_graph else 0
+        cfg.grid_blocks = grid_blocks
+        self.config = cfg
+        self.num_docs, self.vocab = index.num_docs, index.vocab
+        handle = C.c_void_p()
+        _lib.check(self._lib.bm25mi_create(_lib.ptr(index.offsets), index.vocab, _lib.ptr(index.docs),
+                                           _lib.ptr(index.tf), index.postings, _lib.ptr(index.norm),
+                                           index.num_docs, _lib.ptr(index.idf), C.byref(cfg), C.byref(handle)))
+        self._h = handle
+
+    @property
+    def handle(self):
+        if self._h is None or not self._h.value:
+            raise _lib.Bm25Error(_lib.ILLEGAL_OPERATION, 'Session closed')
+        return self._h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            _lib.check(self._lib.bm25mi_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _prepare(queries, k):
+        if isinstance(queries, tuple):
+            offsets, terms = queries
+            offsets = np.ascontiguousarray(offsets, np.int64)
+            terms = np.ascontiguousarray(terms, np.int32)
+        else:
+            offsets, terms = pack(queries)
+        nq = len(offsets) - 1
+        ks = np.ascontiguousarray(np.full(nq, k, np.int32) if np.isscalar(k) else np.asarray(k, np.int32))
+        if len(ks) != nq:
+            raise ValueError('one K per query required')
+        return offsets, terms, nq, ks
+
+    def search(self, queries, k, thresholds=None, timed=False):
+        """Exact top-K per query. queries: list of term-ID lists or (offsets, terms).
+
+        k: int or per-query array. thresholds: optional per-query predicted K-th
+        scores (learned path; results are still exact). Returns (scores, ids,
+        fallback) with rows concatenated per query (shape (Q, K) when k is an int).
+        """
+        offsets, terms, nq, ks = self._prepare(queries, k)
+        rows = int(ks.sum())
+        scores = np.empty(rows, np.float32)
+        ids = np.empty(rows, np.int32)
+        fallback = np.zeros(nq, np.int32)
+        thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
+        h = self.handle
+        if timed:
+            rc = self._lib.bm25mi_search_timed(h, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
+                                               _lib.ptr(thr), _lib.ptr(scores), _lib.ptr(ids), _lib.ptr(fallback))
+        elif thr is not None:
+            rc = self._lib.bm25mi_search_pruned(h, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
+                                                _lib.ptr(thr), _lib.ptr(scores), _lib.ptr(ids), _lib.ptr(fallback))
+        else:
+            rc = self._lib.bm25mi_search_batch(h, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
+                                               _lib.ptr(scores), _lib.ptr(ids))
+        _lib.check(rc, h)
+        if np.isscalar(k):
+            scores, ids = scores.reshape(nq, int(k)), ids.reshape(nq, int(k))
+        return scores, ids, fallback
+
+    def features(self, queries, k):
+        offsets, terms, nq, ks = self._prepare(queries, k)
+        out = np.empty((nq, _lib.FEATURES), np.float32)
+        _lib.check(self._lib.bm25mi_query_features(self.handle, _lib.ptr(offsets), _lib.ptr(terms), nq,
+                                                   _lib.ptr(ks), _lib.ptr(out)), self.handle)
+        return out
+
+    def stats(self):
+        st = _lib.Stats()
+        _lib.check(self._lib.bm25mi_last_stats(self.handle, C.byref(st)), self.handle)
+        return st.as_dict()
+
+
+class SearchSession:
+    """Search(query term IDs) -> (scores f32[Q,K], ids i32[Q,K]), K fixed per session.
+
+    Request checks follow standalone/run_bm25s_index_search.py:38-46 (a list of
+    1..max_queries rows, each a list; total words <= max_words; every word an int,
+    not bool, in [0, INT32_MAX-1]); response checks follow :56-75 (shape, dtype,
+    finite non-negative scores, in-range unique IDs, descending scores, ascending
+    IDs among equal scores).
+    """
+
+    def __init__(self, index, k, max_queries=8, max_words=24, **kwargs):
+        if not (_is_int(k, 1, min(_lib.MAX_K, index.num_docs)) and _is_int(max_queries, 1, 10000)
+                and _is_int(max_words, 0, LIMIT)):
+            raise ValueError('Invalid Search configuration')
+        self.k, self.maxq, self.maxwords = k, max_queries, max_words
+        self.num_docs = index.num_docs
+        self.session = Session(index, max_queries=max_queries, max_words=max(1, max_words), max_k=k, **kwargs)
+
+    def search(self, queries):
+        if not (type(queries) is list and 1 <= len(queries) <= self.maxq):
+            raise ValueError('Query batch is invalid or exceeds declared capacity')
+        total = 0
+        for row in queries:
+            if not (type(row) is list and len(row) <= self.maxwords - total):
+                raise ValueError('Query words exceed declared capacity')
+            for word in row:
+                if not _is_int(word, 0, LIMIT - 1):
+                    raise ValueError('Word IDs must be non-negative INT32 integers (bool rejected)')
+            total += len(row)
+        scores, ids, _ = self.session.search(queries, self.k)
+        if not (scores.shape == ids.shape == (len(queries), self.k) and scores.dtype == np.float32
+                and ids.dtype == np.int32 and np.isfinite(scores).all() and (scores >= 0).all()
+                and (ids >= 0).all() and (ids < self.num_docs).all()):
+            raise RuntimeError('Search output shape/dtype/score/id invalid')
+        for s, i in zip(scores, ids):
+            if len(np.unique(i)) != self.k or not (s[:-1] >= s[1:]).all() \
+                    or ((s[:-1] == s[1:]) & (i[:-1] >= i[1:])).any():
+                raise RuntimeError('Search returned duplicate IDs or broke the strict order')
+        return scores, ids
+
+    def close(self):
+        self.session.close()
+
+
+# APP_ERR classes of the facade (Bm25ErrorAdapter.h:11-45).
+APP_ERR_OK = _lib.OK
+APP_ERR_INVALID_PARAM = _lib.INVALID_PARAM
+APP_ERR_ILLEGAL_OPERATION = _lib.ILLEGAL_OPERATION
+APP_ERR_INNER_ERROR = _lib.INNER
+
+
+@dataclass
+class Bm25Config:
+    """Bm25Config (AscendIndexBm25.h:17-25); the SDK context/resources/vendor fields
+    have no counterpart on this stack."""
+    deviceId: int = 0
+    queryCount: int = 8
+    maxQueryWords: int = 24
+    topK: int = 10
+    k1: float = 1.2
+    blockDocs: int = 8192
+
+
+@dataclass
+class Bm25SearchResult:
+    scores: np.ndarray = field(default_factory=lambda: np.empty(0, np.float32))
+    labels: np.ndarray = field(default_factory=lambda: np.empty(0, np.int64))
+
+
+@dataclass
+class ErrorInfo:
+    aclCode: int = 0
+    message: str = ''
+
+
+class AscendIndexBm25:
+    """Init/Search/Close/GetLastError with the reference facade's semantics:
+    a second Init without Close is ILLEGAL_OPERATION; Search before Init is
+    ILLEGAL_OPERATION; the result is replaced only on success; input errors keep
+    the session usable; Close is idempotent; messages are cut to 1023 chars."""
+
+    def __init__(self):
+        self._mutex = threading.Lock()
+        self._session = None
+        self._k = 0
+        self._error = ErrorInfo()
+
+    def _invoke(self, op):
+        self._error = ErrorInfo()
+        try:
+            op()
+            return APP_ERR_OK
+        except _lib.Bm25Error as e:
+            self._error = ErrorInfo(e.device_code, e.message[:1023])
+            return e.code
+        except ValueError as e:
+            self._error = ErrorInfo(0, str(e)[:1023])
+            return APP_ERR_INVALID_PARAM
+        except Exception as e:  # noqa: BLE001 - the facade maps everything to a code
+            self._error = ErrorInfo(0, str(e)[:1023])
+            return APP_ERR_INNER_ERROR
+
+    def Init(self, index, config):
+        with self._mutex:
+            def op():
+                if self._session is not None:
+                    raise _lib.Bm25Error(APP_ERR_ILLEGAL_OPERATION, 'BM25 is already initialized; Close before Init')
+                checked = index.copy()
+                checked.validate()
+                self._session = Session(checked, max_queries=config.queryCount, max_words=config.maxQueryWords,
+                                        max_k=config.topK, block_docs=config.blockDocs, k1=config.k1,
+                                        device=config.deviceId)
+                self._k = config.topK
+            return self._invoke(op)
+
+    def Search(self, queries, result):
+        with self._mutex:
+            def op():
+                if self._session is None:
+                    raise _lib.Bm25Error(APP_ERR_ILLEGAL_OPERATION, 'BM25 is not initialized')
+                if not (isinstance(queries, list) and all(isinstance(r, list) for r in queries)):
+                    raise ValueError('Queries must be a list of lists of word IDs')
+                for r in queries:
+                    for w in r:
+                        if not _is_int(w, 0, LIMIT - 1):
+                            raise ValueError('Invalid public query word ID')
+                scores, ids, _ = self._session.search(queries, self._k)
+                result.scores = scores.reshape(-1).copy()
+                result.labels = ids.reshape(-1).astype(np.int64)
+            return self._invoke(op)
+
+    def Close(self):
+        with self._mutex:
+            def op():
+                if self._session is not None:
+                    self._session.close()
+                    self._session = None
+            return self._invoke(op)
+
+    def GetLastError(self):
+        with self._mutex:
+            return ErrorInfo(self._error.aclCode, self._error.message)

@@ -1,0 +1,53 @@
+"""Builds libbm25mi.so for gfx950 in-tree (bm25-npu_amd/lib/) with hipcc.
+
+Usage: python bm25-npu_amd/build.py [--jobs N]
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / 'csrc'
+LIB = HERE / 'lib'
+OBJ = HERE / 'lib' / 'obj'
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+# FP32 contract of the reference build (-ffp-contract=off -fno-fast-math); packed FP32
+# VALU ops are disabled for all device code.
+COMMON = ['-O3', '-fPIC', '-std=c++17', '-ffp-contract=off', '-fno-fast-math', '-pthread', '-Wall',
+          '-Wno-unused-command-line-argument']
+DEVICE = ['--offload-arch=gfx950', '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+SOURCES = ['search.hip', 'session.hip', 'mlp.hip', 'corpus.cpp', 'index.cpp']
+
+
+def compile_one(src):
+    out = OBJ / (src + '.o')
+    cmd = [HIPCC] + COMMON + (DEVICE + ['-x', 'hip'] if src.endswith('.hip') else []) + ['-c', str(CSRC / src), '-o', str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(f'compile {src} failed:\n{r.stdout}\n{r.stderr}')
+    return out
+
+
+def build(jobs=8):
+    OBJ.mkdir(parents=True, exist_ok=True)
+    sources = [s for s in SOURCES if (CSRC / s).exists()]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, sources))
+    target = LIB / 'libbm25mi.so'
+    tmp = LIB / 'libbm25mi.so.tmp'
+    cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=gfx950', '-pthread', '-o', str(tmp)] + [str(o) for o in objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(f'link failed:\n{r.stdout}\n{r.stderr}')
+    os.replace(tmp, target)
+    return target
+
+
+if __name__ == '__main__':
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--jobs', type=int, default=8)
+    a = ap.parse_args()
+    print(build(a.jobs))

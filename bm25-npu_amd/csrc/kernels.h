@@ -1,0 +1,56 @@
+// Device-side data structures shared by the Search kernels and the host session.
+#pragma once
+
+#include <cstdint>
+
+namespace bm25mi {
+
+constexpr int kThreads = 256;              // scoring workgroup: 4 wave64
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxTerms = 64;              // terms per query (fixed_query_packet.h:32)
+constexpr int kLongMin = 4096;             // lists this long use the block directory; shorter
+                                           // ones a two-round 64-ary in-kernel search (64*64)
+constexpr int kFinalThreads = 256;         // final selection workgroup
+constexpr int kFinalLds = 4096;            // pool entries staged in LDS by the final selection
+constexpr int kStats = 8;
+
+// Stats slots (unsigned long long).
+enum { kStatPostings = 0, kStatScored = 1, kStatSkipped = 2, kStatCandidates = 3, kStatOverflow = 4 };
+
+// Read-only resident index.
+struct IndexView {
+    const uint2 *post;      // {doc ID, impact bits} per posting, CSC order
+    const int64_t *toff;    // term offsets [vocab+1]
+    const int32_t *slot;    // long-term slot or -1 [vocab]
+    const uint32_t *dir;    // [longTerms * (granules+1)]
+    const float *bmax;      // [longTerms * granules]
+    const float *tmax;      // [vocab]
+    int64_t vocab;
+    int64_t granules;
+    int32_t ndocs;
+    int32_t granuleShift;
+};
+
+// One Search batch, laid out in a single device packet uploaded per Search.
+struct BatchView {
+    const int32_t *header;  // [0] = queries
+    const int64_t *qoff;    // [maxQ+1]
+    const int32_t *k;       // [maxQ]
+    const float *thr;       // [maxQ] pruning threshold (0 = plain)
+    const int64_t *poolOff; // [maxQ+1]
+    const int64_t *outOff;  // [maxQ+1]
+    const int32_t *terms;   // [maxWords]
+};
+
+// Per-Search device scratch and results.
+struct WorkView {
+    unsigned long long *pool; // composite keys (score bits << 32 | ~doc)
+    uint32_t *count;          // [maxQ] pool fill
+    uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
+    unsigned long long *stats;// [kStats]
+    int32_t *status;          // [maxQ] 0 ok, 1 fewer than K survivors under a threshold
+    float *scores;            // [maxQ*maxK]
+    int32_t *ids;             // [maxQ*maxK]
+};
+
+} // namespace bm25mi

@@ -1,0 +1,682 @@
+// Host runtime of libbm25mi: the resident-index session behind the C ABI.
+//
+// Mirrors bm25large::Session (FR/ascendhost/src/bm25/large_search.cpp:107-212, FR =
+// feature_retrieval/src/ascendfaiss): the constructor validates and uploads the
+// index once, allocates every per-Search buffer, captures H2D packet copy ->
+// kernels into one hipGraph and runs an empty bootstrap Search that must return
+// IDs 0..K-1 with score 0 (:184-187); Search validates the whole request before
+// touching the reusable packet (FR/common/bm25/fixed_query_packet.h:26-49),
+// replays the graph, synchronises and checks the result the way Session::Receive
+// does (:64-85). A device error marks the session failed for good (:208-210).
+//
+// Differences by design: impacts are precomputed at create time; the directory
+// covers only long posting lists; results land in mapped pinned host memory
+// written by the last kernel, so no D2H copy node is needed; Q and K may vary
+// per call up to the configured capacity, and K may differ per query.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "layout.h"
+
+namespace bm25mi {
+
+void LaunchReset(const BatchView &bt, const WorkView &wk, int32_t maxQ, hipStream_t stream);
+bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
+                 int32_t granulesPerBlock, int32_t grid, hipStream_t stream);
+void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, int32_t ndocs, hipStream_t stream);
+void LaunchPublish(const WorkView &wk, unsigned long long *hostStats, hipStream_t stream);
+
+#define BM25MI_HIP(expr, what)                                                                                         \
+    do {                                                                                                               \
+        const hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+            throw DeviceError(static_cast<int64_t>(e_), std::string(what) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+static size_t Align(size_t n, size_t a) { return (n + a - 1) / a * a; }
+
+struct Query {
+    const int64_t *offsets;
+    const int32_t *terms;
+    int32_t count;
+    const int32_t *k;   // per query
+    const float *thr;   // per query or nullptr
+};
+
+class Session {
+public:
+    bm25mi_config cfg{};
+    int64_t ndocs = 0, vocab = 0, postings = 0;
+    int32_t blockDocs = 8192, granule = 1024, granulesPerBlock = 8, blocks = 0, grid = 0, cus = 0;
+    int64_t poolPerQuery = 0;
+    std::vector<int64_t> toffHost;
+    std::vector<float> idfHost, tmaxHost, tmeanHost;
+    std::vector<int32_t> slotHost;
+    std::vector<float> bmaxHost;
+    int64_t granules = 0;
+
+    // Device memory.
+    void *dPost = nullptr, *dToff = nullptr, *dSlot = nullptr, *dDir = nullptr, *dBmax = nullptr, *dTmax = nullptr;
+    void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
+    // Pinned host memory: request packet (copied by the graph) and result block (written by the device).
+    void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
+    size_t packetBytes = 0, resultBytes = 0;
+    size_t offQoff = 0, offK = 0, offThr = 0, offPool = 0, offOut = 0, offTerms = 0;
+    size_t resStatus = 0, resScores = 0, resIds = 0;
+
+    IndexView ix{};
+    BatchView bt{};
+    WorkView wk{};
+
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int device = 0;
+
+    bool failed = false, closed = false;
+    ErrorInfo error;
+    std::mutex mutex;
+    bm25mi_stats last{};
+
+    Session(const int64_t *offsets, int64_t v, const int32_t *docs, const float *tf, int64_t m, const float *norm,
+            int64_t n, const float *idf, const bm25mi_config &config);
+    ~Session() { Release(); }
+
+    void Release() noexcept;
+    void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
+    void Run(bool timed);
+    void Bootstrap();
+
+private:
+    void Capture();
+    void Enqueue(bool timed);
+    template <class T> T *Alloc(size_t count, const char *what)
+    {
+        void *p = nullptr;
+        BM25MI_HIP(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)), what);
+        return static_cast<T *>(p);
+    }
+};
+
+Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const float *tf, int64_t m,
+                 const float *norm, int64_t n, const float *idf, const bm25mi_config &config)
+    : cfg(config), ndocs(n), vocab(v), postings(m)
+{
+    Require(cfg.max_queries >= 1 && cfg.max_queries <= BM25MI_MAX_QUERIES, "max_queries must be in [1, 10000]");
+    Require(cfg.max_words >= 0 && cfg.max_words <= 16777216, "max_words must be in [0, 2^24]");
+    Require(cfg.max_k >= 1 && cfg.max_k <= BM25MI_MAX_K, "max_k must be in [1, 1024]");
+    Require(std::isfinite(cfg.k1) && cfg.k1 > 0, "k1 must be finite and > 0");
+    blockDocs = cfg.block_docs ? cfg.block_docs : 8192;
+    Require(blockDocs == 2048 || blockDocs == 4096 || blockDocs == 8192, "block_docs must be 2048, 4096 or 8192");
+    ValidateIndex(offsets, v, docs, tf, m, norm, n, idf, 0);
+    Require(m >= 1, "Index needs at least one posting");
+    Require(cfg.max_k <= n, "max_k exceeds the document count");
+    device = cfg.device_id;
+
+    // Host-side tables: impacts, directory, block maxima.
+    granule = std::min(1024, blockDocs);
+    granulesPerBlock = blockDocs / granule;
+    blocks = static_cast<int32_t>((n + blockDocs - 1) / blockDocs);
+    Layout layout;
+    std::vector<float> impact(static_cast<size_t>(m));
+    ComputeImpacts(offsets, v, docs, tf, norm, idf, cfg.k1, impact.data(), 0);
+    BuildLayout(offsets, v, docs, impact.data(), n, granule, kLongMin, 0, layout);
+    granules = layout.granules;
+    toffHost.assign(offsets, offsets + v + 1);
+    idfHost.assign(idf, idf + v);
+    tmaxHost = layout.termMax;
+    tmeanHost = layout.termMean;
+    slotHost = layout.termSlot;
+
+    int count = 0;
+    BM25MI_HIP(hipGetDeviceCount(&count), "count devices");
+    Require(device >= 0 && device < count, "device_id out of range");
+    BM25MI_HIP(hipSetDevice(device), "set device");
+    hipDeviceProp_t prop;
+    BM25MI_HIP(hipGetDeviceProperties(&prop, device), "device properties");
+    cus = prop.multiProcessorCount;
+    const int ldsPerWg = blockDocs * 4 + 1024;
+    const int perCu = std::max(1, std::min(8, static_cast<int>(prop.maxSharedMemoryPerMultiProcessor) / ldsPerWg));
+    grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
+
+    size_t freeBytes = 0, totalBytes = 0;
+    BM25MI_HIP(hipMemGetInfo(&freeBytes, &totalBytes), "memory info");
+    poolPerQuery = std::min<int64_t>(static_cast<int64_t>(blocks) * cfg.max_k, n);
+    const uint64_t known = static_cast<uint64_t>(m) * 8 + (v + 1) * 8 + v * 8 +
+                           layout.dir.size() * 4 + layout.blockMax.size() * 4 +
+                           static_cast<uint64_t>(cfg.max_queries) * poolPerQuery * 8;
+    // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
+    Require(known <= static_cast<uint64_t>(freeBytes) / 5 * 4, "Index, pools and arenas exceed 80% of free HBM");
+
+    BM25MI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create stream");
+    for (auto &e : ev)
+        BM25MI_HIP(hipEventCreate(&e), "create event");
+
+    // Postings as {doc, impact bits}: one 8-byte load per posting.
+    {
+        std::vector<uint2> packed(static_cast<size_t>(m));
+        ParallelChunks(m, Threads(0), [&](int, int64_t b, int64_t e) {
+            for (int64_t p = b; p < e; ++p) {
+                uint32_t bitsv;
+                std::memcpy(&bitsv, &impact[p], 4);
+                packed[p] = make_uint2(static_cast<uint32_t>(docs[p]), bitsv);
+            }
+        });
+        std::vector<float>().swap(impact);
+        dPost = Alloc<uint2>(m, "alloc postings");
+        BM25MI_HIP(hipMemcpy(dPost, packed.data(), m * sizeof(uint2), hipMemcpyHostToDevice), "upload postings");
+    }
+    dToff = Alloc<int64_t>(v + 1, "alloc term offsets");
+    BM25MI_HIP(hipMemcpy(dToff, offsets, (v + 1) * 8, hipMemcpyHostToDevice), "upload term offsets");
+    dSlot = Alloc<int32_t>(v, "alloc slots");
+    BM25MI_HIP(hipMemcpy(dSlot, layout.termSlot.data(), v * 4, hipMemcpyHostToDevice), "upload slots");
+    dTmax = Alloc<float>(v, "alloc term max");
+    BM25MI_HIP(hipMemcpy(dTmax, layout.termMax.data(), v * 4, hipMemcpyHostToDevice), "upload term max");
+    dDir = Alloc<uint32_t>(layout.dir.size(), "alloc directory");
+    if (!layout.dir.empty())
+        BM25MI_HIP(hipMemcpy(dDir, layout.dir.data(), layout.dir.size() * 4, hipMemcpyHostToDevice), "upload dir");
+    dBmax = Alloc<float>(layout.blockMax.size(), "alloc block max");
+    if (!layout.blockMax.empty())
+        BM25MI_HIP(hipMemcpy(dBmax, layout.blockMax.data(), layout.blockMax.size() * 4, hipMemcpyHostToDevice),
+                   "upload block max");
+    bmaxHost.swap(layout.blockMax);
+
+    // Request packet.
+    const size_t q = static_cast<size_t>(cfg.max_queries);
+    size_t at = 16;
+    offQoff = at;
+    at = Align(at + (q + 1) * 8, 8);
+    offK = at;
+    at = Align(at + q * 4, 8);
+    offThr = at;
+    at = Align(at + q * 4, 8);
+    offPool = at;
+    at = Align(at + (q + 1) * 8, 8);
+    offOut = at;
+    at = Align(at + (q + 1) * 8, 8);
+    offTerms = at;
+    at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 4, 256);
+    packetBytes = at;
+    BM25MI_HIP(hipHostMalloc(&hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
+    std::memset(hPacket, 0, packetBytes);
+    dPacket = Alloc<unsigned char>(packetBytes, "alloc device packet");
+
+    // Result block in mapped pinned memory.
+    resStatus = kStats * 8;
+    resScores = Align(resStatus + q * 4, 16);
+    resIds = Align(resScores + q * cfg.max_k * 4, 16);
+    resultBytes = Align(resIds + q * cfg.max_k * 4, 256);
+    BM25MI_HIP(hipHostMalloc(&hResult, resultBytes, hipHostMallocMapped), "alloc pinned result");
+    std::memset(hResult, 0, resultBytes);
+    BM25MI_HIP(hipHostGetDevicePointer(&hResultDev, hResult, 0), "map pinned result");
+
+    dPool = Alloc<unsigned long long>(q * poolPerQuery, "alloc candidate pools");
+    dCount = Alloc<uint32_t>(q, "alloc pool counts");
+    dTau = Alloc<uint32_t>(q, "alloc thresholds");
+    dStats = Alloc<unsigned long long>(kStats, "alloc stats");
+
+    ix = IndexView{static_cast<const uint2 *>(dPost),  static_cast<const int64_t *>(dToff),
+                   static_cast<const int32_t *>(dSlot), static_cast<const uint32_t *>(dDir),
+                   static_cast<const float *>(dBmax),  static_cast<const float *>(dTmax),
+                   v, granules, static_cast<int32_t>(n), __builtin_ctz(static_cast<unsigned>(granule))};
+    auto *pk = static_cast<unsigned char *>(dPacket);
+    bt = BatchView{reinterpret_cast<const int32_t *>(pk),           reinterpret_cast<const int64_t *>(pk + offQoff),
+                   reinterpret_cast<const int32_t *>(pk + offK),    reinterpret_cast<const float *>(pk + offThr),
+                   reinterpret_cast<const int64_t *>(pk + offPool), reinterpret_cast<const int64_t *>(pk + offOut),
+                   reinterpret_cast<const int32_t *>(pk + offTerms)};
+    auto *rs = static_cast<unsigned char *>(hResultDev);
+    wk = WorkView{static_cast<unsigned long long *>(dPool), static_cast<uint32_t *>(dCount),
+                  static_cast<uint32_t *>(dTau),            static_cast<unsigned long long *>(dStats),
+                  reinterpret_cast<int32_t *>(rs + resStatus), reinterpret_cast<float *>(rs + resScores),
+                  reinterpret_cast<int32_t *>(rs + resIds)};
+    if (cfg.use_graph)
+        Capture();
+    Bootstrap();
+}
+
+void Session::Enqueue(bool timed)
+{
+    BM25MI_HIP(hipMemcpyAsync(dPacket, hPacket, packetBytes, hipMemcpyHostToDevice, stream), "packet H2D");
+    LaunchReset(bt, wk, cfg.max_queries, stream);
+    if (timed)
+        BM25MI_HIP(hipEventRecord(ev[1], stream), "record score start");
+    Require(LaunchScore(blockDocs, ix, bt, wk, blocks, granulesPerBlock, grid, stream), "Unsupported block size");
+    if (timed)
+        BM25MI_HIP(hipEventRecord(ev[2], stream), "record score end");
+    LaunchFinal(bt, wk, cfg.max_queries, static_cast<int32_t>(ndocs), stream);
+    LaunchPublish(wk, static_cast<unsigned long long *>(hResultDev), stream);
+}
+
+void Session::Capture()
+{
+    BM25MI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    try {
+        Enqueue(false);
+    } catch (...) {
+        hipGraph_t discarded = nullptr;
+        (void)hipStreamEndCapture(stream, &discarded);
+        if (discarded)
+            (void)hipGraphDestroy(discarded);
+        throw;
+    }
+    BM25MI_HIP(hipStreamEndCapture(stream, &graph), "end graph capture");
+    BM25MI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate graph");
+}
+
+void Session::Run(bool timed)
+{
+    BM25MI_HIP(hipEventRecord(ev[0], stream), "record start");
+    if (exec && !timed) {
+        BM25MI_HIP(hipGraphLaunch(exec, stream), "replay Search graph");
+    } else {
+        Enqueue(timed);
+    }
+    BM25MI_HIP(hipEventRecord(ev[3], stream), "record end");
+    BM25MI_HIP(hipStreamSynchronize(stream), "complete Search");
+    float ms = 0;
+    BM25MI_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]), "device time");
+    last.device_ms = ms;
+    if (timed) {
+        BM25MI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]), "score kernel time");
+        last.score_kernel_ms = ms;
+    } else {
+        last.score_kernel_ms = -1;
+    }
+}
+
+void Session::Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed)
+{
+    // Validate the complete request before modifying the reusable packet.
+    const int32_t nq = query.count;
+    Require(nq >= 1 && nq <= cfg.max_queries, "Query batch exceeds declared capacity or is empty");
+    Require(query.offsets && query.k && outScores && outIds, "Null query or output array");
+    Require(query.offsets[0] == 0, "query_offsets[0] must be 0");
+    int64_t words = 0;
+    for (int32_t q = 0; q < nq; ++q) {
+        const int64_t len = query.offsets[q + 1] - query.offsets[q];
+        Require(len >= 0 && len <= kMaxTerms && len <= cfg.max_words - words,
+                "Query exceeds word capacity or the 64-term limit");
+        words += len;
+        Require(query.k[q] >= 1 && query.k[q] <= cfg.max_k && query.k[q] <= ndocs, "K out of range");
+        if (query.thr)
+            Require(std::isfinite(query.thr[q]) && query.thr[q] >= 0, "Threshold must be finite and >= 0");
+    }
+    Require(words == 0 || query.terms, "Null query terms");
+    for (int64_t w = 0; w < words; ++w)
+        Require(query.terms[w] >= 0 && query.terms[w] < INT32_MAX, "Invalid public word ID");
+
+    auto *pk = static_cast<unsigned char *>(hPacket);
+    reinterpret_cast<int32_t *>(pk)[0] = nq;
+    auto *qoff = reinterpret_cast<int64_t *>(pk + offQoff);
+    auto *kk = reinterpret_cast<int32_t *>(pk + offK);
+    auto *thr = reinterpret_cast<float *>(pk + offThr);
+    auto *pool = reinterpret_cast<int64_t *>(pk + offPool);
+    auto *out = reinterpret_cast<int64_t *>(pk + offOut);
+    qoff[0] = pool[0] = out[0] = 0;
+    int64_t total = 0;
+    for (int32_t q = 0; q < nq; ++q) {
+        qoff[q + 1] = query.offsets[q + 1];
+        kk[q] = query.k[q];
+        thr[q] = query.thr ? query.thr[q] : 0.0F;
+        pool[q + 1] = pool[q] + std::min<int64_t>(static_cast<int64_t>(blocks) * query.k[q], ndocs);
+        out[q + 1] = out[q] + query.k[q];
+        for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w) {
+            const int32_t t = query.terms[w];
+            if (t < vocab)
+                total += toffHost[t + 1] - toffHost[t];
+        }
+    }
+    if (words)
+        std::memcpy(pk + offTerms, query.terms, words * 4);
+
+    try {
+        Run(timed);
+        auto *res = static_cast<unsigned char *>(hResult);
+        const auto *stats = reinterpret_cast<const unsigned long long *>(res);
+        const auto *st = reinterpret_cast<const int32_t *>(res + resStatus);
+        const auto *sc = reinterpret_cast<const float *>(res + resScores);
+        const auto *id = reinterpret_cast<const int32_t *>(res + resIds);
+        if (stats[kStatOverflow])
+            throw DeviceError(0, "Candidate pool overflow");
+        int32_t fallbacks = 0;
+        for (int32_t q = 0; q < nq; ++q) {
+            status[q] = st[q];
+            fallbacks += st[q] ? 1 : 0;
+            if (st[q])
+                continue;
+            // Receive checks of large_search.cpp:64-85.
+            for (int64_t r = out[q]; r < out[q + 1]; ++r) {
+                if (!(id[r] >= 0 && id[r] < ndocs && std::isfinite(sc[r]) && sc[r] >= 0))
+                    throw DeviceError(0, "Invalid real-document result");
+                if (r > out[q] &&
+                    !(sc[r - 1] > sc[r] || (sc[r - 1] == sc[r] && id[r - 1] < id[r])))
+                    throw DeviceError(0, "Strict global ranking failure");
+            }
+            std::memcpy(outScores + out[q], sc + out[q], (out[q + 1] - out[q]) * 4);
+            std::memcpy(outIds + out[q], id + out[q], (out[q + 1] - out[q]) * 4);
+        }
+        last.postings_scanned = static_cast<int64_t>(stats[kStatPostings]);
+        last.postings_total = total;
+        last.blocks_scored = static_cast<int64_t>(stats[kStatScored]);
+        last.blocks_skipped = static_cast<int64_t>(stats[kStatSkipped]);
+        last.candidates = static_cast<int64_t>(stats[kStatCandidates]);
+        last.fallbacks = fallbacks;
+        last.queries = nq;
+    } catch (...) {
+        failed = true;
+        throw;
+    }
+}
+
+void Session::Bootstrap()
+{
+    const int32_t q = cfg.max_queries, k = cfg.max_k;
+    std::vector<int64_t> offsets(q + 1, 0);
+    std::vector<int32_t> ks(q, k), status(q);
+    std::vector<float> scores(static_cast<size_t>(q) * k);
+    std::vector<int32_t> ids(scores.size());
+    Query query{offsets.data(), nullptr, q, ks.data(), nullptr};
+    Search(query, scores.data(), ids.data(), status.data(), false);
+    for (size_t i = 0; i < ids.size(); ++i)
+        if (!(scores[i] == 0 && ids[i] == static_cast<int32_t>(i % k)))
+            throw DeviceError(0, "Empty bootstrap Search failed");
+}
+
+void Session::Release() noexcept
+{
+    if (exec)
+        (void)hipGraphExecDestroy(exec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    for (auto &e : ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    for (void **p : {&dPost, &dToff, &dSlot, &dDir, &dBmax, &dTmax, &dPacket, &dPool, &dCount, &dTau, &dStats})
+        if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    for (void **p : {&hPacket, &hResult})
+        if (*p) {
+            (void)hipHostFree(*p);
+            *p = nullptr;
+        }
+}
+
+} // namespace bm25mi
+
+using namespace bm25mi;
+
+struct bm25mi_session {
+    std::unique_ptr<Session> impl;
+    ErrorInfo error;
+    std::mutex mutex;
+};
+
+extern "C" void bm25mi_config_default(bm25mi_config *cfg)
+{
+    if (!cfg)
+        return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->device_id = 0;
+    cfg->max_queries = 8;
+    cfg->max_words = 24;
+    cfg->max_k = 10;
+    cfg->block_docs = 8192;
+    cfg->k1 = 1.2;
+    cfg->use_graph = 1;
+    cfg->grid_blocks = 0;
+}
+
+extern "C" int bm25mi_create(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,
+                             int64_t postings, const float *doc_norm, int64_t docs, const float *idf,
+                             const bm25mi_config *cfg, bm25mi_session **out)
+{
+    return Invoke([&] {
+        Require(out != nullptr, "Null session output");
+        *out = nullptr;
+        bm25mi_config c;
+        bm25mi_config_default(&c);
+        if (cfg)
+            c = *cfg;
+        auto s = std::make_unique<bm25mi_session>();
+        s->impl = std::make_unique<Session>(term_offsets, vocab, doc_ids, tf, postings, doc_norm, docs, idf, c);
+        *out = s.release();
+    });
+}
+
+static int SearchCommon(bm25mi_session *s, const Query &query, float *scores, int32_t *ids, int32_t *fallback,
+                        bool pruned, bool timed)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            const auto start = std::chrono::steady_clock::now();
+            if (!s->impl || s->impl->closed || s->impl->failed)
+                throw InvalidState("BM25 session closed/failed");
+            Session &ss = *s->impl;
+            std::vector<int32_t> status(std::max(0, query.count));
+            ss.Search(query, scores, ids, status.data(), timed);
+            int32_t reruns = 0;
+            if (pruned) {
+                // Plain-path rerun of every query that kept fewer than K docs above its threshold.
+                std::vector<int32_t> which;
+                for (int32_t q = 0; q < query.count; ++q) {
+                    if (fallback)
+                        fallback[q] = status[q];
+                    if (status[q])
+                        which.push_back(q);
+                }
+                reruns = static_cast<int32_t>(which.size());
+                if (!which.empty()) {
+                    const bm25mi_stats first = ss.last;
+                    std::vector<int64_t> offs(1, 0);
+                    std::vector<int32_t> terms, ks, st(which.size());
+                    std::vector<int64_t> rowOf;
+                    int64_t rows = 0;
+                    std::vector<int64_t> prefix(query.count + 1, 0);
+                    for (int32_t q = 0; q < query.count; ++q)
+                        prefix[q + 1] = prefix[q] + query.k[q];
+                    for (int32_t q : which) {
+                        terms.insert(terms.end(), query.terms + query.offsets[q], query.terms + query.offsets[q + 1]);
+                        offs.push_back(static_cast<int64_t>(terms.size()));
+                        ks.push_back(query.k[q]);
+                        rowOf.push_back(rows);
+                        rows += query.k[q];
+                    }
+                    std::vector<float> sc(rows);
+                    std::vector<int32_t> id(rows);
+                    Query sub{offs.data(), terms.data(), static_cast<int32_t>(which.size()), ks.data(), nullptr};
+                    ss.Search(sub, sc.data(), id.data(), st.data(), timed);
+                    for (size_t i = 0; i < which.size(); ++i) {
+                        const int32_t q = which[i];
+                        std::memcpy(scores + prefix[q], sc.data() + rowOf[i], query.k[q] * 4);
+                        std::memcpy(ids + prefix[q], id.data() + rowOf[i], query.k[q] * 4);
+                    }
+                    ss.last.device_ms += first.device_ms;
+                    if (first.score_kernel_ms >= 0 && ss.last.score_kernel_ms >= 0)
+                        ss.last.score_kernel_ms += first.score_kernel_ms;
+                    ss.last.postings_scanned += first.postings_scanned;
+                    ss.last.postings_total = first.postings_total;
+                    ss.last.blocks_scored += first.blocks_scored;
+                    ss.last.blocks_skipped += first.blocks_skipped;
+                    ss.last.candidates += first.candidates;
+                    ss.last.queries = query.count;
+                }
+            } else if (fallback) {
+                for (int32_t q = 0; q < query.count; ++q)
+                    fallback[q] = 0;
+            }
+            ss.last.fallbacks = reruns;
+            ss.last.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+        },
+        &s->error);
+}
+
+extern "C" int bm25mi_search(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                             int32_t queries, int32_t k, float *out_scores, int32_t *out_ids)
+{
+    std::vector<int32_t> ks(queries > 0 ? queries : 0, k);
+    Query q{query_offsets, query_terms, queries, ks.data(), nullptr};
+    return SearchCommon(s, q, out_scores, out_ids, nullptr, false, false);
+}
+
+extern "C" int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                   int32_t queries, const int32_t *k_per_query, float *out_scores, int32_t *out_ids)
+{
+    Query q{query_offsets, query_terms, queries, k_per_query, nullptr};
+    return SearchCommon(s, q, out_scores, out_ids, nullptr, false, false);
+}
+
+extern "C" int bm25mi_search_pruned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                    int32_t queries, const int32_t *k_per_query, const float *thresholds,
+                                    float *out_scores, int32_t *out_ids, int32_t *fallback_out)
+{
+    Query q{query_offsets, query_terms, queries, k_per_query, thresholds};
+    return SearchCommon(s, q, out_scores, out_ids, fallback_out, true, false);
+}
+
+// Same as bm25mi_search_batch / bm25mi_search_pruned, but runs the kernels without the
+// graph and with HIP events around the scoring kernel (bm25mi_stats.score_kernel_ms).
+extern "C" int bm25mi_search_timed(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                   int32_t queries, const int32_t *k_per_query, const float *thresholds,
+                                   float *out_scores, int32_t *out_ids, int32_t *fallback_out)
+{
+    Query q{query_offsets, query_terms, queries, k_per_query, thresholds};
+    return SearchCommon(s, q, out_scores, out_ids, fallback_out, thresholds != nullptr, true);
+}
+
+// Per-query features of the learned K-th-score model. Every feature depends only on
+// index statistics of the query's terms (IDs >= vocab ignored, repeats counted).
+extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                     int32_t queries, const int32_t *k_per_query, float *out_features)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            Require(s->impl && !s->impl->closed, "Session closed");
+            Require(query_offsets && k_per_query && out_features && queries >= 1, "Null feature arguments");
+            const Session &ss = *s->impl;
+            const double n = static_cast<double>(ss.ndocs);
+            for (int32_t q = 0; q < queries; ++q) {
+                std::vector<double> m;
+                double sumMean = 0, sumIdf = 0, maxIdf = 0, sumDf = 0, maxDf = 0, minDf = -1, density = 0,
+                       logAll = 0;
+                for (int64_t w = query_offsets[q]; w < query_offsets[q + 1]; ++w) {
+                    const int32_t t = query_terms[w];
+                    Require(t >= 0, "Invalid public word ID");
+                    if (t >= ss.vocab)
+                        continue;
+                    const double df = static_cast<double>(ss.toffHost[t + 1] - ss.toffHost[t]);
+                    m.push_back(ss.tmaxHost[t]);
+                    sumMean += ss.tmeanHost[t];
+                    sumIdf += ss.idfHost[t];
+                    maxIdf = std::max(maxIdf, static_cast<double>(ss.idfHost[t]));
+                    sumDf += df;
+                    maxDf = std::max(maxDf, df);
+                    minDf = minDf < 0 ? df : std::min(minDf, df);
+                    density += ss.tmaxHost[t] * df / n;
+                    logAll += std::log(std::max(df, 0.5) / n);
+                }
+                std::sort(m.begin(), m.end(), std::greater<double>());
+                double ub = 0, rest = 0;
+                for (size_t i = 0; i < m.size(); ++i) {
+                    ub += m[i];
+                    if (i >= 3)
+                        rest += m[i];
+                }
+                float *f = out_features + static_cast<int64_t>(q) * BM25MI_FEATURES;
+                const double vals[BM25MI_FEATURES] = {
+                    std::log(static_cast<double>(k_per_query[q])), static_cast<double>(m.size()), ub,
+                    m.size() > 0 ? m[0] : 0.0, m.size() > 1 ? m[1] : 0.0, m.size() > 2 ? m[2] : 0.0, rest, sumMean,
+                    sumIdf, maxIdf, std::log1p(sumDf), std::log1p(maxDf), std::log1p(std::max(minDf, 0.0)), density,
+                    std::log(n), std::max(-40.0, logAll)};
+                for (int i = 0; i < BM25MI_FEATURES; ++i)
+                    f[i] = static_cast<float>(vals[i]);
+            }
+        },
+        &s->error);
+}
+
+extern "C" int bm25mi_last_stats(const bm25mi_session *s, bm25mi_stats *out)
+{
+    return Invoke([&] {
+        Require(s && out, "Null session or stats output");
+        auto *ms = const_cast<bm25mi_session *>(s);
+        std::lock_guard<std::mutex> guard(ms->mutex);
+        Require(ms->impl != nullptr, "Session closed");
+        *out = ms->impl->last;
+    });
+}
+
+extern "C" int bm25mi_destroy(bm25mi_session *s)
+{
+    if (!s)
+        return BM25MI_OK; // Close is idempotent
+    int rc;
+    {
+        std::lock_guard<std::mutex> guard(s->mutex);
+        rc = Invoke(
+            [&] {
+                if (s->impl) {
+                    s->impl->closed = true;
+                    BM25MI_HIP(hipSetDevice(s->impl->device), "set device for close");
+                    s->impl.reset();
+                }
+            },
+            &s->error);
+    }
+    if (rc == BM25MI_OK)
+        delete s;
+    return rc;
+}
+
+extern "C" int64_t bm25mi_last_error(const bm25mi_session *s, char *buf, size_t size)
+{
+    const ErrorInfo &info = s ? s->error : ThreadError();
+    if (buf && size) {
+        std::strncpy(buf, info.message, size - 1);
+        buf[size - 1] = '\0';
+    }
+    return info.code;
+}
+
+extern "C" int bm25mi_device_info(int32_t device_id, int32_t *cus, int32_t *lds_per_block, int32_t *lds_per_cu)
+{
+    return Invoke([&] {
+        int count = 0;
+        BM25MI_HIP(hipGetDeviceCount(&count), "count devices");
+        Require(device_id >= 0 && device_id < count, "device_id out of range");
+        hipDeviceProp_t prop;
+        BM25MI_HIP(hipGetDeviceProperties(&prop, device_id), "device properties");
+        if (cus)
+            *cus = prop.multiProcessorCount;
+        if (lds_per_block)
+            *lds_per_block = static_cast<int32_t>(prop.sharedMemPerBlock);
+        if (lds_per_cu)
+            *lds_per_cu = static_cast<int32_t>(prop.maxSharedMemoryPerMultiProcessor);
+    });
+}

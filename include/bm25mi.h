@@ -1,0 +1,206 @@
+/*
+ * bm25mi.h - C ABI of the MI355X (gfx950) BM25 full-Search library.
+ *
+ * One shared library, libbm25mi.so, keeps a BM25 index resident in HBM and
+ * answers Search(query term IDs, K) -> (doc IDs, FP32 scores) under strict
+ * ranking: score descending, then doc ID ascending. Every signature uses plain
+ * pointers and sizes; no framework types cross this boundary.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout,
+ * FR = feature_retrieval/src/ascendfaiss):
+ *   bm25mi_create   <- faiss::ascend::AscendIndexBm25::Init
+ *                      (FR/ascendhost/include/index/AscendIndexBm25.h:47,
+ *                       FR/ascendhost/src/bm25/AscendIndexBm25.cpp:33-48) and the
+ *                      bm25large::Session constructor it runs
+ *                      (FR/ascendhost/src/bm25/large_search.cpp:107-188)
+ *   bm25mi_search   <- AscendIndexBm25::Search (AscendIndexBm25.h:49,
+ *                      AscendIndexBm25.cpp:50-65) -> Session::Search
+ *                      (large_search.cpp:190-212); the two aclnn operators it
+ *                      replays, aclnnBm25FusedScoreTopK and aclnnBm25TopKMerge
+ *                      (FR/ascendhost/src/bm25/fused_runtime.h:28-39)
+ *   bm25mi_destroy  <- AscendIndexBm25::Close (AscendIndexBm25.h:51,
+ *                      AscendIndexBm25.cpp:67-78)
+ *   bm25mi_last_error <- AscendIndexBm25::GetLastError (AscendIndexBm25.h:52;
+ *                      ErrorInfo in FR/common/bm25/error_info.h:8-20)
+ *
+ * Error convention: every call returns a status that mirrors the four classes
+ * of FR/ascendhost/include/impl/Bm25ErrorAdapter.h:11-45. Input errors leave a
+ * session usable; a device error makes the session fail for good and later
+ * searches return BM25MI_ERR_ILLEGAL_OPERATION (large_search.cpp:193-195,208-210).
+ */
+#ifndef BM25MI_H
+#define BM25MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BM25MI_OK = 0,
+    BM25MI_ERR_INVALID_PARAM = 1,     /* InvalidArgument -> APP_ERR_INVALID_PARAM */
+    BM25MI_ERR_ILLEGAL_OPERATION = 2, /* InvalidState    -> APP_ERR_ILLEGAL_OPERATION */
+    BM25MI_ERR_INNER = 3              /* DeviceError/other -> APP_ERR_INNER_ERROR */
+};
+
+/* Hard limits of this build. */
+#define BM25MI_MAX_K 1024            /* reference: 32 (FR/common/bm25/large_search_plan.h:34) */
+#define BM25MI_MAX_TERMS_PER_QUERY 64 /* FR/common/bm25/fixed_query_packet.h:32 */
+#define BM25MI_MAX_QUERIES 10000      /* fixed_query_packet.h:28 */
+
+typedef struct bm25mi_session bm25mi_session;
+
+/* Bm25Config (AscendIndexBm25.h:17-25) plus the index-time constants the
+ * reference hardcodes (k1 = 1.2 in large_corpus.py:174 and 2.2F = k1 + 1 in
+ * cpu_index_search.h:185). */
+typedef struct {
+    int32_t device_id;       /* deviceId */
+    int32_t max_queries;     /* queryCount: largest batch one Search accepts */
+    int64_t max_words;       /* maxQueryWords: total term IDs per batch */
+    int32_t max_k;           /* topK upper bound; each Search names its own K <= max_k */
+    int32_t block_docs;      /* docs per scoring block: 2048, 4096, 8192 or 16384; 0 = 8192 */
+    double k1;               /* BM25 k1; impact = idf*(tf*(k1+1)) / (tf+norm) in FP32 */
+    int32_t use_graph;       /* 1: replay one captured hipGraph per Search (default) */
+    int32_t grid_blocks;     /* persistent scoring workgroups; 0 = 4 per CU */
+} bm25mi_config;
+
+/* Per-Search counters, filled by bm25mi_last_stats. */
+typedef struct {
+    double device_ms;          /* HIP-event time of the device work of the last Search */
+    double score_kernel_ms;    /* HIP-event time of the scoring kernel alone */
+    double host_ms;            /* wall time of the whole bm25mi_search* call */
+    int64_t postings_scanned;  /* postings streamed from HBM */
+    int64_t postings_total;    /* postings of the query terms (plain-path work) */
+    int64_t blocks_scored;     /* (query, block) tasks scored */
+    int64_t blocks_skipped;    /* (query, block) tasks skipped by a threshold */
+    int64_t candidates;        /* entries written to the candidate pools */
+    int32_t fallbacks;         /* queries rerun on the plain path */
+    int32_t queries;
+} bm25mi_stats;
+
+void bm25mi_config_default(bm25mi_config *cfg);
+
+/* Copies, validates (FR/common/bm25/index_data.h:13-41) and uploads the index,
+ * precomputes FP32 impacts, builds the block directory and block-max table,
+ * and captures the Search graph. Arrays follow the reference index files
+ * (standalone/resident_index.py:12-18): term_offsets int64[vocab+1],
+ * doc_ids int32[postings], tf float[postings], doc_norm float[docs],
+ * idf float[vocab]. */
+int bm25mi_create(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,
+                  int64_t postings, const float *doc_norm, int64_t docs, const float *idf,
+                  const bm25mi_config *cfg, bm25mi_session **out);
+
+/* Plain full Search. query_offsets int64[queries+1] (offsets[0] == 0), query_terms
+ * int32[offsets[queries]] in query order (duplicates kept, IDs >= vocab skipped,
+ * negative IDs rejected). Writes out_scores float[queries*k], out_ids
+ * int32[queries*k], row-major, strict ranking. */
+int bm25mi_search(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms, int32_t queries,
+                  int32_t k, float *out_scores, int32_t *out_ids);
+
+/* Uneven batch: per-query K (k_per_query[q] in [1, max_k]). Results of query q
+ * start at row offset sum_{p<q} k_per_query[p]. */
+int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                        int32_t queries, const int32_t *k_per_query, float *out_scores, int32_t *out_ids);
+
+/* Threshold-pruned Search (no reference counterpart: the learned path).
+ * thresholds[q] is a predicted K-th score. Blocks whose block-max upper bound
+ * is below it are skipped; only docs scoring at least the threshold are kept.
+ * A query with fewer than K survivors is rerun on the plain path, so results
+ * always equal bm25mi_search_batch. fallback_out[q] (optional) is set to 1
+ * for each rerun query. */
+int bm25mi_search_pruned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                         int32_t queries, const int32_t *k_per_query, const float *thresholds,
+                         float *out_scores, int32_t *out_ids, int32_t *fallback_out);
+
+/* Same as bm25mi_search_batch (thresholds == NULL) or bm25mi_search_pruned, but
+ * launches the kernels directly instead of replaying the graph, with HIP events
+ * around the scoring kernel (bm25mi_stats.score_kernel_ms). For measurement. */
+int bm25mi_search_timed(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                        int32_t queries, const int32_t *k_per_query, const float *thresholds, float *out_scores,
+                        int32_t *out_ids, int32_t *fallback_out);
+
+/* Query features for the learned threshold (no reference counterpart):
+ * writes float[queries * BM25MI_FEATURES]. */
+#define BM25MI_FEATURES 16
+int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                          int32_t queries, const int32_t *k_per_query, float *out_features);
+
+int bm25mi_last_stats(const bm25mi_session *s, bm25mi_stats *out);
+
+/* Close: idempotent; frees device memory. The handle is freed on success. */
+int bm25mi_destroy(bm25mi_session *s);
+
+/* Copies the last error message of s (or of the calling thread when s is NULL)
+ * into buf (truncated to size-1 chars, NUL-terminated) and returns the raw
+ * HIP error code of a device failure, else 0 (ErrorInfo::aclCode). */
+int64_t bm25mi_last_error(const bm25mi_session *s, char *buf, size_t size);
+
+/* Device properties of the session's GPU: compute units, LDS bytes per block. */
+int bm25mi_device_info(int32_t device_id, int32_t *cus, int32_t *lds_per_block, int32_t *lds_per_cu);
+
+/* ---------------------------------------------------------------------------
+ * Index builder (host C++, no GPU). The reference's Python builder is
+ * standalone/large_corpus.py:95-297; the synthetic generator here follows the
+ * seeded Zipf/lognormal corpus spec of this project instead of the
+ * reference's uniform vocab-1024 stream.
+ * ------------------------------------------------------------------------- */
+typedef struct bm25mi_corpus bm25mi_corpus;
+
+typedef struct {
+    int64_t num_docs;
+    int64_t vocab;
+    double zipf_s;     /* token rank r in [1, vocab] drawn with P(r) ~ r^-s; term ID = r-1 */
+    double mean_len;   /* lognormal doc length mean (tokens) */
+    double sigma;      /* lognormal sigma */
+    uint64_t seed;
+    int32_t threads;   /* 0 = hardware concurrency; output never depends on it */
+} bm25mi_corpus_spec;
+
+int bm25mi_corpus_generate(const bm25mi_corpus_spec *spec, bm25mi_corpus **out);
+/* Borrowed views, valid until bm25mi_corpus_free: term_offsets int64[vocab+1],
+ * doc_ids int32[postings], tf float[postings], doc_len int32[num_docs]. */
+int bm25mi_corpus_view(const bm25mi_corpus *c, int64_t *num_docs, int64_t *vocab, int64_t *postings,
+                       const int64_t **term_offsets, const int32_t **doc_ids, const float **tf,
+                       const int32_t **doc_len);
+void bm25mi_corpus_free(bm25mi_corpus *c);
+
+/* idf = f32(log1p((N-df+0.5)/(df+0.5))) and norm = f32(k1*(1-b+b*len/avgdl)), both
+ * evaluated in double (standalone/large_corpus.py:148,174). */
+int bm25mi_bm25_weights(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_len, int64_t docs,
+                        double k1, double b, float *out_idf, float *out_norm, int32_t threads);
+
+/* impact[p] = f32(f32(idf[t] * f32(tf[p] * f32(k1+1))) / f32(tf[p] + norm[doc[p]]))
+ * (FR/test/ascendhost/bm25/reference/cpu_index_search.h:185-186). */
+int bm25mi_impacts(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,
+                   const float *doc_norm, const float *idf, double k1, float *out_impact, int32_t threads);
+
+/* Index validation of FR/common/bm25/index_data.h:13-41; returns
+ * BM25MI_ERR_INVALID_PARAM with a message on the first violation. */
+int bm25mi_validate_index(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,
+                          int64_t postings, const float *doc_norm, int64_t docs, const float *idf);
+
+/* ---------------------------------------------------------------------------
+ * Learned K-th-score model kernels (no reference counterpart). A 3-layer MLP,
+ * in -> 64 -> 64 -> 1, ReLU, FP32. All pointers are device pointers;
+ * params is the flat layout W1[64*in] b1[64] W2[64*64] b2[64] W3[64] b3[1].
+ * ------------------------------------------------------------------------- */
+#define BM25MI_MLP_HIDDEN 64
+int64_t bm25mi_mlp_param_count(int32_t in_features);
+/* y[n] = MLP(x[n, in]) */
+int bm25mi_mlp_forward(const float *params, const float *x, int32_t n, int32_t in_features, float *y,
+                       void *stream);
+/* Mean-squared-error loss and gradient: grad (same layout as params) +=
+ * d/dparams of (1/n) * sum (y - target)^2 ; loss_out[0] = that mean. */
+int bm25mi_mlp_backward(const float *params, const float *x, const float *target, int32_t n, int32_t in_features,
+                        float *grad, float *loss_out, void *stream);
+/* Adam step over count parameters; state m, v are device arrays of count floats. */
+int bm25mi_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t count, float lr, float beta1,
+                    float beta2, float eps, int32_t step, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BM25MI_H */

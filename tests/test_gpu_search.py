@@ -1,0 +1,211 @@
+"""GPU parity: libbm25mi Search vs the CPU oracle, bit for bit (doc IDs and FP32 score
+bits), through the C ABI. Cases follow the reference's own checks: its tiny
+known-answer corpora (FR/ut/cases/bm25/cpu_index_search_tests.cpp:95-134), its
+selector edge cases (FR/test/ascendhost/bm25/selector_check.cpp:63-118: zero, K-1,
+K, K+1 positive docs, ties), empty/unknown/duplicate/2147483646 terms, the fixed
+query sets fixtures/A.txt and B.txt, and the golden vectors of the reference binary.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import bm25mi
+from oracle import oracle as orc
+from tests import corpora
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / 'golden'
+
+
+def assert_same(got_scores, got_ids, want_scores, want_ids, label=''):
+    got_scores, want_scores = np.asarray(got_scores, np.float32), np.asarray(want_scores, np.float32)
+    assert np.array_equal(np.asarray(got_ids), np.asarray(want_ids)), f'{label}: doc IDs differ'
+    assert np.array_equal(got_scores.view(np.uint32), want_scores.view(np.uint32)), f'{label}: score bits differ'
+
+
+@pytest.fixture(scope='module')
+def synth():
+    corpus = bm25mi.generate('synthetic8k')
+    return corpus, corpus.index()
+
+
+@pytest.fixture(scope='module')
+def synth_session(synth, need_gpu):
+    _, index = synth
+    s = bm25mi.Session(index, max_queries=64, max_words=64 * 64, max_k=1024, block_docs=8192)
+    yield s
+    s.close()
+
+
+def test_synthetic8k_q8x3_k10(synth, synth_session):
+    corpus, index = synth
+    for seed in range(5):
+        queries = bm25mi.draw_queries(corpus.offsets, 8, 3, seed=seed)
+        offsets, terms = bm25mi.pack(queries)
+        want_s, want_i = orc.search(index, offsets, terms, 10)
+        got_s, got_i, _ = synth_session.search(queries, 10)
+        assert_same(got_s, got_i, want_s, want_i, f'seed {seed}')
+
+
+@pytest.mark.parametrize('k', [1, 2, 10, 32, 100, 1000, 1024])
+def test_synthetic8k_all_k(synth, synth_session, k):
+    corpus, index = synth
+    queries = bm25mi.draw_queries(corpus.offsets, 16, 4, seed=100 + k)
+    offsets, terms = bm25mi.pack(queries)
+    want_s, want_i = orc.search(index, offsets, terms, k)
+    got_s, got_i, _ = synth_session.search(queries, k)
+    assert_same(got_s, got_i, want_s, want_i, f'k={k}')
+
+
+def test_uneven_batch(synth, synth_session):
+    corpus, index = synth
+    rng = np.random.default_rng(9)
+    for trial in range(4):
+        nq = int(rng.integers(1, 65))
+        lengths = rng.integers(0, 33, size=nq)
+        queries = bm25mi.draw_queries(corpus.offsets, nq, lengths, seed=trial)
+        ks = rng.choice([1, 10, 100, 1000], size=nq).astype(np.int32)
+        offsets, terms = bm25mi.pack(queries)
+        want_s, want_i = orc.search(index, offsets, terms, ks)
+        got_s, got_i, _ = synth_session.search(queries, ks)
+        assert_same(got_s, got_i, want_s, want_i, f'trial {trial}')
+
+
+def test_edge_queries(synth, synth_session):
+    _, index = synth
+    queries = [[], [2147483646], [index.vocab], [0, 0], [4999, 4999, 4999], [3, 2147483646, 3],
+               list(range(64)), [4998], [0]]
+    offsets, terms = bm25mi.pack(queries)
+    for k in (1, 10, 1000):
+        want_s, want_i = orc.search(index, offsets, terms, k)
+        got_s, got_i, _ = synth_session.search(queries, k)
+        assert_same(got_s, got_i, want_s, want_i, f'edge k={k}')
+    # Empty query: IDs 0..K-1 with score 0 (large_search.cpp:184-187).
+    s, i, _ = synth_session.search([[]], 10)
+    assert i[0].tolist() == list(range(10)) and not s.any()
+
+
+def test_rejections_keep_session_usable(synth, synth_session):
+    bad = [([[-1]], 10), ([[1] * 65], 10), ([[1]] * 65, 10), ([[1]], 0), ([[1]], 1025)]
+    for queries, k in bad:
+        with pytest.raises(bm25mi.Bm25Error) as err:
+            synth_session.search(queries, k)
+        assert err.value.code == bm25mi._lib.INVALID_PARAM
+    test_synthetic8k_q8x3_k10(synth, synth_session)
+
+
+@pytest.mark.parametrize('n', [1, 6, 129, 2048])
+def test_reference_kat_tiny_corpora(n, need_gpu):
+    docs = corpora.tiny_documents(n)
+    index = corpora.index_from_documents(docs, 7)
+    queries_sets = ([[0, 0], [], [6], [7], [4, 5, 0], [2147483646, 2], [0, 1, 2, 3]], [[4], [0], [], [7], [2, 2, 1]],
+                    [[], [], []])
+    ks = sorted({1, min(n, 3), min(n, 10), min(n, 32)})
+    for block in (2048, 8192):
+        with bm25mi.Session(index, max_queries=7, max_words=128, max_k=max(ks), block_docs=block) as s:
+            for queries in queries_sets:
+                expected = corpora.independent_scores(docs, queries, 7)
+                for k in ks:
+                    got_s, got_i, _ = s.search(queries, k)
+                    for q in range(len(queries)):
+                        top = corpora.strict_top(expected[q], k)
+                        assert_same(got_s[q], got_i[q], expected[q, top], top, f'n={n} k={k} q={q}')
+
+
+def test_selector_edge_counts(need_gpu):
+    """Positive-doc counts around K and massive ties (selector_check.cpp:63-118)."""
+    n = 8192 * 3 + 77
+    rng = np.random.default_rng(5)
+    for positives in (0, 1, 9, 10, 11, 63, 64, 65, 1023, 1024, 1025, 4097, n):
+        docs_t0 = np.sort(rng.choice(n, size=positives, replace=False)).astype(np.int32)
+        # Term 0: the chosen docs with tf 1 and equal norms -> all tied; term 1: ties at two levels.
+        docs_t1 = np.arange(0, n, 7, dtype=np.int32)
+        offsets = np.array([0, len(docs_t0), len(docs_t0) + len(docs_t1)], np.int64)
+        docs = np.concatenate([docs_t0, docs_t1])
+        tf = np.concatenate([np.ones(len(docs_t0)), 1 + (docs_t1 % 2)]).astype(np.float32)
+        norm = np.ones(n, np.float32)
+        idf = np.array([2.0, 0.5], np.float32)
+        index = bm25mi.Index(offsets, docs, tf, norm, idf, k1=1.2)
+        queries = [[0], [0, 1], [1, 0], [1]]
+        offs, terms = bm25mi.pack(queries)
+        with bm25mi.Session(index, max_queries=4, max_words=8, max_k=1024, block_docs=4096) as s:
+            for k in (1, 10, 32, 1000):
+                want_s, want_i = orc.search(index, offs, terms, k)
+                got_s, got_i, _ = s.search(queries, k)
+                assert_same(got_s, got_i, want_s, want_i, f'positives={positives} k={k}')
+
+
+def test_golden_vectors_of_reference_binary(need_gpu):
+    record = json.loads((GOLDEN / 'reference_cpu_golden.json').read_text())
+    sessions = {}
+    try:
+        for case in record['cases']:
+            if case['corpus'] not in sessions:
+                index = corpora.golden_index(case['corpus'])
+                sessions[case['corpus']] = bm25mi.Session(index, max_queries=16, max_words=256, max_k=32)
+            got_s, got_i, _ = sessions[case['corpus']].search(case['queries'], case['k'])
+            assert got_i.tolist() == case['ids'], case['name']
+            assert got_s.view(np.uint32).tolist() == case['score_bits'], case['name']
+    finally:
+        for s in sessions.values():
+            s.close()
+
+
+@pytest.mark.parametrize('block', [2048, 4096, 8192])
+@pytest.mark.parametrize('graph', [True, False])
+def test_million_docs_block_sizes(block, graph, need_gpu, million):
+    corpus, index, queries = million
+    offsets, terms = bm25mi.pack(queries)
+    want_s, want_i = orc.search(index, offsets, terms, 10, threads=8)
+    with bm25mi.Session(index, max_queries=16, max_words=512, max_k=100, block_docs=block,
+                        use_graph=graph) as s:
+        got_s, got_i, _ = s.search(queries, 10)
+        assert_same(got_s, got_i, want_s, want_i, f'block={block} graph={graph}')
+        ks = np.array([1, 10, 100, 37] * 4, np.int32)
+        want_s, want_i = orc.search(index, offsets, terms, ks, threads=8)
+        got_s, got_i, _ = s.search(queries, ks)
+        assert_same(got_s, got_i, want_s, want_i, f'uneven block={block}')
+
+
+@pytest.fixture(scope='module')
+def million():
+    config = bm25mi.CorpusConfig(1_000_000, 1_000_000, 1.07, 56.0, seed=3)
+    corpus = bm25mi.generate(config)
+    index = corpus.index()
+    queries = bm25mi.draw_queries(corpus.offsets, 8, 3, seed=4) + bm25mi.draw_queries(corpus.offsets, 8, 12, seed=5)
+    return corpus, index, queries
+
+
+def test_pruned_path_equals_plain(synth, synth_session):
+    corpus, index = synth
+    queries = bm25mi.draw_queries(corpus.offsets, 16, 3, seed=77)
+    offsets, terms = bm25mi.pack(queries)
+    for k in (10, 100):
+        want_s, want_i = orc.search(index, offsets, terms, k)
+        kth = want_s[:, -1].astype(np.float32)
+        for factor, expect_fallback in ((0.5, False), (1.0, False), (1.01, True)):
+            thr = (kth * np.float32(factor)).astype(np.float32)
+            got_s, got_i, fb = synth_session.search(queries, k, thresholds=thr)
+            assert_same(got_s, got_i, want_s, want_i, f'pruned k={k} f={factor}')
+            if expect_fallback:
+                # Above the K-th score fewer than K docs survive: every positive-K query reruns.
+                assert fb[kth > 0].all()
+            else:
+                assert not fb.any()
+        st = synth_session.stats()
+        assert st['postings_total'] > 0
+
+
+def test_timed_path_reports_kernel_time(synth, synth_session):
+    corpus, index = synth
+    queries = bm25mi.draw_queries(corpus.offsets, 8, 3, seed=1)
+    offsets, terms = bm25mi.pack(queries)
+    want_s, want_i = orc.search(index, offsets, terms, 10)
+    got_s, got_i, _ = synth_session.search(queries, 10, timed=True)
+    assert_same(got_s, got_i, want_s, want_i, 'timed')
+    st = synth_session.stats()
+    assert st['score_kernel_ms'] > 0 and st['device_ms'] >= st['score_kernel_ms']
+    assert st['postings_scanned'] == st['postings_total']
