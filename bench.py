@@ -110,17 +110,25 @@ def main():
     log(f'timed {args.steps} Searches: median {statistics.median(samples):.4f} ms')
 
     # Scoring-kernel time with HIP events on the launch stream (direct launches).
-    kernel_ms, bytes_per = [], []
+    kernel_ms, bytes_per, pilot_ms = [], [], []
     for step in range(max(20, min(args.steps, 100))):
         offs, terms = packed[step % 2]
         session.search(packed[step % 2], args.k, timed=True)
         st = session.stats()
         kernel_ms.append(st['score_kernel_ms'])
+        pilot_ms.append(st['pilot_ms'])
         bytes_per.append(algorithmic_bytes(session, offs, terms, args.k, st))
     k_ms = statistics.mean(kernel_ms)
     b_avg = statistics.mean(bytes_per)
     achieved = b_avg / (k_ms * 1e-3) / 1e9
     st = session.stats()
+    graph_stats = []
+    for step in range(20):
+        session.search(packed[step % 2], args.k)
+        graph_stats.append(session.stats())
+    import ctypes as C
+    cus, lds_block, lds_cu = C.c_int32(), C.c_int32(), C.c_int32()
+    bm25mi._lib.check(bm25mi._lib.load().bm25mi_device_info(0, C.byref(cus), C.byref(lds_block), C.byref(lds_cu)))
 
     record = {
         'metric': 'Q8/K10 full-Search ms @10M docs; QPS; % HBM roofline; learned vs plain',
@@ -145,6 +153,17 @@ def main():
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
                      'kernel': 'KScore', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg},
         'postings_per_search': st['postings_total'],
+        'search_breakdown': {
+            'device_ms_graph': statistics.mean(g['device_ms'] for g in graph_stats),
+            'host_call_ms': statistics.mean(g['host_ms'] for g in graph_stats),
+            'candidates_per_search': statistics.mean(g['candidates'] for g in graph_stats),
+            'tasks_per_search': st['blocks_scored'] + st['blocks_skipped'],
+            'pilot_and_threshold_ms': statistics.mean(pilot_ms),
+            'pilot_postings': st['pilot_postings'],
+            'emitting_tasks': st['emitting_blocks'],
+            'exact_tasks': st['exact_blocks'],
+        },
+        'device': {'compute_units': cus.value, 'lds_per_block': lds_block.value, 'lds_per_cu': lds_cu.value},
     }
 
     if not args.profile and not args.no_cpu:

@@ -42,6 +42,10 @@ class Stats(C.Structure):
         ('candidates', C.c_int64),
         ('fallbacks', C.c_int32),
         ('queries', C.c_int32),
+        ('pilot_ms', C.c_double),
+        ('pilot_postings', C.c_int64),
+        ('emitting_blocks', C.c_int64),
+        ('exact_blocks', C.c_int64),
     ]
 
     def as_dict(self):

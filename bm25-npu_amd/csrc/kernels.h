@@ -10,12 +10,24 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kMaxTerms = 64;              // terms per query (fixed_query_packet.h:32)
 constexpr int kLongMin = 4096;             // lists this long use the block directory; shorter
                                            // ones a two-round 64-ary in-kernel search (64*64)
-constexpr int kFinalThreads = 256;         // final selection workgroup
-constexpr int kFinalLds = 4096;            // pool entries staged in LDS by the final selection
+constexpr int kFinalThreads = 1024;        // selection workgroup (threshold and final kernels)
+constexpr int kSortCap = 16384;            // pool entries a selection workgroup sorts in LDS (128 KB)
+constexpr int kPilotStride = 16;           // every 16th block is a pilot block, scored first
+constexpr int kPilotCap = 4096;            // pilot sample keys per query (sorted in a scoring WG's LDS)
+constexpr int kLoadsPerThread = 8;         // posting loads a thread keeps in flight per round
 constexpr int kStats = 8;
 
 // Stats slots (unsigned long long).
-enum { kStatPostings = 0, kStatScored = 1, kStatSkipped = 2, kStatCandidates = 3, kStatOverflow = 4 };
+enum {
+    kStatPostings = 0,
+    kStatScored = 1,
+    kStatSkipped = 2,
+    kStatCandidates = 3,
+    kStatOverflow = 4,
+    kStatEmitting = 5, // tasks that appended at least one candidate
+    kStatExact = 6,    // tasks that needed the exact K-th-score bisection
+    kStatPilot = 7     // postings streamed by the pilot pass
+};
 
 // Read-only resident index.
 struct IndexView {
@@ -39,13 +51,18 @@ struct BatchView {
     const float *thr;       // [maxQ] pruning threshold (0 = plain)
     const int64_t *poolOff; // [maxQ+1]
     const int64_t *outOff;  // [maxQ+1]
-    const int32_t *terms;   // [maxWords]
+    // Per query word, resolved on the host from the term ID: {first posting, posting
+    // count (0 for IDs >= vocab), long-term slot or -1, largest impact bits}.
+    const uint4 *words;     // [maxWords]
 };
 
 // Per-Search device scratch and results.
 struct WorkView {
     unsigned long long *pool; // composite keys (score bits << 32 | ~doc)
     uint32_t *count;          // [maxQ] pool fill
+    unsigned long long *pilot;// pilot sample keys, [maxQ * kPilotCap]
+    uint32_t *pilotCount;     // [maxQ] pilot sample fill
+    uint32_t *pilotDone;      // [maxQ] finished pilot tasks
     uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
     unsigned long long *stats;// [kStats]
     int32_t *status;          // [maxQ] 0 ok, 1 fewer than K survivors under a threshold

@@ -129,243 +129,22 @@ __device__ __forceinline__ void WaveBounds(const uint2 *post, uint32_t lo, uint3
     rb = r0b + __popcll(__ballot(vb && xb < b));
 }
 
-template <int B>
-__global__ __launch_bounds__(kThreads) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks,
-                                                    int32_t granulesPerBlock)
+// Wave-aggregated append: returns this lane's slot (counter += lanes with pred), or
+// 0xFFFFFFFF when pred is false. Every lane of the wave calls.
+__device__ __forceinline__ uint32_t WaveAppend(bool pred, uint32_t *counter)
 {
-    constexpr int kRows = B / (4 * kThreads); // float4 rows per thread
-    __shared__ __align__(16) float acc[B];
-    __shared__ uint32_t sLo[kMaxTerms], sHi[kMaxTerms];
-    __shared__ float sUb[kMaxTerms];
-    __shared__ uint32_t sRed[kWaves];
-    __shared__ uint32_t sShared[2];
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int32_t nq = bt.header[0];
-    const int64_t total = static_cast<int64_t>(nq) * blocks;
-    unsigned long long scanned = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0;
-
-    for (int64_t task = blockIdx.x; task < total; task += gridDim.x) {
-        const int q = static_cast<int>(task % nq);
-        const int blk = static_cast<int>(task / nq);
-        const int32_t base = blk * B;
-        const int32_t end = min(base + B, ix.ndocs);
-        const int64_t w0 = bt.qoff[q];
-        const int n = static_cast<int>(bt.qoff[q + 1] - w0);
-        const int32_t k = bt.k[q];
-        const float thr = bt.thr[q];
-        __syncthreads(); // previous task is done with LDS
-
-        // 1. Posting slice [lo, hi) of every query term inside this block.
-        for (int j = wave; j < n; j += kWaves) {
-            const int32_t t = bt.terms[w0 + j];
-            uint32_t lo = 0, hi = 0;
-            float ub = 0.0F;
-            if (t >= 0 && t < ix.vocab) {
-                const uint32_t t0 = static_cast<uint32_t>(ix.toff[t]), t1 = static_cast<uint32_t>(ix.toff[t + 1]);
-                const int32_t sl = ix.slot[t];
-                if (sl >= 0) {
-                    const uint32_t *d = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
-                    const int64_t g0 = static_cast<int64_t>(blk) * granulesPerBlock;
-                    const int64_t g1 = min(g0 + granulesPerBlock, ix.granules);
-                    lo = t0 + d[g0];
-                    hi = t0 + d[g1];
-                    if (thr > 0.0F && hi > lo) {
-                        float m = 0.0F;
-                        for (int64_t g = g0 + lane; g < g1; g += 64)
-                            m = fmaxf(m, ix.bmax[static_cast<int64_t>(sl) * ix.granules + g]);
-                        ub = __uint_as_float(WaveMax(__float_as_uint(m)));
-                    }
-                } else {
-                    WaveBounds(ix.post, t0, t1, base, end, lo, hi);
-                    ub = ix.tmax[t];
-                }
-            }
-            if (lane == 0) {
-                sLo[j] = lo;
-                sHi[j] = hi;
-                sUb[j] = ub;
-            }
-        }
-        float4 *acc4 = reinterpret_cast<float4 *>(acc);
-#pragma unroll
-        for (int r = 0; r < kRows; ++r)
-            acc4[r * kThreads + tid] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
-        if (tid == 0)
-            sShared[0] = max(__atomic_load_n(&wk.tau[q], __ATOMIC_RELAXED), 1u);
-        __syncthreads();
-
-        uint32_t tau = sShared[0];
-        if (thr > 0.0F) {
-            // Upper bound of any doc score in the block: block-max impacts summed in term order.
-            float bound = 0.0F;
-            for (int j = 0; j < n; ++j)
-                if (sHi[j] > sLo[j])
-                    bound += sUb[j];
-            if (bound < thr) {
-                ++skipped;
-                continue;
-            }
-            tau = max(tau, __float_as_uint(thr));
-        }
-        ++scored;
-
-        // 2. Term-ordered FP32 accumulation. Within one term each doc appears once, so
-        //    lanes never collide; the barrier orders the terms.
-        for (int j = 0; j < n; ++j) {
-            const uint32_t lo = sLo[j], hi = sHi[j];
-            if (lo >= hi)
-                continue;
-            if (tid == 0)
-                scanned += hi - lo;
-            uint32_t p = lo + tid;
-            for (; p + 3 * kThreads < hi; p += 4 * kThreads) {
-                const uint2 e0 = ix.post[p];
-                const uint2 e1 = ix.post[p + kThreads];
-                const uint2 e2 = ix.post[p + 2 * kThreads];
-                const uint2 e3 = ix.post[p + 3 * kThreads];
-                acc[static_cast<int32_t>(e0.x) - base] += __uint_as_float(e0.y);
-                acc[static_cast<int32_t>(e1.x) - base] += __uint_as_float(e1.y);
-                acc[static_cast<int32_t>(e2.x) - base] += __uint_as_float(e2.y);
-                acc[static_cast<int32_t>(e3.x) - base] += __uint_as_float(e3.y);
-            }
-            for (; p < hi; p += kThreads) {
-                const uint2 e = ix.post[p];
-                acc[static_cast<int32_t>(e.x) - base] += __uint_as_float(e.y);
-            }
-            __syncthreads();
-        }
-
-        // 3. Block top-K among docs with score bits >= tau (tau >= 1 excludes zeros).
-        uint32_t bits[kRows * 4];
-#pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            const float4 v = acc4[r * kThreads + tid];
-            bits[4 * r + 0] = __float_as_uint(v.x);
-            bits[4 * r + 1] = __float_as_uint(v.y);
-            bits[4 * r + 2] = __float_as_uint(v.z);
-            bits[4 * r + 3] = __float_as_uint(v.w);
-        }
-        auto countGe = [&](uint32_t x) {
-            uint32_t c = 0;
-#pragma unroll
-            for (int i = 0; i < kRows * 4; ++i)
-                c += bits[i] >= x ? 1u : 0u;
-            return BlockSum<kThreads>(c, sRed);
-        };
-        const uint32_t candidates = countGe(tau);
-        if (candidates == 0)
-            continue;
-        uint32_t cut = tau;          // emit bits > cut, or bits == cut with local index <= tieIdx
-        uint32_t tieIdx = B;         // B: every tie qualifies
-        bool exact = false;
-        if (candidates > static_cast<uint32_t>(k)) {
-            // K-th largest score bits by bisection: count(>= lo) >= k > count(>= hi).
-            uint32_t mx = 0;
-#pragma unroll
-            for (int i = 0; i < kRows * 4; ++i)
-                mx = max(mx, bits[i]);
-            mx = BlockMax<kThreads>(mx, sRed);
-            uint32_t lo = tau, hi = mx + 1;
-            while (hi - lo > 1) {
-                const uint32_t mid = lo + (hi - lo) / 2;
-                if (countGe(mid) >= static_cast<uint32_t>(k))
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            cut = lo;
-            exact = true;
-            const uint32_t greater = lo + 1 <= lo ? 0u : countGe(lo + 1);
-            const uint32_t need = static_cast<uint32_t>(k) - greater;
-            uint32_t ties = 0;
-#pragma unroll
-            for (int i = 0; i < kRows * 4; ++i)
-                ties += bits[i] == cut ? 1u : 0u;
-            ties = BlockSum<kThreads>(ties, sRed);
-            if (ties > need) {
-                // Smallest local index x with #{ties at index <= x} >= need.
-                int32_t a = -1, z = B - 1; // count(<= a) < need <= count(<= z)
-                while (z - a > 1) {
-                    const int32_t mid = a + (z - a) / 2;
-                    uint32_t c = 0;
-#pragma unroll
-                    for (int r = 0; r < kRows; ++r)
-#pragma unroll
-                        for (int cc = 0; cc < 4; ++cc) {
-                            const int32_t e = r * 4 * kThreads + tid * 4 + cc;
-                            c += (bits[4 * r + cc] == cut && e <= mid) ? 1u : 0u;
-                        }
-                    if (BlockSum<kThreads>(c, sRed) >= need)
-                        z = mid;
-                    else
-                        a = mid;
-                }
-                tieIdx = static_cast<uint32_t>(z);
-            }
-        }
-        // 4. Append the kept docs to the query's pool.
-        uint32_t mine = 0;
-#pragma unroll
-        for (int r = 0; r < kRows; ++r)
-#pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                const uint32_t e = r * 4 * kThreads + tid * 4 + cc;
-                const uint32_t x = bits[4 * r + cc];
-                mine += (x > cut || (x == cut && e <= tieIdx)) ? 1u : 0u;
-            }
-        uint32_t kept = 0;
-        const uint32_t before = BlockExclusive<kThreads>(mine, sRed, &kept);
-        if (tid == 0) {
-            sShared[1] = atomicAdd(&wk.count[q], kept);
-            if (exact)
-                atomicMax(&wk.tau[q], cut);
-        }
-        __syncthreads();
-        const uint32_t at = sShared[1];
-        const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
-        unsigned long long *pool = wk.pool + bt.poolOff[q];
-        uint32_t slot = at + before;
-#pragma unroll
-        for (int r = 0; r < kRows; ++r)
-#pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                const uint32_t e = r * 4 * kThreads + tid * 4 + cc;
-                const uint32_t x = bits[4 * r + cc];
-                if (x > cut || (x == cut && e <= tieIdx)) {
-                    if (static_cast<int64_t>(slot) < cap)
-                        pool[slot] = (static_cast<unsigned long long>(x) << 32) |
-                                     static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e));
-                    else
-                        ++overflow;
-                    ++slot;
-                }
-            }
-        if (tid == 0)
-            emitted += kept;
-    }
-    if (tid == 0) {
-        atomicAdd(&wk.stats[kStatPostings], scanned);
-        atomicAdd(&wk.stats[kStatScored], scored);
-        atomicAdd(&wk.stats[kStatSkipped], skipped);
-        atomicAdd(&wk.stats[kStatCandidates], emitted);
-    }
-    if (overflow)
-        atomicAdd(&wk.stats[kStatOverflow], overflow);
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = mask ? static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1) : 0u;
+    uint32_t base = 0;
+    if (mask && lane == leader)
+        base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, static_cast<int>(leader));
+    const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
+    return pred ? base + rank : 0xFFFFFFFFu;
 }
 
-__global__ void KReset(BatchView bt, WorkView wk, int32_t maxQ)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < maxQ) {
-        wk.count[i] = 0;
-        wk.tau[i] = 1;
-    }
-    if (i < kStats)
-        wk.stats[i] = 0;
-}
-
-// Bitonic sort of s[0, n) (n a power of two <= 2 * NT * iterations), descending.
+// Bitonic sort of s[0, n) (n a power of two), descending. All NT threads call.
 template <int NT, class T> __device__ void BitonicDesc(T *s, int n)
 {
     for (int size = 2; size <= n; size <<= 1)
@@ -384,10 +163,397 @@ template <int NT, class T> __device__ void BitonicDesc(T *s, int n)
         }
 }
 
-__global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView wk, int32_t ndocs)
+// Pilot sample size per block: K' = min(K, cap / pilot blocks), where cap is the number
+// of 64-bit keys the sorting workgroup's accumulator LDS holds.
+__device__ __forceinline__ int32_t PilotK(int32_t k, int32_t pilotBlocks, int32_t cap)
 {
-    __shared__ unsigned long long sKeys[kFinalLds];
-    __shared__ unsigned long long sSel[1024];
+    return min(k, cap / max(pilotBlocks, 1));
+}
+
+// The scoring kernel: persistent workgroups walk (query, block) tasks. Pilot blocks
+// (every kPilotStride-th block) come first; each keeps its exact block top-K (and its
+// top-K' in the pilot sample); the last pilot task of a query sorts the sample and
+// raises the query's lower bound tau to the sample's K-th score. Every later task keeps
+// its block top-K among docs scoring at least tau, so few candidates reach the pool.
+template <int B>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void KScore(
+    IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t granulesPerBlock)
+{
+    constexpr int kRows = B / (4 * kThreads); // float4 rows per thread
+    constexpr int U = kLoadsPerThread;        // 256-posting columns per round
+    constexpr int kSampleCap = B / 2; // pilot keys sortable in the accumulator LDS
+    static_assert(kSampleCap <= kPilotCap, "pilot pool stride");
+    __shared__ __align__(16) float acc[B];
+    __shared__ uint32_t sLo[kMaxTerms], sPre[kMaxTerms + 1], sCol[kMaxTerms + 1];
+    __shared__ float sUb[kMaxTerms];
+    __shared__ uint2 sRound[U];
+    __shared__ uint32_t sRed[kWaves];
+    __shared__ uint32_t sShared[4];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t nq = bt.header[0];
+    const int32_t pilotBlocks = (blocks + kPilotStride - 1) / kPilotStride;
+    const int64_t pilotTasks = static_cast<int64_t>(nq) * pilotBlocks;
+    const int64_t total = static_cast<int64_t>(nq) * blocks;
+    unsigned long long scanned = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0, emitting = 0, exactTasks = 0;
+
+    for (int64_t task = blockIdx.x; task < total; task += gridDim.x) {
+        int q, blk;
+        const bool isPilot = task < pilotTasks;
+        if (isPilot) {
+            q = static_cast<int>(task % nq);
+            blk = static_cast<int>(task / nq) * kPilotStride;
+        } else {
+            const int64_t t = task - pilotTasks;
+            q = static_cast<int>(t % nq);
+            const int64_t j = t / nq; // j-th block that is not a pilot block
+            blk = static_cast<int>(kPilotStride * (j / (kPilotStride - 1)) + j % (kPilotStride - 1) + 1);
+        }
+        const float thr = bt.thr[q];
+        const bool samples = isPilot && thr <= 0.0F; // a predicted threshold replaces the sample
+        const int32_t base = blk * B;
+        const int32_t end = min(base + B, ix.ndocs);
+        const int64_t w0 = bt.qoff[q];
+        const int n = static_cast<int>(bt.qoff[q + 1] - w0);
+        const int32_t k = bt.k[q];
+        __syncthreads(); // previous task is done with LDS
+
+        // 1. Posting slice of every query word inside this block: block directory for
+        //    long lists, a two-round 64-ary wave search for short ones.
+        for (int j = wave; j < n; j += kWaves) {
+            const uint4 info = bt.words[w0 + j];
+            uint32_t lo = 0, hi = 0;
+            float ub = 0.0F;
+            if (info.y > 0) {
+                const int32_t sl = static_cast<int32_t>(info.z);
+                if (sl >= 0) {
+                    const uint32_t *d = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
+                    const int64_t g0 = static_cast<int64_t>(blk) * granulesPerBlock;
+                    const int64_t g1 = min(g0 + granulesPerBlock, ix.granules);
+                    lo = info.x + d[g0];
+                    hi = info.x + d[g1];
+                    if (thr > 0.0F && hi > lo) {
+                        float m = 0.0F;
+                        for (int64_t g = g0 + lane; g < g1; g += 64)
+                            m = fmaxf(m, ix.bmax[static_cast<int64_t>(sl) * ix.granules + g]);
+                        ub = __uint_as_float(WaveMax(__float_as_uint(m)));
+                    }
+                } else {
+                    WaveBounds(ix.post, info.x, info.x + info.y, base, end, lo, hi);
+                    ub = __uint_as_float(info.w);
+                }
+            }
+            if (lane == 0) {
+                sLo[j] = lo;
+                sPre[j + 1] = hi - lo; // length; turned into a prefix below
+                sUb[j] = ub;
+            }
+        }
+        float4 *acc4 = reinterpret_cast<float4 *>(acc);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            acc4[r * kThreads + tid] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+        __syncthreads();
+        if (wave == 0) {
+            // Prefixes of slice lengths and of 256-posting column counts (one lane per word).
+            const uint32_t len = lane < n ? sPre[lane + 1] : 0u;
+            uint32_t x = len, c = (len + kThreads - 1) / kThreads;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o), z = __shfl_up(c, o);
+                if (lane >= o) {
+                    x += y;
+                    c += z;
+                }
+            }
+            if (lane < n) {
+                sPre[lane + 1] = x;
+                sCol[lane + 1] = c;
+            }
+            if (lane == 0) {
+                sPre[0] = 0;
+                sCol[0] = 0;
+                sShared[0] = max(__atomic_load_n(&wk.tau[q], __ATOMIC_RELAXED), 1u);
+                sShared[2] = 1;
+            }
+        }
+        __syncthreads();
+        if (thr > 0.0F && tid == 0) {
+            // Upper bound of any score in the block: per-word maxima summed in word order.
+            float bound = 0.0F;
+            for (int j = 0; j < n; ++j)
+                if (sPre[j + 1] > sPre[j])
+                    bound += sUb[j];
+            sShared[2] = bound < thr ? 0u : 1u;
+        }
+        if (thr > 0.0F)
+            __syncthreads();
+        const bool skip = sShared[2] == 0;
+        uint32_t tau = samples ? 1u : sShared[0];
+        if (thr > 0.0F)
+            tau = max(tau, __float_as_uint(thr));
+        if (skip) {
+            ++skipped;
+        } else {
+            ++scored;
+            // 2. Term-ordered FP32 accumulation. Each word's slice is cut into 256-posting
+            //    columns; a round issues up to U columns' loads (one 8-byte load per thread
+            //    per column) before any add, then adds them column by column with a barrier
+            //    where the word changes. Within one word each doc appears once, so lanes never
+            //    collide, and every doc's score is summed in query-word order.
+            const uint32_t columns = sCol[n];
+            if (tid == 0)
+                scanned += sPre[n];
+            for (uint32_t r0 = 0; r0 < columns; r0 += U) {
+                if (wave == 0 && lane < U) {
+                    const uint32_t col = r0 + lane;
+                    uint2 entry = make_uint2(0u, 0xFFFF0000u);
+                    if (col < columns) {
+                        int j = 0;
+                        while (sCol[j + 1] <= col)
+                            ++j;
+                        const uint32_t i = col - sCol[j];
+                        const uint32_t len = sPre[j + 1] - sPre[j];
+                        entry = make_uint2(sLo[j] + i * kThreads,
+                                           min(static_cast<uint32_t>(kThreads), len - i * kThreads) |
+                                               (static_cast<uint32_t>(j) << 16));
+                    }
+                    sRound[lane] = entry;
+                }
+                __syncthreads();
+                uint2 e[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint2 c = sRound[u];
+                    if (static_cast<uint32_t>(tid) < (c.y & 0xFFFFu))
+                        e[u] = ix.post[c.x + tid];
+                }
+                // Unused columns carry word 0xFFFF and count 0: no barrier, no add. The
+                // barrier condition reads LDS values shared by the whole workgroup.
+                uint32_t word = sRound[0].y >> 16;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t meta = sRound[u].y;
+                    const uint32_t w = meta >> 16;
+                    if (w != 0xFFFFu && w != word) {
+                        __syncthreads();
+                        word = w;
+                    }
+                    if (static_cast<uint32_t>(tid) < (meta & 0xFFFFu))
+                        acc[static_cast<int32_t>(e[u].x) - base] += __uint_as_float(e[u].y);
+                }
+                __syncthreads();
+            }
+
+            // 3. Block selection. Scores are re-read from LDS in each pass (8 x 16-byte
+            //    reads per thread) so they do not occupy registers across the passes.
+            const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
+            auto countIf = [&](auto pred) {
+                uint32_t c = 0;
+#pragma unroll
+                for (int r = 0; r < kRows; ++r) {
+                    const uint4 v = bits4[r * kThreads + tid];
+                    const uint32_t e = r * 4 * kThreads + tid * 4;
+                    c += (pred(v.x, e) ? 1u : 0u) + (pred(v.y, e + 1) ? 1u : 0u) + (pred(v.z, e + 2) ? 1u : 0u) +
+                         (pred(v.w, e + 3) ? 1u : 0u);
+                }
+                return c;
+            };
+            auto countGe = [&](uint32_t x) {
+                return BlockSum<kThreads>(countIf([x](uint32_t b, uint32_t) { return b >= x; }), sRed);
+            };
+            // Cut for the block's top-kk among docs with bits >= floor: keep bits > cut, or
+            // bits == cut at local index <= tieIdx. Returns false when nothing qualifies.
+            auto selectCut = [&](uint32_t kk, uint32_t floor, uint32_t &cut, uint32_t &tieIdx, bool &exact) {
+                const uint32_t candidates = countGe(floor);
+                cut = floor;
+                tieIdx = B;
+                exact = false;
+                if (candidates == 0)
+                    return false;
+                if (candidates <= kk)
+                    return true;
+                // kk-th largest score bits by bisection: count(>= lo) >= kk > count(>= hi).
+                uint32_t mx = 0;
+#pragma unroll
+                for (int r = 0; r < kRows; ++r) {
+                    const uint4 v = bits4[r * kThreads + tid];
+                    mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
+                }
+                mx = BlockMax<kThreads>(mx, sRed);
+                uint32_t lo = floor, hi = mx + 1;
+                while (hi - lo > 1) {
+                    const uint32_t mid = lo + (hi - lo) / 2;
+                    if (countGe(mid) >= kk)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                cut = lo;
+                exact = true;
+                const uint32_t need = kk - countGe(lo + 1);
+                const uint32_t c0 = cut;
+                const uint32_t ties = BlockSum<kThreads>(countIf([c0](uint32_t b, uint32_t) { return b == c0; }), sRed);
+                if (ties > need) {
+                    // Smallest local index x with #{ties at index <= x} >= need.
+                    int32_t a = -1, z = B - 1; // count(<= a) < need <= count(<= z)
+                    while (z - a > 1) {
+                        const int32_t mid = a + (z - a) / 2;
+                        const uint32_t c = countIf([c0, mid](uint32_t b, uint32_t e) {
+                            return b == c0 && static_cast<int32_t>(e) <= mid;
+                        });
+                        if (BlockSum<kThreads>(c, sRed) >= need)
+                            z = mid;
+                        else
+                            a = mid;
+                    }
+                    tieIdx = static_cast<uint32_t>(z);
+                }
+                return true;
+            };
+            // Appends the kept docs to pool[0, cap) through *counter; returns how many.
+            auto emit = [&](uint32_t cut, uint32_t tieIdx, unsigned long long *pool, uint32_t *counter, int64_t cap) {
+                auto keepIt = [cut, tieIdx](uint32_t b, uint32_t e) { return b > cut || (b == cut && e <= tieIdx); };
+                const uint32_t mine = countIf(keepIt);
+                uint32_t kept = 0;
+                const uint32_t before = BlockExclusive<kThreads>(mine, sRed, &kept);
+                if (tid == 0)
+                    sShared[1] = atomicAdd(counter, kept);
+                __syncthreads();
+                if (mine > 0) {
+                    uint32_t slot = sShared[1] + before;
+#pragma unroll
+                    for (int r = 0; r < kRows; ++r) {
+                        const uint4 v = bits4[r * kThreads + tid];
+                        const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int cc = 0; cc < 4; ++cc) {
+                            const uint32_t e = r * 4 * kThreads + tid * 4 + cc;
+                            if (keepIt(vals[cc], e)) {
+                                if (static_cast<int64_t>(slot) < cap)
+                                    pool[slot] = (static_cast<unsigned long long>(vals[cc]) << 32) |
+                                                 static_cast<unsigned long long>(0xFFFFFFFFu -
+                                                                                 (static_cast<uint32_t>(base) + e));
+                                else
+                                    ++overflow;
+                                ++slot;
+                            }
+                        }
+                    }
+                }
+                return kept;
+            };
+
+            uint32_t cut, tieIdx;
+            bool exact;
+            if (selectCut(static_cast<uint32_t>(k), tau, cut, tieIdx, exact)) {
+                const uint32_t kept =
+                    emit(cut, tieIdx, wk.pool + bt.poolOff[q], &wk.count[q], bt.poolOff[q + 1] - bt.poolOff[q]);
+                if (exact) {
+                    ++exactTasks;
+                    if (tid == 0)
+                        atomicMax(&wk.tau[q], cut); // the block's K-th score bounds the query's
+                }
+                if (tid == 0) {
+                    emitted += kept;
+                    ++emitting;
+                }
+            }
+            if (samples) {
+                // The sample keeps the block's top-K' (the same cut as above when K' == K).
+                const int32_t kp = PilotK(k, pilotBlocks, kSampleCap);
+                if (kp > 0) {
+                    if (kp < k)
+                        selectCut(static_cast<uint32_t>(kp), 1u, cut, tieIdx, exact);
+                    emit(cut, tieIdx, wk.pilot + static_cast<int64_t>(q) * kPilotCap, &wk.pilotCount[q],
+                         static_cast<int64_t>(pilotBlocks) * kp);
+                }
+            }
+        }
+        if (samples) {
+            // Publish this pilot task; the last one of the query turns the sample into tau.
+            __threadfence();
+            __syncthreads();
+            if (tid == 0)
+                sShared[3] = atomicAdd(&wk.pilotDone[q], 1u);
+            __syncthreads();
+            if (sShared[3] == static_cast<uint32_t>(pilotBlocks - 1)) {
+                __threadfence();
+                const int32_t kp = PilotK(k, pilotBlocks, kSampleCap);
+                const uint32_t p = min(__atomic_load_n(&wk.pilotCount[q], __ATOMIC_RELAXED),
+                                       static_cast<uint32_t>(pilotBlocks * kp));
+                if (p >= static_cast<uint32_t>(k)) {
+                    unsigned long long *sorted = reinterpret_cast<unsigned long long *>(acc);
+                    const unsigned long long *sample = wk.pilot + static_cast<int64_t>(q) * kPilotCap;
+                    int n2 = 1;
+                    while (n2 < static_cast<int>(p))
+                        n2 <<= 1;
+                    __syncthreads();
+                    for (int i = tid; i < n2; i += kThreads)
+                        sorted[i] = i < static_cast<int>(p) ? sample[i] : 0ULL;
+                    __syncthreads();
+                    BitonicDesc<kThreads>(sorted, n2);
+                    if (tid == 0)
+                        atomicMax(&wk.tau[q], max(1u, static_cast<uint32_t>(sorted[k - 1] >> 32)));
+                }
+                if (tid == 0) {
+                    wk.pilotCount[q] = 0;
+                    wk.pilotDone[q] = 0;
+                }
+            }
+        }
+    }
+    if (tid == 0) {
+        atomicAdd(&wk.stats[kStatPostings], scanned);
+        atomicAdd(&wk.stats[kStatScored], scored);
+        atomicAdd(&wk.stats[kStatSkipped], skipped);
+        atomicAdd(&wk.stats[kStatCandidates], emitted);
+        atomicAdd(&wk.stats[kStatEmitting], emitting);
+        atomicAdd(&wk.stats[kStatExact], exactTasks);
+    }
+    if (overflow)
+        atomicAdd(&wk.stats[kStatOverflow], overflow);
+}
+
+// Initial state of the per-query counters; run once at session creation. After that
+// the scoring and final kernels restore it for the next Search.
+__global__ void KReset(WorkView wk, int32_t maxQ)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < maxQ) {
+        wk.count[i] = 0;
+        wk.pilotCount[i] = 0;
+        wk.pilotDone[i] = 0;
+        wk.tau[i] = 1;
+    }
+    if (i < kStats)
+        wk.stats[i] = 0;
+}
+
+// The k-th largest of p unique keys in global memory (p > k), by a 64-step bisection
+// with block-wide counts. All threads call; every thread gets the result.
+__device__ unsigned long long KthFromGlobal(const unsigned long long *keys, uint32_t p, uint32_t k, uint32_t *red)
+{
+    unsigned long long kappa = 0;
+    for (int bit = 63; bit >= 0; --bit) {
+        const unsigned long long cand = kappa | (1ULL << bit);
+        uint32_t c = 0;
+        for (uint32_t i = threadIdx.x; i < p; i += kFinalThreads)
+            c += keys[i] >= cand ? 1u : 0u;
+        if (BlockSum<kFinalThreads>(c, red) >= k)
+            kappa = cand;
+    }
+    return kappa;
+}
+
+// Per query: the exact top-K of the candidate pool. Entries scoring below the query's
+// final lower bound tau cannot be in the top-K and are dropped; the rest are sorted in
+// LDS (score desc, doc asc) when they fit, else reduced by a 64-step bisection first.
+// Fewer than K positive docs are padded with the lowest-ID zero-score docs.
+__global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView wk, unsigned long long *hostStats)
+{
+    __shared__ unsigned long long sKeys[kSortCap];
+    __shared__ unsigned long long sIds[1024];
     __shared__ uint32_t sRed[kFinalThreads / 64];
     __shared__ uint32_t sN;
     const int q = blockIdx.x;
@@ -395,78 +561,95 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     if (q >= nq)
         return;
     const int tid = threadIdx.x;
+    if (q == 0 && tid < kStats) {
+        // The scoring kernel has finished: publish its counters and clear them.
+        hostStats[tid] = wk.stats[tid];
+        wk.stats[tid] = 0;
+    }
     const uint32_t k = static_cast<uint32_t>(bt.k[q]);
     const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
-    const uint32_t filled = __atomic_load_n(&wk.count[q], __ATOMIC_RELAXED);
-    const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(filled), cap));
+    const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(wk.count[q]), cap));
+    const uint32_t tau = wk.tau[q];
     const unsigned long long *pool = wk.pool + bt.poolOff[q];
     const bool pruned = bt.thr[q] > 0.0F;
-    if (pruned && p < k) {
+    if (tid == 0)
+        sN = 0;
+    __syncthreads();
+    if (tid == 0) {
+        // Restore the per-query counters for the next Search.
+        wk.count[q] = 0;
+        wk.tau[q] = 1;
+    }
+    // Keep entries with score bits >= tau.
+    for (uint32_t i0 = 0; i0 < p; i0 += kFinalThreads) {
+        const uint32_t i = i0 + tid;
+        const unsigned long long key = i < p ? pool[i] : 0ULL;
+        const uint32_t at = WaveAppend(i < p && static_cast<uint32_t>(key >> 32) >= tau, &sN);
+        if (at < static_cast<uint32_t>(kSortCap))
+            sKeys[at] = key;
+    }
+    __syncthreads();
+    const uint32_t c = sN;
+    if (pruned && c < k) {
         if (tid == 0)
             wk.status[q] = 1; // fewer than K survivors: the host reruns this query on the plain path
         return;
     }
-    const bool inLds = p <= static_cast<uint32_t>(kFinalLds);
-    if (inLds)
+    const uint32_t m = c < k ? c : k;
+    if (c <= static_cast<uint32_t>(kSortCap)) {
+        int n2 = 1;
+        while (n2 < static_cast<int>(c))
+            n2 <<= 1;
+        for (int i = static_cast<int>(c) + tid; i < n2; i += kFinalThreads)
+            sKeys[i] = 0ULL;
+        __syncthreads();
+        BitonicDesc<kFinalThreads>(sKeys, n2);
+    } else {
+        // Too many to sort: the K-th key over the whole pool, then the K keys at or above it.
+        const unsigned long long kappa = KthFromGlobal(pool, p, k, sRed);
+        if (tid == 0)
+            sN = 0;
+        __syncthreads();
         for (uint32_t i = tid; i < p; i += kFinalThreads)
-            sKeys[i] = pool[i];
-    __syncthreads();
-    unsigned long long kappa = 0;
-    if (p > k) {
-        for (int bit = 63; bit >= 0; --bit) {
-            const unsigned long long cand = kappa | (1ULL << bit);
-            uint32_t c = 0;
-            for (uint32_t i = tid; i < p; i += kFinalThreads)
-                c += (inLds ? sKeys[i] : pool[i]) >= cand ? 1u : 0u;
-            if (BlockSum<kFinalThreads>(c, sRed) >= k)
-                kappa = cand;
-        }
+            if (pool[i] >= kappa) {
+                const uint32_t at = atomicAdd(&sN, 1u);
+                if (at < k)
+                    sIds[at] = pool[i];
+            }
+        __syncthreads();
+        int n2 = 1;
+        while (n2 < static_cast<int>(k))
+            n2 <<= 1;
+        for (int i = tid; i < n2; i += kFinalThreads)
+            sKeys[i] = i < static_cast<int>(k) ? sIds[i] : 0ULL;
+        __syncthreads();
+        BitonicDesc<kFinalThreads>(sKeys, n2);
     }
-    const uint32_t m = p < k ? p : k;
-    if (tid == 0)
-        sN = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i < p; i += kFinalThreads) {
-        const unsigned long long key = inLds ? sKeys[i] : pool[i];
-        if (key >= kappa) {
-            const uint32_t at = atomicAdd(&sN, 1u);
-            if (at < 1024)
-                sSel[at] = key;
-        }
-    }
-    __syncthreads();
-    int n2 = 1;
-    while (n2 < static_cast<int>(m))
-        n2 <<= 1;
-    for (int i = static_cast<int>(m) + tid; i < n2; i += kFinalThreads)
-        sSel[i] = 0;
-    __syncthreads();
-    BitonicDesc<kFinalThreads>(sSel, n2);
     const int64_t row = bt.outOff[q];
     for (uint32_t r = tid; r < m; r += kFinalThreads) {
-        const unsigned long long key = sSel[r];
+        const unsigned long long key = sKeys[r];
         wk.scores[row + r] = __uint_as_float(static_cast<uint32_t>(key >> 32));
         wk.ids[row + r] = static_cast<int32_t>(0xFFFFFFFFu - static_cast<uint32_t>(key & 0xFFFFFFFFu));
     }
     if (tid == 0)
         wk.status[q] = 0;
     if (m < k) {
-        // Pad with the lowest-ID docs that scored zero (every positive doc is in the pool).
+        // Pad with the lowest-ID docs that scored zero (every positive doc is in the pool):
+        // sort the positive IDs ascending (descending on their complements), then walk.
+        int n2 = 1;
+        while (n2 < static_cast<int>(m))
+            n2 <<= 1;
+        for (int r = tid; r < n2; r += kFinalThreads)
+            sIds[r] = r < static_cast<int>(m)
+                          ? ~static_cast<unsigned long long>(0xFFFFFFFFu - static_cast<uint32_t>(sKeys[r] & 0xFFFFFFFFu))
+                          : 0ULL;
         __syncthreads();
-        for (uint32_t r = tid; r < static_cast<uint32_t>(n2); r += kFinalThreads)
-            sKeys[r] = r < m ? static_cast<unsigned long long>(0xFFFFFFFFu - static_cast<uint32_t>(sSel[r] & 0xFFFFFFFFu))
-                             : ~0ULL;
-        __syncthreads();
-        // Ascending by doc ID: sort descending on the complemented IDs.
-        for (uint32_t r = tid; r < static_cast<uint32_t>(n2); r += kFinalThreads)
-            sKeys[r] = ~sKeys[r];
-        __syncthreads();
-        BitonicDesc<kFinalThreads>(sKeys, n2);
+        BitonicDesc<kFinalThreads>(sIds, n2);
         if (tid == 0) {
             uint32_t j = 0;
             int32_t d = 0;
             for (uint32_t r = m; r < k; ++r) {
-                while (j < m && static_cast<int32_t>(~sKeys[j]) == d) {
+                while (j < m && static_cast<int32_t>(~sIds[j]) == d) {
                     ++j;
                     ++d;
                 }
@@ -478,18 +661,11 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     }
 }
 
-// Copies the device stats into the host-visible result block.
-__global__ void KPublish(WorkView wk, unsigned long long *hostStats)
-{
-    if (threadIdx.x < kStats)
-        hostStats[threadIdx.x] = wk.stats[threadIdx.x];
-}
-
-void LaunchReset(const BatchView &bt, const WorkView &wk, int32_t maxQ, hipStream_t stream)
+void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 {
     const int threads = 256;
     const int grid = (max(maxQ, kStats) + threads - 1) / threads;
-    hipLaunchKernelGGL(KReset, dim3(grid), dim3(threads), 0, stream, bt, wk, maxQ);
+    hipLaunchKernelGGL(KReset, dim3(grid), dim3(threads), 0, stream, wk, maxQ);
 }
 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
@@ -510,14 +686,10 @@ bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const 
     }
 }
 
-void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, int32_t ndocs, hipStream_t stream)
+void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
+                 hipStream_t stream)
 {
-    hipLaunchKernelGGL(KFinal, dim3(maxQ), dim3(kFinalThreads), 0, stream, bt, wk, ndocs);
-}
-
-void LaunchPublish(const WorkView &wk, unsigned long long *hostStats, hipStream_t stream)
-{
-    hipLaunchKernelGGL(KPublish, dim3(1), dim3(64), 0, stream, wk, hostStats);
+    hipLaunchKernelGGL(KFinal, dim3(maxQ), dim3(kFinalThreads), 0, stream, bt, wk, hostStats);
 }
 
 } // namespace bm25mi

@@ -29,11 +29,11 @@
 
 namespace bm25mi {
 
-void LaunchReset(const BatchView &bt, const WorkView &wk, int32_t maxQ, hipStream_t stream);
+void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t granulesPerBlock, int32_t grid, hipStream_t stream);
-void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, int32_t ndocs, hipStream_t stream);
-void LaunchPublish(const WorkView &wk, unsigned long long *hostStats, hipStream_t stream);
+void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
+                 hipStream_t stream);
 
 #define BM25MI_HIP(expr, what)                                                                                         \
     do {                                                                                                               \
@@ -67,6 +67,7 @@ public:
     // Device memory.
     void *dPost = nullptr, *dToff = nullptr, *dSlot = nullptr, *dDir = nullptr, *dBmax = nullptr, *dTmax = nullptr;
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
+    void *dPilot = nullptr, *dPilotCount = nullptr, *dPilotDone = nullptr;
     // Pinned host memory: request packet (copied by the graph) and result block (written by the device).
     void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     size_t packetBytes = 0, resultBytes = 0;
@@ -80,7 +81,7 @@ public:
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int device = 0;
 
     bool failed = false, closed = false;
@@ -205,7 +206,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     offOut = at;
     at = Align(at + (q + 1) * 8, 8);
     offTerms = at;
-    at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 4, 256);
+    at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 16, 256);
     packetBytes = at;
     BM25MI_HIP(hipHostMalloc(&hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
     std::memset(hPacket, 0, packetBytes);
@@ -222,6 +223,9 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
 
     dPool = Alloc<unsigned long long>(q * poolPerQuery, "alloc candidate pools");
     dCount = Alloc<uint32_t>(q, "alloc pool counts");
+    dPilot = Alloc<unsigned long long>(q * kPilotCap, "alloc pilot samples");
+    dPilotCount = Alloc<uint32_t>(q, "alloc pilot counts");
+    dPilotDone = Alloc<uint32_t>(q, "alloc pilot completion counters");
     dTau = Alloc<uint32_t>(q, "alloc thresholds");
     dStats = Alloc<unsigned long long>(kStats, "alloc stats");
 
@@ -233,12 +237,16 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     bt = BatchView{reinterpret_cast<const int32_t *>(pk),           reinterpret_cast<const int64_t *>(pk + offQoff),
                    reinterpret_cast<const int32_t *>(pk + offK),    reinterpret_cast<const float *>(pk + offThr),
                    reinterpret_cast<const int64_t *>(pk + offPool), reinterpret_cast<const int64_t *>(pk + offOut),
-                   reinterpret_cast<const int32_t *>(pk + offTerms)};
+                   reinterpret_cast<const uint4 *>(pk + offTerms)};
     auto *rs = static_cast<unsigned char *>(hResultDev);
     wk = WorkView{static_cast<unsigned long long *>(dPool), static_cast<uint32_t *>(dCount),
+                  static_cast<unsigned long long *>(dPilot), static_cast<uint32_t *>(dPilotCount),
+                  static_cast<uint32_t *>(dPilotDone),
                   static_cast<uint32_t *>(dTau),            static_cast<unsigned long long *>(dStats),
                   reinterpret_cast<int32_t *>(rs + resStatus), reinterpret_cast<float *>(rs + resScores),
                   reinterpret_cast<int32_t *>(rs + resIds)};
+    LaunchInit(wk, cfg.max_queries, stream);
+    BM25MI_HIP(hipStreamSynchronize(stream), "initialise counters");
     if (cfg.use_graph)
         Capture();
     Bootstrap();
@@ -247,14 +255,12 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
 void Session::Enqueue(bool timed)
 {
     BM25MI_HIP(hipMemcpyAsync(dPacket, hPacket, packetBytes, hipMemcpyHostToDevice, stream), "packet H2D");
-    LaunchReset(bt, wk, cfg.max_queries, stream);
     if (timed)
         BM25MI_HIP(hipEventRecord(ev[1], stream), "record score start");
     Require(LaunchScore(blockDocs, ix, bt, wk, blocks, granulesPerBlock, grid, stream), "Unsupported block size");
     if (timed)
         BM25MI_HIP(hipEventRecord(ev[2], stream), "record score end");
-    LaunchFinal(bt, wk, cfg.max_queries, static_cast<int32_t>(ndocs), stream);
-    LaunchPublish(wk, static_cast<unsigned long long *>(hResultDev), stream);
+    LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(hResultDev), stream);
 }
 
 void Session::Capture()
@@ -289,8 +295,10 @@ void Session::Run(bool timed)
     if (timed) {
         BM25MI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]), "score kernel time");
         last.score_kernel_ms = ms;
+        last.pilot_ms = 0;
     } else {
         last.score_kernel_ms = -1;
+        last.pilot_ms = -1;
     }
 }
 
@@ -330,14 +338,21 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
         thr[q] = query.thr ? query.thr[q] : 0.0F;
         pool[q + 1] = pool[q] + std::min<int64_t>(static_cast<int64_t>(blocks) * query.k[q], ndocs);
         out[q + 1] = out[q] + query.k[q];
-        for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w) {
-            const int32_t t = query.terms[w];
-            if (t < vocab)
-                total += toffHost[t + 1] - toffHost[t];
+    }
+    auto *info = reinterpret_cast<uint4 *>(pk + offTerms);
+    for (int64_t w = 0; w < words; ++w) {
+        const int32_t t = query.terms[w];
+        if (t < vocab) {
+            const int64_t len = toffHost[t + 1] - toffHost[t];
+            uint32_t ub;
+            std::memcpy(&ub, &tmaxHost[t], 4);
+            info[w] = make_uint4(static_cast<uint32_t>(toffHost[t]), static_cast<uint32_t>(len),
+                                 static_cast<uint32_t>(slotHost[t]), ub);
+            total += len;
+        } else {
+            info[w] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
         }
     }
-    if (words)
-        std::memcpy(pk + offTerms, query.terms, words * 4);
 
     try {
         Run(timed);
@@ -366,6 +381,9 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
             std::memcpy(outIds + out[q], id + out[q], (out[q + 1] - out[q]) * 4);
         }
         last.postings_scanned = static_cast<int64_t>(stats[kStatPostings]);
+        last.pilot_postings = static_cast<int64_t>(stats[kStatPilot]);
+        last.emitting_blocks = static_cast<int64_t>(stats[kStatEmitting]);
+        last.exact_blocks = static_cast<int64_t>(stats[kStatExact]);
         last.postings_total = total;
         last.blocks_scored = static_cast<int64_t>(stats[kStatScored]);
         last.blocks_skipped = static_cast<int64_t>(stats[kStatSkipped]);
@@ -410,7 +428,8 @@ void Session::Release() noexcept
         (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
-    for (void **p : {&dPost, &dToff, &dSlot, &dDir, &dBmax, &dTmax, &dPacket, &dPool, &dCount, &dTau, &dStats})
+    for (void **p : {&dPost, &dToff, &dSlot, &dDir, &dBmax, &dTmax, &dPacket, &dPool, &dCount, &dTau, &dStats,
+                     &dPilot, &dPilotCount, &dPilotDone})
         if (*p) {
             (void)hipFree(*p);
             *p = nullptr;
@@ -518,6 +537,7 @@ static int SearchCommon(bm25mi_session *s, const Query &query, float *scores, in
                     if (first.score_kernel_ms >= 0 && ss.last.score_kernel_ms >= 0)
                         ss.last.score_kernel_ms += first.score_kernel_ms;
                     ss.last.postings_scanned += first.postings_scanned;
+                    ss.last.pilot_postings += first.pilot_postings;
                     ss.last.postings_total = first.postings_total;
                     ss.last.blocks_scored += first.blocks_scored;
                     ss.last.blocks_skipped += first.blocks_skipped;

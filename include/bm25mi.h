@@ -78,6 +78,10 @@ typedef struct {
     int64_t candidates;        /* entries written to the candidate pools */
     int32_t fallbacks;         /* queries rerun on the plain path */
     int32_t queries;
+    double pilot_ms;           /* HIP-event time of the pilot pass + threshold (timed Search only) */
+    int64_t pilot_postings;    /* postings streamed by the pilot pass */
+    int64_t emitting_blocks;   /* scoring tasks that appended candidates */
+    int64_t exact_blocks;      /* tasks that needed the exact block K-th-score bisection */
 } bm25mi_stats;
 
 void bm25mi_config_default(bm25mi_config *cfg);
