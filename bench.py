@@ -56,10 +56,10 @@ def cpu_model():
 
 
 def algorithmic_bytes(session, offsets, terms, k, stats):
-    """Bytes one Search must move: 8 B per scanned posting ({doc, impact}), 8 B of
-    directory per (long term, block) pair, the query packet and the top-K output."""
+    """Bytes the main scoring launch must move: 8 B per posting it streams ({doc, impact};
+    the pilot pass streams the rest), the query packet and the top-K output."""
     nq = len(offsets) - 1
-    return 8 * stats['postings_scanned'] + 8 * (nq + 1) + 4 * len(terms) + 8 * nq * k
+    return 8 * (stats['postings_scanned'] - stats['pilot_postings']) + 8 * (nq + 1) + 4 * len(terms) + 8 * nq * k
 
 
 def main():
@@ -72,6 +72,7 @@ def main():
     ap.add_argument('--terms', type=int, default=3)
     ap.add_argument('--k', type=int, default=10)
     ap.add_argument('--block-docs', type=int, default=8192)
+    ap.add_argument('--grid', type=int, default=0, help='persistent scoring workgroups (0 = fill the GPU)')
     ap.add_argument('--cpu-calls', type=int, default=10, help='timed CPU oracle calls (A/B alternating)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: no CPU leg')
@@ -92,7 +93,7 @@ def main():
 
     t0 = time.time()
     session = bm25mi.Session(index, max_queries=args.queries, max_words=args.queries * args.terms, max_k=args.k,
-                             block_docs=args.block_docs)
+                             block_docs=args.block_docs, grid_blocks=args.grid)
     log(f'session create {time.time() - t0:.1f}s')
 
     results = {}

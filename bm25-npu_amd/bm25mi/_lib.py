@@ -46,10 +46,13 @@ class Stats(C.Structure):
         ('pilot_postings', C.c_int64),
         ('emitting_blocks', C.c_int64),
         ('exact_blocks', C.c_int64),
+        ('phase_cycles', C.c_int64 * 8),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {name: getattr(self, name) for name, _ in self._fields_}
+        out['phase_cycles'] = list(self.phase_cycles)
+        return out
 
 
 class CorpusSpec(C.Structure):

@@ -19,19 +19,25 @@ HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 COMMON = ['-O3', '-fPIC', '-std=c++17', '-ffp-contract=off', '-fno-fast-math', '-pthread', '-Wall',
           '-Wno-unused-command-line-argument']
 DEVICE = ['--offload-arch=gfx950', '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+EXTRA = os.environ.get('BM25MI_CFLAGS', '').split()
 SOURCES = ['search.hip', 'session.hip', 'mlp.hip', 'corpus.cpp', 'index.cpp']
 
 
 def compile_one(src):
     out = OBJ / (src + '.o')
-    cmd = [HIPCC] + COMMON + (DEVICE + ['-x', 'hip'] if src.endswith('.hip') else []) + ['-c', str(CSRC / src), '-o', str(out)]
+    cmd = [HIPCC] + COMMON + EXTRA + (DEVICE + ['-x', 'hip'] if src.endswith('.hip') else []) + \
+        ['-c', str(CSRC / src), '-o', str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(f'compile {src} failed:\n{r.stdout}\n{r.stderr}')
     return out
 
 
-def build(jobs=8):
+def build(jobs=8, lib_dir=None):
+    global LIB, OBJ
+    if lib_dir is not None:
+        LIB = Path(lib_dir)
+        OBJ = LIB / 'obj'
     OBJ.mkdir(parents=True, exist_ok=True)
     sources = [s for s in SOURCES if (CSRC / s).exists()]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -49,5 +55,6 @@ def build(jobs=8):
 if __name__ == '__main__':
     ap = argparse.ArgumentParser()
     ap.add_argument('--jobs', type=int, default=8)
+    ap.add_argument('--lib-dir', default=None, help='output directory (default bm25-npu_amd/lib)')
     a = ap.parse_args()
-    print(build(a.jobs))
+    print(build(a.jobs, a.lib_dir))

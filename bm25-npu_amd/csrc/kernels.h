@@ -12,10 +12,19 @@ constexpr int kLongMin = 4096;             // lists this long use the block dire
                                            // ones a two-round 64-ary in-kernel search (64*64)
 constexpr int kFinalThreads = 1024;        // selection workgroup (threshold and final kernels)
 constexpr int kSortCap = 16384;            // pool entries a selection workgroup sorts in LDS (128 KB)
-constexpr int kPilotStride = 16;           // every 16th block is a pilot block, scored first
-constexpr int kPilotCap = 4096;            // pilot sample keys per query (sorted in a scoring WG's LDS)
-constexpr int kLoadsPerThread = 8;         // posting loads a thread keeps in flight per round
-constexpr int kStats = 8;
+constexpr int kPilotStride = 16;           // every 16th block is scored first (spreads the first round)
+constexpr int kEmitCap = 512;              // pool entries per block reserved beyond K (sizing only)
+// Score histogram of kept candidates, per query: bucket = (score bits >> 19) - kHistBase,
+// i.e. 16 buckets per octave from 2^-10 up (lower scores fall in bucket 0, higher ones in
+// the last bucket).
+constexpr int kHistBins = 320;
+constexpr uint32_t kHistBase = (127u - 10u) << 4;
+#ifndef BM25MI_LOADS
+#define BM25MI_LOADS 8
+#endif
+constexpr int kLoadsPerThread = BM25MI_LOADS; // posting loads a thread keeps in flight per round
+constexpr int kStats = 16;
+constexpr int kCachedQueries = 256;      // per-query headers kept in LDS by the scoring kernel
 
 // Stats slots (unsigned long long).
 enum {
@@ -26,7 +35,8 @@ enum {
     kStatOverflow = 4,
     kStatEmitting = 5, // tasks that appended at least one candidate
     kStatExact = 6,    // tasks that needed the exact K-th-score bisection
-    kStatPilot = 7     // postings streamed by the pilot pass
+    kStatPilot = 7,    // postings streamed by the pilot pass
+    kStatPhase = 8     // 8..15: scoring-kernel cycles per phase, main then pilot pass (BM25MI_PHASES builds)
 };
 
 // Read-only resident index.
@@ -60,14 +70,16 @@ struct BatchView {
 struct WorkView {
     unsigned long long *pool; // composite keys (score bits << 32 | ~doc)
     uint32_t *count;          // [maxQ] pool fill
-    unsigned long long *pilot;// pilot sample keys, [maxQ * kPilotCap]
-    uint32_t *pilotCount;     // [maxQ] pilot sample fill
-    uint32_t *pilotDone;      // [maxQ] finished pilot tasks
+    uint32_t *hist;           // [maxQ * kHistBins] histogram of kept candidates' scores
     uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
     unsigned long long *stats;// [kStats]
     int32_t *status;          // [maxQ] 0 ok, 1 fewer than K survivors under a threshold
     float *scores;            // [maxQ*maxK]
     int32_t *ids;             // [maxQ*maxK]
+    // Posting slice {first, end} of every (query word, block), written by the planning
+    // kernel: entry blocks * qoff[q] + blk * n_q + j for word j of query q.
+    uint2 *slices;            // [blocks * maxWords]
+    float *bound;             // [maxQ * blocks] block upper bound (threshold queries only)
 };
 
 } // namespace bm25mi

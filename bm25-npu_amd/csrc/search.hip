@@ -163,100 +163,200 @@ template <int NT, class T> __device__ void BitonicDesc(T *s, int n)
         }
 }
 
-// Pilot sample size per block: K' = min(K, cap / pilot blocks), where cap is the number
-// of 64-bit keys the sorting workgroup's accumulator LDS holds.
-__device__ __forceinline__ int32_t PilotK(int32_t k, int32_t pilotBlocks, int32_t cap)
+__device__ __forceinline__ uint32_t HistBucket(uint32_t bits)
 {
-    return min(k, cap / max(pilotBlocks, 1));
+    const uint32_t b = bits >> 19;
+    return b <= kHistBase ? 0u : min(b - kHistBase, static_cast<uint32_t>(kHistBins - 1));
 }
 
-// The scoring kernel: persistent workgroups walk (query, block) tasks. Pilot blocks
-// (every kPilotStride-th block) come first; each keeps its exact block top-K (and its
-// top-K' in the pilot sample); the last pilot task of a query sorts the sample and
-// raises the query's lower bound tau to the sample's K-th score. Every later task keeps
-// its block top-K among docs scoring at least tau, so few candidates reach the pool.
+// Smallest score bits of a bucket (bucket 0 also holds scores below 2^-10: its edge is the
+// smallest positive score).
+__device__ __forceinline__ uint32_t HistEdge(uint32_t bucket)
+{
+    return bucket == 0 ? 1u : (bucket + kHistBase) << 19;
+}
+
+// Lower bound of a query's K-th score from its kept-candidate histogram, computed by one
+// wave: the edge of the highest bucket b with at least k counted scores at or above it
+// (those k candidates score at least that edge). 1 when fewer than k are counted.
+__device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t k)
+{
+    constexpr int kPer = (kHistBins + 63) / 64;
+    const int lane = static_cast<int>(__lane_id());
+    uint32_t h[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int b = lane * kPer + i;
+        h[i] = b < kHistBins ? __atomic_load_n(hist + b, __ATOMIC_RELAXED) : 0u;
+        sum += h[i];
+    }
+    // above = counts in buckets of higher lanes (suffix over lanes).
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_down(x, o);
+        if (lane + o < 64)
+            x += y;
+    }
+    uint32_t above = x - sum;
+    uint32_t best = 0; // bucket + 1, or 0
+#pragma unroll
+    for (int i = kPer - 1; i >= 0; --i) {
+        above += h[i];
+        if (best == 0 && above >= k)
+            best = static_cast<uint32_t>(lane * kPer + i) + 1;
+    }
+    const uint32_t top = WaveMax(best);
+    return top ? HistEdge(top - 1) : 1u;
+}
+
+// Planning kernel, one wave per (query, block) task: the posting slice {first, end} of
+// every query word inside the block (block directory for long lists, a two-round 64-ary
+// search for short ones), and for threshold queries the block's score upper bound
+// (per-word block maxima summed in word order).
+__global__ __launch_bounds__(256) void KPlan(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t blockDocs,
+                                             int32_t granulesPerBlock)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / 64);
+    const int32_t nq = bt.header[0];
+    const int64_t total = static_cast<int64_t>(nq) * blocks;
+    for (int64_t task = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); task < total;
+         task += waves) {
+        const int q = static_cast<int>(task % nq);
+        const int blk = static_cast<int>(task / nq);
+        const int32_t base = blk * blockDocs;
+        const int32_t end = min(base + blockDocs, ix.ndocs);
+        const int64_t w0 = bt.qoff[q];
+        const int n = static_cast<int>(bt.qoff[q + 1] - w0);
+        const float thr = bt.thr[q];
+        uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+        if (lane < n)
+            info = bt.words[w0 + lane];
+        uint32_t lo = 0, hi = 0;
+        float ub = 0.0F;
+        const int32_t sl = static_cast<int32_t>(info.z);
+        const int64_t g0 = static_cast<int64_t>(blk) * granulesPerBlock;
+        const int64_t g1 = min(g0 + granulesPerBlock, ix.granules);
+        if (info.y > 0 && sl >= 0) {
+            const uint32_t *d = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
+            lo = info.x + d[g0];
+            hi = info.x + d[g1];
+            if (thr > 0.0F && hi > lo)
+                for (int64_t g = g0; g < g1; ++g)
+                    ub = fmaxf(ub, ix.bmax[static_cast<int64_t>(sl) * ix.granules + g]);
+        }
+        // Short lists: the whole wave searches each one in turn.
+        unsigned long long shortMask = __ballot(lane < n && info.y > 0 && sl < 0);
+        while (shortMask) {
+            const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
+            shortMask &= shortMask - 1;
+            const uint32_t first = __shfl(info.x, j), count = __shfl(info.y, j);
+            uint32_t a, b;
+            WaveBounds(ix.post, first, first + count, base, end, a, b);
+            if (lane == j) {
+                lo = a;
+                hi = b;
+                ub = __uint_as_float(info.w);
+            }
+        }
+        if (lane < n)
+            wk.slices[static_cast<int64_t>(blocks) * w0 + static_cast<int64_t>(blk) * n + lane] = make_uint2(lo, hi);
+        if (thr > 0.0F) {
+            // Sum in word order on one lane so the bound rounds the way scores do.
+            const float contrib = hi > lo ? ub : 0.0F;
+            float bound = 0.0F;
+            for (int j = 0; j < n; ++j)
+                bound += __shfl(contrib, j);
+            if (lane == 0)
+                wk.bound[static_cast<int64_t>(q) * blocks + blk] = bound;
+        }
+    }
+}
+
+// The scoring kernel; persistent workgroups walk (query, block) tasks, every
+// kPilotStride-th block first so that the first round samples the whole corpus. Each task
+// keeps its block's top-K among docs scoring at least the query's lower bound tau and
+// counts them in the query's score histogram. tau is the larger of the best block K-th
+// score seen so far and the histogram bound (the K-th best kept score, to 1/16 octave),
+// so after the first round few docs qualify and most tasks keep nothing.
 template <int B>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void KScore(
-    IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t granulesPerBlock)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt,
+                                                                                          WorkView wk, int32_t blocks)
 {
     constexpr int kRows = B / (4 * kThreads); // float4 rows per thread
     constexpr int U = kLoadsPerThread;        // 256-posting columns per round
-    constexpr int kSampleCap = B / 2; // pilot keys sortable in the accumulator LDS
-    static_assert(kSampleCap <= kPilotCap, "pilot pool stride");
     __shared__ __align__(16) float acc[B];
     __shared__ uint32_t sLo[kMaxTerms], sPre[kMaxTerms + 1], sCol[kMaxTerms + 1];
-    __shared__ float sUb[kMaxTerms];
     __shared__ uint2 sRound[U];
     __shared__ uint32_t sRed[kWaves];
     __shared__ uint32_t sShared[4];
+    __shared__ unsigned long long sTop[kThreads]; // small exact selections (also per-thread maxima)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int32_t nq = bt.header[0];
-    const int32_t pilotBlocks = (blocks + kPilotStride - 1) / kPilotStride;
-    const int64_t pilotTasks = static_cast<int64_t>(nq) * pilotBlocks;
+    const int32_t firstBlocks = (blocks + kPilotStride - 1) / kPilotStride;
+    const int64_t firstTasks = static_cast<int64_t>(nq) * firstBlocks;
     const int64_t total = static_cast<int64_t>(nq) * blocks;
+    // Task order: blocks 0, s, 2s, ... for every query, then the other blocks.
+    auto blockOf = [&](int64_t t) {
+        if (t < firstTasks)
+            return static_cast<int>(t / nq) * kPilotStride;
+        const int64_t j = (t - firstTasks) / nq;
+        return static_cast<int>(kPilotStride * (j / (kPilotStride - 1)) + j % (kPilotStride - 1) + 1);
+    };
+    auto queryOf = [&](int64_t t) { return static_cast<int>((t < firstTasks ? t : t - firstTasks) % nq); };
+    // This lane's word slice of a task (lane j: word j of the task's query).
+    auto sliceOf = [&](int64_t t) {
+        uint2 v = make_uint2(0u, 0u);
+        if (t < total) {
+            const int tq = queryOf(t);
+            const int64_t tw0 = bt.qoff[tq];
+            const int tn = static_cast<int>(bt.qoff[tq + 1] - tw0);
+            if (lane < tn)
+                v = wk.slices[static_cast<int64_t>(blocks) * tw0 + static_cast<int64_t>(blockOf(t)) * tn + lane];
+        }
+        return v;
+    };
+    uint2 nextSlice = make_uint2(0u, 0u);
+    if (wave == 0)
+        nextSlice = sliceOf(blockIdx.x);
     unsigned long long scanned = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0, emitting = 0, exactTasks = 0;
+#ifdef BM25MI_PHASES
+    // Cycle counters of thread 0 per phase: setup, postings, selection, rest.
+    unsigned long long phase[4] = {0, 0, 0, 0};
+    unsigned long long stamp = clock64();
+#define BM25MI_PHASE(i)                                                                                                \
+    do {                                                                                                               \
+        if (tid == 0) {                                                                                                \
+            const unsigned long long now_ = clock64();                                                                 \
+            phase[i] += now_ - stamp;                                                                                  \
+            stamp = now_;                                                                                              \
+        }                                                                                                              \
+    } while (0)
+#else
+#define BM25MI_PHASE(i)                                                                                                \
+    do {                                                                                                               \
+    } while (0)
+#endif
 
     for (int64_t task = blockIdx.x; task < total; task += gridDim.x) {
-        int q, blk;
-        const bool isPilot = task < pilotTasks;
-        if (isPilot) {
-            q = static_cast<int>(task % nq);
-            blk = static_cast<int>(task / nq) * kPilotStride;
-        } else {
-            const int64_t t = task - pilotTasks;
-            q = static_cast<int>(t % nq);
-            const int64_t j = t / nq; // j-th block that is not a pilot block
-            blk = static_cast<int>(kPilotStride * (j / (kPilotStride - 1)) + j % (kPilotStride - 1) + 1);
-        }
+        const int q = queryOf(task);
+        const int blk = blockOf(task);
         const float thr = bt.thr[q];
-        const bool samples = isPilot && thr <= 0.0F; // a predicted threshold replaces the sample
         const int32_t base = blk * B;
-        const int32_t end = min(base + B, ix.ndocs);
         const int64_t w0 = bt.qoff[q];
         const int n = static_cast<int>(bt.qoff[q + 1] - w0);
         const int32_t k = bt.k[q];
         __syncthreads(); // previous task is done with LDS
 
-        // 1. Posting slice of every query word inside this block: block directory for
-        //    long lists, a two-round 64-ary wave search for short ones.
-        for (int j = wave; j < n; j += kWaves) {
-            const uint4 info = bt.words[w0 + j];
-            uint32_t lo = 0, hi = 0;
-            float ub = 0.0F;
-            if (info.y > 0) {
-                const int32_t sl = static_cast<int32_t>(info.z);
-                if (sl >= 0) {
-                    const uint32_t *d = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
-                    const int64_t g0 = static_cast<int64_t>(blk) * granulesPerBlock;
-                    const int64_t g1 = min(g0 + granulesPerBlock, ix.granules);
-                    lo = info.x + d[g0];
-                    hi = info.x + d[g1];
-                    if (thr > 0.0F && hi > lo) {
-                        float m = 0.0F;
-                        for (int64_t g = g0 + lane; g < g1; g += 64)
-                            m = fmaxf(m, ix.bmax[static_cast<int64_t>(sl) * ix.granules + g]);
-                        ub = __uint_as_float(WaveMax(__float_as_uint(m)));
-                    }
-                } else {
-                    WaveBounds(ix.post, info.x, info.x + info.y, base, end, lo, hi);
-                    ub = __uint_as_float(info.w);
-                }
-            }
-            if (lane == 0) {
-                sLo[j] = lo;
-                sPre[j + 1] = hi - lo; // length; turned into a prefix below
-                sUb[j] = ub;
-            }
-        }
-        float4 *acc4 = reinterpret_cast<float4 *>(acc);
-#pragma unroll
-        for (int r = 0; r < kRows; ++r)
-            acc4[r * kThreads + tid] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
-        __syncthreads();
         if (wave == 0) {
-            // Prefixes of slice lengths and of 256-posting column counts (one lane per word).
-            const uint32_t len = lane < n ? sPre[lane + 1] : 0u;
+            // This block's slice of each query word (lane j: word j; loaded during the previous
+            // task), then prefixes of slice lengths and of 256-posting column counts.
+            const uint2 sl = nextSlice;
+            nextSlice = sliceOf(task + gridDim.x);
+            const uint32_t len = sl.y - sl.x;
             uint32_t x = len, c = (len + kThreads - 1) / kThreads;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -267,119 +367,163 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 }
             }
             if (lane < n) {
+                sLo[lane] = sl.x;
                 sPre[lane + 1] = x;
                 sCol[lane + 1] = c;
             }
+            const uint32_t hb = WaveHistBound(wk.hist + static_cast<int64_t>(q) * kHistBins, static_cast<uint32_t>(k));
             if (lane == 0) {
                 sPre[0] = 0;
                 sCol[0] = 0;
-                sShared[0] = max(__atomic_load_n(&wk.tau[q], __ATOMIC_RELAXED), 1u);
-                sShared[2] = 1;
+                sShared[0] = max(max(__atomic_load_n(&wk.tau[q], __ATOMIC_RELAXED), hb), 1u);
+                sShared[2] = (thr > 0.0F && wk.bound[static_cast<int64_t>(q) * blocks + blk] < thr) ? 0u : 1u;
             }
         }
+        float4 *acc4 = reinterpret_cast<float4 *>(acc);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            acc4[r * kThreads + tid] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
         __syncthreads();
-        if (thr > 0.0F && tid == 0) {
-            // Upper bound of any score in the block: per-word maxima summed in word order.
-            float bound = 0.0F;
-            for (int j = 0; j < n; ++j)
-                if (sPre[j + 1] > sPre[j])
-                    bound += sUb[j];
-            sShared[2] = bound < thr ? 0u : 1u;
+        BM25MI_PHASE(0);
+        if (sShared[2] == 0) {
+            ++skipped; // the block's score upper bound is below the threshold
+            continue;
         }
-        if (thr > 0.0F)
-            __syncthreads();
-        const bool skip = sShared[2] == 0;
-        uint32_t tau = samples ? 1u : sShared[0];
+        ++scored;
+        uint32_t tau = sShared[0];
         if (thr > 0.0F)
             tau = max(tau, __float_as_uint(thr));
-        if (skip) {
-            ++skipped;
-        } else {
-            ++scored;
-            // 2. Term-ordered FP32 accumulation. Each word's slice is cut into 256-posting
-            //    columns; a round issues up to U columns' loads (one 8-byte load per thread
-            //    per column) before any add, then adds them column by column with a barrier
-            //    where the word changes. Within one word each doc appears once, so lanes never
-            //    collide, and every doc's score is summed in query-word order.
-            const uint32_t columns = sCol[n];
-            if (tid == 0)
-                scanned += sPre[n];
-            for (uint32_t r0 = 0; r0 < columns; r0 += U) {
-                if (wave == 0 && lane < U) {
-                    const uint32_t col = r0 + lane;
-                    uint2 entry = make_uint2(0u, 0xFFFF0000u);
-                    if (col < columns) {
-                        int j = 0;
-                        while (sCol[j + 1] <= col)
-                            ++j;
-                        const uint32_t i = col - sCol[j];
-                        const uint32_t len = sPre[j + 1] - sPre[j];
-                        entry = make_uint2(sLo[j] + i * kThreads,
-                                           min(static_cast<uint32_t>(kThreads), len - i * kThreads) |
-                                               (static_cast<uint32_t>(j) << 16));
-                    }
-                    sRound[lane] = entry;
-                }
-                __syncthreads();
-                uint2 e[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint2 c = sRound[u];
-                    if (static_cast<uint32_t>(tid) < (c.y & 0xFFFFu))
-                        e[u] = ix.post[c.x + tid];
-                }
-                // Unused columns carry word 0xFFFF and count 0: no barrier, no add. The
-                // barrier condition reads LDS values shared by the whole workgroup.
-                uint32_t word = sRound[0].y >> 16;
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t meta = sRound[u].y;
-                    const uint32_t w = meta >> 16;
-                    if (w != 0xFFFFu && w != word) {
-                        __syncthreads();
-                        word = w;
-                    }
-                    if (static_cast<uint32_t>(tid) < (meta & 0xFFFFu))
-                        acc[static_cast<int32_t>(e[u].x) - base] += __uint_as_float(e[u].y);
-                }
-                __syncthreads();
-            }
 
-            // 3. Block selection. Scores are re-read from LDS in each pass (8 x 16-byte
-            //    reads per thread) so they do not occupy registers across the passes.
-            const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
-            auto countIf = [&](auto pred) {
-                uint32_t c = 0;
-#pragma unroll
-                for (int r = 0; r < kRows; ++r) {
-                    const uint4 v = bits4[r * kThreads + tid];
-                    const uint32_t e = r * 4 * kThreads + tid * 4;
-                    c += (pred(v.x, e) ? 1u : 0u) + (pred(v.y, e + 1) ? 1u : 0u) + (pred(v.z, e + 2) ? 1u : 0u) +
-                         (pred(v.w, e + 3) ? 1u : 0u);
+        // Term-ordered FP32 accumulation. Each word's slice is cut into 256-posting
+        // columns; a round issues up to U columns' loads (one 8-byte load per thread per
+        // column) before any add, then adds them column by column with a barrier where the
+        // word changes. Within one word each doc appears once, so lanes never collide, and
+        // every doc's score is summed in query-word order.
+        const uint32_t columns = sCol[n];
+        if (tid == 0)
+            scanned += sPre[n];
+        for (uint32_t r0 = 0; r0 < columns; r0 += U) {
+            if (wave == 0 && lane < U) {
+                const uint32_t col = r0 + lane;
+                uint2 entry = make_uint2(0u, 0xFFFF0000u);
+                if (col < columns) {
+                    int w = 0;
+                    while (sCol[w + 1] <= col)
+                        ++w;
+                    const uint32_t i = col - sCol[w];
+                    const uint32_t len = sPre[w + 1] - sPre[w];
+                    entry = make_uint2(sLo[w] + i * kThreads, min(static_cast<uint32_t>(kThreads), len - i * kThreads) |
+                                                                  (static_cast<uint32_t>(w) << 16));
                 }
-                return c;
-            };
-            auto countGe = [&](uint32_t x) {
-                return BlockSum<kThreads>(countIf([x](uint32_t b, uint32_t) { return b >= x; }), sRed);
-            };
-            // Cut for the block's top-kk among docs with bits >= floor: keep bits > cut, or
-            // bits == cut at local index <= tieIdx. Returns false when nothing qualifies.
-            auto selectCut = [&](uint32_t kk, uint32_t floor, uint32_t &cut, uint32_t &tieIdx, bool &exact) {
-                const uint32_t candidates = countGe(floor);
-                cut = floor;
-                tieIdx = B;
-                exact = false;
-                if (candidates == 0)
-                    return false;
-                if (candidates <= kk)
-                    return true;
-                // kk-th largest score bits by bisection: count(>= lo) >= kk > count(>= hi).
-                uint32_t mx = 0;
+                sRound[lane] = entry;
+            }
+            __syncthreads();
+            uint2 e[U];
 #pragma unroll
-                for (int r = 0; r < kRows; ++r) {
-                    const uint4 v = bits4[r * kThreads + tid];
-                    mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
+            for (int u = 0; u < U; ++u) {
+                const uint2 c = sRound[u];
+                if (static_cast<uint32_t>(tid) < (c.y & 0xFFFFu))
+                    e[u] = ix.post[c.x + tid];
+            }
+            // Unused columns carry word 0xFFFF and count 0: no barrier, no add. The barrier
+            // condition reads LDS values shared by the whole workgroup.
+            uint32_t word = sRound[0].y >> 16;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t meta = sRound[u].y;
+                const uint32_t w = meta >> 16;
+                if (w != 0xFFFFu && w != word) {
+                    __syncthreads();
+                    word = w;
                 }
+                if (static_cast<uint32_t>(tid) < (meta & 0xFFFFu))
+                    acc[static_cast<int32_t>(e[u].x) - base] += __uint_as_float(e[u].y);
+            }
+            __syncthreads();
+        }
+        BM25MI_PHASE(1);
+
+        // Block selection. Scores are re-read from LDS in each pass (8 x 16-byte reads per
+        // thread) so they do not occupy registers across the passes.
+        const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
+        auto countIf = [&](auto pred) {
+            uint32_t c = 0;
+#pragma unroll 2
+            for (int r = 0; r < kRows; ++r) {
+                const uint4 v = bits4[r * kThreads + tid];
+                const uint32_t e = r * 4 * kThreads + tid * 4;
+                c += (pred(v.x, e) ? 1u : 0u) + (pred(v.y, e + 1) ? 1u : 0u) + (pred(v.z, e + 2) ? 1u : 0u) +
+                     (pred(v.w, e + 3) ? 1u : 0u);
+            }
+            return c;
+        };
+        auto countGe = [&](uint32_t x) {
+            return BlockSum<kThreads>(countIf([x](uint32_t b, uint32_t) { return b >= x; }), sRed);
+        };
+        auto keyOf = [base](uint32_t b, uint32_t e) {
+            return (static_cast<unsigned long long>(b) << 32) |
+                   static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e));
+        };
+        unsigned long long *pool = wk.pool + bt.poolOff[q];
+        uint32_t *hist = wk.hist + static_cast<int64_t>(q) * kHistBins;
+        const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
+        const uint32_t kk = static_cast<uint32_t>(k);
+        uint32_t floor = tau;
+        uint32_t mine = 0, mx = 0;
+#pragma unroll 2
+        for (int r = 0; r < kRows; ++r) {
+            const uint4 v = bits4[r * kThreads + tid];
+            mine += (v.x >= floor ? 1u : 0u) + (v.y >= floor ? 1u : 0u) + (v.z >= floor ? 1u : 0u) +
+                    (v.w >= floor ? 1u : 0u);
+            mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
+        }
+        uint32_t candidates = BlockSum<kThreads>(mine, sRed);
+        if (candidates > 0) {
+            uint32_t cut = floor, tieIdx = B; // keep bits > cut, or bits == cut at index <= tieIdx
+            uint32_t kth = 0;                 // the block's K-th score bits when it has more than K
+            bool sorted = false;
+            if (candidates > kk && kk <= static_cast<uint32_t>(kThreads)) {
+                if (candidates > static_cast<uint32_t>(kThreads)) {
+                    // Raise the floor to the kk-th largest per-thread maximum: kk docs reach
+                    // it, so the block's kk-th score is at least that.
+                    sTop[tid] = mx;
+                    __syncthreads();
+                    BitonicDesc<kThreads>(sTop, kThreads);
+                    floor = max(floor, static_cast<uint32_t>(sTop[kk - 1]));
+                    __syncthreads();
+                    mine = countIf([floor](uint32_t b, uint32_t) { return b >= floor; });
+                    candidates = BlockSum<kThreads>(mine, sRed);
+                }
+                if (candidates <= static_cast<uint32_t>(kThreads)) {
+                    // Few enough to sort: compact their composite keys and sort them; the
+                    // first kk are the block's top-kk.
+                    uint32_t count = 0;
+                    uint32_t at = BlockExclusive<kThreads>(mine, sRed, &count);
+#pragma unroll 1
+                    for (int r = 0; r < kRows; ++r) {
+                        const uint4 v = bits4[r * kThreads + tid];
+                        const uint32_t e = r * 4 * kThreads + tid * 4;
+                        if (v.x >= floor)
+                            sTop[at++] = keyOf(v.x, e);
+                        if (v.y >= floor)
+                            sTop[at++] = keyOf(v.y, e + 1);
+                        if (v.z >= floor)
+                            sTop[at++] = keyOf(v.z, e + 2);
+                        if (v.w >= floor)
+                            sTop[at++] = keyOf(v.w, e + 3);
+                    }
+                    __syncthreads();
+                    if (static_cast<uint32_t>(tid) >= count)
+                        sTop[tid] = 0ULL;
+                    __syncthreads();
+                    BitonicDesc<kThreads>(sTop, kThreads);
+                    kth = static_cast<uint32_t>(sTop[kk - 1] >> 32);
+                    sorted = true;
+                }
+            }
+            if (!sorted && candidates > kk) {
+                // General exact cut: kk-th largest score bits by bisection,
+                // count(>= lo) >= kk > count(>= hi), then ties by lowest doc ID.
                 mx = BlockMax<kThreads>(mx, sRed);
                 uint32_t lo = floor, hi = mx + 1;
                 while (hi - lo > 1) {
@@ -390,7 +534,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                         hi = mid;
                 }
                 cut = lo;
-                exact = true;
+                kth = cut;
                 const uint32_t need = kk - countGe(lo + 1);
                 const uint32_t c0 = cut;
                 const uint32_t ties = BlockSum<kThreads>(countIf([c0](uint32_t b, uint32_t) { return b == c0; }), sRed);
@@ -409,100 +553,67 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     }
                     tieIdx = static_cast<uint32_t>(z);
                 }
-                return true;
-            };
-            // Appends the kept docs to pool[0, cap) through *counter; returns how many.
-            auto emit = [&](uint32_t cut, uint32_t tieIdx, unsigned long long *pool, uint32_t *counter, int64_t cap) {
-                auto keepIt = [cut, tieIdx](uint32_t b, uint32_t e) { return b > cut || (b == cut && e <= tieIdx); };
-                const uint32_t mine = countIf(keepIt);
-                uint32_t kept = 0;
-                const uint32_t before = BlockExclusive<kThreads>(mine, sRed, &kept);
+            }
+            // Append the kept docs and count them in the histogram.
+            uint32_t kept;
+            if (sorted) {
+                kept = kk;
                 if (tid == 0)
-                    sShared[1] = atomicAdd(counter, kept);
+                    sShared[1] = atomicAdd(&wk.count[q], kk);
+                __syncthreads();
+                if (static_cast<uint32_t>(tid) < kk) {
+                    const uint32_t slot = sShared[1] + tid;
+                    if (static_cast<int64_t>(slot) < cap)
+                        pool[slot] = sTop[tid];
+                    else
+                        ++overflow;
+                    atomicAdd(hist + HistBucket(static_cast<uint32_t>(sTop[tid] >> 32)), 1u);
+                }
+            } else {
+                auto keepIt = [cut, tieIdx](uint32_t b, uint32_t e) { return b > cut || (b == cut && e <= tieIdx); };
+                mine = countIf(keepIt);
+                uint32_t before = BlockExclusive<kThreads>(mine, sRed, &kept);
+                if (tid == 0)
+                    sShared[1] = atomicAdd(&wk.count[q], kept);
                 __syncthreads();
                 if (mine > 0) {
                     uint32_t slot = sShared[1] + before;
-#pragma unroll
+#pragma unroll 1
                     for (int r = 0; r < kRows; ++r) {
                         const uint4 v = bits4[r * kThreads + tid];
+                        const uint32_t e = r * 4 * kThreads + tid * 4;
                         const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                        for (int cc = 0; cc < 4; ++cc) {
-                            const uint32_t e = r * 4 * kThreads + tid * 4 + cc;
-                            if (keepIt(vals[cc], e)) {
+                        for (int cc = 0; cc < 4; ++cc)
+                            if (keepIt(vals[cc], e + cc)) {
                                 if (static_cast<int64_t>(slot) < cap)
-                                    pool[slot] = (static_cast<unsigned long long>(vals[cc]) << 32) |
-                                                 static_cast<unsigned long long>(0xFFFFFFFFu -
-                                                                                 (static_cast<uint32_t>(base) + e));
+                                    pool[slot] = keyOf(vals[cc], e + cc);
                                 else
                                     ++overflow;
                                 ++slot;
+                                atomicAdd(hist + HistBucket(vals[cc]), 1u);
                             }
-                        }
                     }
                 }
-                return kept;
-            };
-
-            uint32_t cut, tieIdx;
-            bool exact;
-            if (selectCut(static_cast<uint32_t>(k), tau, cut, tieIdx, exact)) {
-                const uint32_t kept =
-                    emit(cut, tieIdx, wk.pool + bt.poolOff[q], &wk.count[q], bt.poolOff[q + 1] - bt.poolOff[q]);
-                if (exact) {
-                    ++exactTasks;
-                    if (tid == 0)
-                        atomicMax(&wk.tau[q], cut); // the block's K-th score bounds the query's
-                }
-                if (tid == 0) {
-                    emitted += kept;
-                    ++emitting;
-                }
             }
-            if (samples) {
-                // The sample keeps the block's top-K' (the same cut as above when K' == K).
-                const int32_t kp = PilotK(k, pilotBlocks, kSampleCap);
-                if (kp > 0) {
-                    if (kp < k)
-                        selectCut(static_cast<uint32_t>(kp), 1u, cut, tieIdx, exact);
-                    emit(cut, tieIdx, wk.pilot + static_cast<int64_t>(q) * kPilotCap, &wk.pilotCount[q],
-                         static_cast<int64_t>(pilotBlocks) * kp);
-                }
+            if (kth) {
+                ++exactTasks;
+                if (tid == 0)
+                    atomicMax(&wk.tau[q], kth); // the block's K-th score bounds the query's
+            }
+            if (tid == 0 && kept) {
+                emitted += kept;
+                ++emitting;
             }
         }
-        if (samples) {
-            // Publish this pilot task; the last one of the query turns the sample into tau.
-            __threadfence();
-            __syncthreads();
-            if (tid == 0)
-                sShared[3] = atomicAdd(&wk.pilotDone[q], 1u);
-            __syncthreads();
-            if (sShared[3] == static_cast<uint32_t>(pilotBlocks - 1)) {
-                __threadfence();
-                const int32_t kp = PilotK(k, pilotBlocks, kSampleCap);
-                const uint32_t p = min(__atomic_load_n(&wk.pilotCount[q], __ATOMIC_RELAXED),
-                                       static_cast<uint32_t>(pilotBlocks * kp));
-                if (p >= static_cast<uint32_t>(k)) {
-                    unsigned long long *sorted = reinterpret_cast<unsigned long long *>(acc);
-                    const unsigned long long *sample = wk.pilot + static_cast<int64_t>(q) * kPilotCap;
-                    int n2 = 1;
-                    while (n2 < static_cast<int>(p))
-                        n2 <<= 1;
-                    __syncthreads();
-                    for (int i = tid; i < n2; i += kThreads)
-                        sorted[i] = i < static_cast<int>(p) ? sample[i] : 0ULL;
-                    __syncthreads();
-                    BitonicDesc<kThreads>(sorted, n2);
-                    if (tid == 0)
-                        atomicMax(&wk.tau[q], max(1u, static_cast<uint32_t>(sorted[k - 1] >> 32)));
-                }
-                if (tid == 0) {
-                    wk.pilotCount[q] = 0;
-                    wk.pilotDone[q] = 0;
-                }
-            }
-        }
+        BM25MI_PHASE(2);
     }
+#ifdef BM25MI_PHASES
+    if (tid == 0)
+        for (int i = 0; i < 3; ++i)
+            atomicAdd(&wk.stats[kStatPhase + i], phase[i]);
+#endif
+#undef BM25MI_PHASE
     if (tid == 0) {
         atomicAdd(&wk.stats[kStatPostings], scanned);
         atomicAdd(&wk.stats[kStatScored], scored);
@@ -519,13 +630,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 // the scoring and final kernels restore it for the next Search.
 __global__ void KReset(WorkView wk, int32_t maxQ)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < maxQ) {
         wk.count[i] = 0;
-        wk.pilotCount[i] = 0;
-        wk.pilotDone[i] = 0;
         wk.tau[i] = 1;
     }
+    if (i < static_cast<int64_t>(maxQ) * kHistBins)
+        wk.hist[i] = 0;
     if (i < kStats)
         wk.stats[i] = 0;
 }
@@ -569,17 +680,29 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     const uint32_t k = static_cast<uint32_t>(bt.k[q]);
     const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
     const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(wk.count[q]), cap));
-    const uint32_t tau = wk.tau[q];
+    uint32_t *hist = wk.hist + static_cast<int64_t>(q) * kHistBins;
+    if (tid < 64)
+        sRed[0] = 0; // placeholder write order: the bound below is computed by wave 0
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t hb = WaveHistBound(hist, k);
+        if (tid == 0)
+            sRed[0] = max(max(wk.tau[q], hb), 1u);
+    }
+    __syncthreads();
+    const uint32_t tau = sRed[0];
     const unsigned long long *pool = wk.pool + bt.poolOff[q];
     const bool pruned = bt.thr[q] > 0.0F;
     if (tid == 0)
         sN = 0;
     __syncthreads();
+    // Restore the per-query state for the next Search.
     if (tid == 0) {
-        // Restore the per-query counters for the next Search.
         wk.count[q] = 0;
         wk.tau[q] = 1;
     }
+    for (int i = tid; i < kHistBins; i += kFinalThreads)
+        hist[i] = 0;
     // Keep entries with score bits >= tau.
     for (uint32_t i0 = 0; i0 < p; i0 += kFinalThreads) {
         const uint32_t i = i0 + tid;
@@ -664,26 +787,54 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 {
     const int threads = 256;
-    const int grid = (max(maxQ, kStats) + threads - 1) / threads;
+    const int grid = static_cast<int>((max(static_cast<int64_t>(maxQ) * kHistBins, static_cast<int64_t>(kStats)) +
+                                       threads - 1) / threads);
     hipLaunchKernelGGL(KReset, dim3(grid), dim3(threads), 0, stream, wk, maxQ);
 }
 
+void LaunchPlan(const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks, int32_t blockDocs,
+                int32_t granulesPerBlock, int32_t grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL(KPlan, dim3(grid), dim3(256), 0, stream, ix, bt, wk, blocks, blockDocs, granulesPerBlock);
+}
+
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t granulesPerBlock, int32_t grid, hipStream_t stream)
+                 int32_t grid, hipStream_t stream)
 {
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks, granulesPerBlock);
+        hipLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks, granulesPerBlock);
+        hipLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks, granulesPerBlock);
+        hipLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks);
         return true;
     default:
         return false;
     }
+}
+
+// Resident scoring workgroups per CU for a block size (registers and LDS), or 0.
+int ScoreBlocksPerCu(int blockDocs)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (blockDocs) {
+    case 2048:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<2048>, kThreads, 0);
+        break;
+    case 4096:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<4096>, kThreads, 0);
+        break;
+    case 8192:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<8192>, kThreads, 0);
+        break;
+    default:
+        break;
+    }
+    return e == hipSuccess ? n : 0;
 }
 
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,

@@ -30,8 +30,11 @@
 namespace bm25mi {
 
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
+void LaunchPlan(const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks, int32_t blockDocs,
+                int32_t granulesPerBlock, int32_t grid, hipStream_t stream);
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t granulesPerBlock, int32_t grid, hipStream_t stream);
+                 int32_t grid, hipStream_t stream);
+int ScoreBlocksPerCu(int blockDocs);
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
                  hipStream_t stream);
 
@@ -67,7 +70,7 @@ public:
     // Device memory.
     void *dPost = nullptr, *dToff = nullptr, *dSlot = nullptr, *dDir = nullptr, *dBmax = nullptr, *dTmax = nullptr;
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dPilot = nullptr, *dPilotCount = nullptr, *dPilotDone = nullptr;
+    void *dHist = nullptr, *dSlices = nullptr, *dBound = nullptr;
     // Pinned host memory: request packet (copied by the graph) and result block (written by the device).
     void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     size_t packetBytes = 0, resultBytes = 0;
@@ -94,6 +97,12 @@ public:
     ~Session() { Release(); }
 
     void Release() noexcept;
+    // Candidate pool entries one query may need: every block keeps at most max(K, kEmitCap)
+    // docs, and no doc is kept twice.
+    int64_t PoolCapacity(int32_t k) const
+    {
+        return std::min<int64_t>(static_cast<int64_t>(blocks) * std::max<int32_t>(k, kEmitCap), ndocs);
+    }
     void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
     void Run(bool timed);
     void Bootstrap();
@@ -146,13 +155,13 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     hipDeviceProp_t prop;
     BM25MI_HIP(hipGetDeviceProperties(&prop, device), "device properties");
     cus = prop.multiProcessorCount;
-    const int ldsPerWg = blockDocs * 4 + 1024;
-    const int perCu = std::max(1, std::min(8, static_cast<int>(prop.maxSharedMemoryPerMultiProcessor) / ldsPerWg));
+    const int perCu = ScoreBlocksPerCu(blockDocs);
+    Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
 
     size_t freeBytes = 0, totalBytes = 0;
     BM25MI_HIP(hipMemGetInfo(&freeBytes, &totalBytes), "memory info");
-    poolPerQuery = std::min<int64_t>(static_cast<int64_t>(blocks) * cfg.max_k, n);
+    poolPerQuery = PoolCapacity(cfg.max_k);
     const uint64_t known = static_cast<uint64_t>(m) * 8 + (v + 1) * 8 + v * 8 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
                            static_cast<uint64_t>(cfg.max_queries) * poolPerQuery * 8;
@@ -223,9 +232,9 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
 
     dPool = Alloc<unsigned long long>(q * poolPerQuery, "alloc candidate pools");
     dCount = Alloc<uint32_t>(q, "alloc pool counts");
-    dPilot = Alloc<unsigned long long>(q * kPilotCap, "alloc pilot samples");
-    dPilotCount = Alloc<uint32_t>(q, "alloc pilot counts");
-    dPilotDone = Alloc<uint32_t>(q, "alloc pilot completion counters");
+    dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
+    dSlices = Alloc<uint2>(static_cast<size_t>(blocks) * std::max<int64_t>(1, cfg.max_words), "alloc slice table");
+    dBound = Alloc<float>(q * blocks, "alloc block bounds");
     dTau = Alloc<uint32_t>(q, "alloc thresholds");
     dStats = Alloc<unsigned long long>(kStats, "alloc stats");
 
@@ -240,11 +249,11 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
                    reinterpret_cast<const uint4 *>(pk + offTerms)};
     auto *rs = static_cast<unsigned char *>(hResultDev);
     wk = WorkView{static_cast<unsigned long long *>(dPool), static_cast<uint32_t *>(dCount),
-                  static_cast<unsigned long long *>(dPilot), static_cast<uint32_t *>(dPilotCount),
-                  static_cast<uint32_t *>(dPilotDone),
+                  static_cast<uint32_t *>(dHist),
                   static_cast<uint32_t *>(dTau),            static_cast<unsigned long long *>(dStats),
                   reinterpret_cast<int32_t *>(rs + resStatus), reinterpret_cast<float *>(rs + resScores),
-                  reinterpret_cast<int32_t *>(rs + resIds)};
+                  reinterpret_cast<int32_t *>(rs + resIds), static_cast<uint2 *>(dSlices),
+                  static_cast<float *>(dBound)};
     LaunchInit(wk, cfg.max_queries, stream);
     BM25MI_HIP(hipStreamSynchronize(stream), "initialise counters");
     if (cfg.use_graph)
@@ -256,8 +265,11 @@ void Session::Enqueue(bool timed)
 {
     BM25MI_HIP(hipMemcpyAsync(dPacket, hPacket, packetBytes, hipMemcpyHostToDevice, stream), "packet H2D");
     if (timed)
+        BM25MI_HIP(hipEventRecord(ev[4], stream), "record plan start");
+    LaunchPlan(ix, bt, wk, blocks, blockDocs, granulesPerBlock, cus * 8, stream);
+    if (timed)
         BM25MI_HIP(hipEventRecord(ev[1], stream), "record score start");
-    Require(LaunchScore(blockDocs, ix, bt, wk, blocks, granulesPerBlock, grid, stream), "Unsupported block size");
+    Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, stream), "Unsupported block size");
     if (timed)
         BM25MI_HIP(hipEventRecord(ev[2], stream), "record score end");
     LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(hResultDev), stream);
@@ -295,7 +307,8 @@ void Session::Run(bool timed)
     if (timed) {
         BM25MI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]), "score kernel time");
         last.score_kernel_ms = ms;
-        last.pilot_ms = 0;
+        BM25MI_HIP(hipEventElapsedTime(&ms, ev[4], ev[1]), "plan time");
+        last.pilot_ms = ms;
     } else {
         last.score_kernel_ms = -1;
         last.pilot_ms = -1;
@@ -336,7 +349,7 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
         qoff[q + 1] = query.offsets[q + 1];
         kk[q] = query.k[q];
         thr[q] = query.thr ? query.thr[q] : 0.0F;
-        pool[q + 1] = pool[q] + std::min<int64_t>(static_cast<int64_t>(blocks) * query.k[q], ndocs);
+        pool[q + 1] = pool[q] + PoolCapacity(query.k[q]);
         out[q + 1] = out[q] + query.k[q];
     }
     auto *info = reinterpret_cast<uint4 *>(pk + offTerms);
@@ -380,7 +393,9 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
             std::memcpy(outScores + out[q], sc + out[q], (out[q + 1] - out[q]) * 4);
             std::memcpy(outIds + out[q], id + out[q], (out[q + 1] - out[q]) * 4);
         }
-        last.postings_scanned = static_cast<int64_t>(stats[kStatPostings]);
+        last.postings_scanned = static_cast<int64_t>(stats[kStatPostings] + stats[kStatPilot]);
+        for (int i = 0; i < 8; ++i)
+            last.phase_cycles[i] = static_cast<int64_t>(stats[kStatPhase + i]);
         last.pilot_postings = static_cast<int64_t>(stats[kStatPilot]);
         last.emitting_blocks = static_cast<int64_t>(stats[kStatEmitting]);
         last.exact_blocks = static_cast<int64_t>(stats[kStatExact]);
@@ -429,7 +444,7 @@ void Session::Release() noexcept
         stream = nullptr;
     }
     for (void **p : {&dPost, &dToff, &dSlot, &dDir, &dBmax, &dTmax, &dPacket, &dPool, &dCount, &dTau, &dStats,
-                     &dPilot, &dPilotCount, &dPilotDone})
+                     &dHist, &dSlices, &dBound})
         if (*p) {
             (void)hipFree(*p);
             *p = nullptr;

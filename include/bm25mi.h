@@ -63,7 +63,7 @@ typedef struct {
     int32_t block_docs;      /* docs per scoring block: 2048, 4096, 8192 or 16384; 0 = 8192 */
     double k1;               /* BM25 k1; impact = idf*(tf*(k1+1)) / (tf+norm) in FP32 */
     int32_t use_graph;       /* 1: replay one captured hipGraph per Search (default) */
-    int32_t grid_blocks;     /* persistent scoring workgroups; 0 = 4 per CU */
+    int32_t grid_blocks;     /* persistent scoring workgroups; 0 = as many as fit the CUs */
 } bm25mi_config;
 
 /* Per-Search counters, filled by bm25mi_last_stats. */
@@ -71,17 +71,19 @@ typedef struct {
     double device_ms;          /* HIP-event time of the device work of the last Search */
     double score_kernel_ms;    /* HIP-event time of the scoring kernel alone */
     double host_ms;            /* wall time of the whole bm25mi_search* call */
-    int64_t postings_scanned;  /* postings streamed from HBM */
+    int64_t postings_scanned;  /* postings streamed from HBM (pilot and main passes) */
     int64_t postings_total;    /* postings of the query terms (plain-path work) */
     int64_t blocks_scored;     /* (query, block) tasks scored */
     int64_t blocks_skipped;    /* (query, block) tasks skipped by a threshold */
     int64_t candidates;        /* entries written to the candidate pools */
     int32_t fallbacks;         /* queries rerun on the plain path */
     int32_t queries;
-    double pilot_ms;           /* HIP-event time of the pilot pass + threshold (timed Search only) */
+    double pilot_ms;           /* HIP-event time of planning, pilot pass and threshold (timed Search only) */
     int64_t pilot_postings;    /* postings streamed by the pilot pass */
     int64_t emitting_blocks;   /* scoring tasks that appended candidates */
-    int64_t exact_blocks;      /* tasks that needed the exact block K-th-score bisection */
+    int64_t exact_blocks;      /* tasks that needed an exact block top-K selection */
+    int64_t phase_cycles[8];   /* scoring cycles per phase (setup, postings, selection, sample) of the
+                                  main then the pilot pass; zero unless built with -DBM25MI_PHASES */
 } bm25mi_stats;
 
 void bm25mi_config_default(bm25mi_config *cfg);
