@@ -8,6 +8,8 @@ namespace bm25mi {
 constexpr int kThreads = 256;              // scoring workgroup: 4 wave64
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxTerms = 64;              // terms per query (fixed_query_packet.h:32)
+constexpr int kMaxSegs = 64;               // posting segments per (query, block) task
+constexpr int kGranule = 256;              // docs per directory / block-max granule
 constexpr int kLongMin = 4096;             // lists this long use the block directory; shorter
                                            // ones a two-round 64-ary in-kernel search (64*64)
 constexpr int kFinalThreads = 1024;        // selection workgroup (threshold and final kernels)
@@ -15,10 +17,10 @@ constexpr int kSortCap = 16384;            // pool entries a selection workgroup
 constexpr int kPilotStride = 16;           // every 16th block is scored first (spreads the first round)
 constexpr int kEmitCap = 512;              // pool entries per block reserved beyond K (sizing only)
 // Score histogram of kept candidates, per query: bucket = (score bits >> 19) - kHistBase,
-// i.e. 16 buckets per octave from 2^-10 up (lower scores fall in bucket 0, higher ones in
-// the last bucket).
-constexpr int kHistBins = 320;
-constexpr uint32_t kHistBase = (127u - 10u) << 4;
+// i.e. 16 buckets per octave over [2^-4, 2^8) (lower scores fall in bucket 0, higher ones
+// in the last bucket, which only makes the bound it yields smaller).
+constexpr int kHistBins = 192;
+constexpr uint32_t kHistBase = (127u - 4u) << 4;
 #ifndef BM25MI_LOADS
 #define BM25MI_LOADS 8
 #endif
@@ -78,8 +80,11 @@ struct WorkView {
     int32_t *ids;             // [maxQ*maxK]
     // Posting slice {first, end} of every (query word, block), written by the planning
     // kernel: entry blocks * qoff[q] + blk * n_q + j for word j of query q.
-    uint2 *slices;            // [blocks * maxWords]
-    float *bound;             // [maxQ * blocks] block upper bound (threshold queries only)
+    // Posting segments of every (query, block) task, written by the planning kernel:
+    // task (q, blk) owns segs[(q * blocks + blk) * kMaxSegs ...], {first posting,
+    // (count << 8) | word}, in word order; segCount[q * blocks + blk] of them.
+    uint2 *segs;
+    uint32_t *segCount;
 };
 
 } // namespace bm25mi

@@ -211,10 +211,15 @@ __device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t
     return top ? HistEdge(top - 1) : 1u;
 }
 
-// Planning kernel, one wave per (query, block) task: the posting slice {first, end} of
-// every query word inside the block (block directory for long lists, a two-round 64-ary
-// search for short ones), and for threshold queries the block's score upper bound
-// (per-word block maxima summed in word order).
+// Planning kernel, one wave per (query, block) task. Finds the posting slice of every
+// query word inside the block (block directory for long lists, a two-round 64-ary search
+// for short ones) and writes the task's posting segments:
+// * plain queries: one segment per word (its block slice);
+// * threshold queries: an upper bound of every 256-doc granule's scores (per-word granule
+//   maxima summed in word order; short words count their largest impact), then per word
+//   only the runs of granules whose bound reaches the threshold. Docs in other granules
+//   can only collect partial sums below the threshold, so they are never kept. A task with
+//   no such granule gets no segment; one needing more than kMaxSegs keeps whole slices.
 __global__ __launch_bounds__(256) void KPlan(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t blockDocs,
                                              int32_t granulesPerBlock)
 {
@@ -231,11 +236,12 @@ __global__ __launch_bounds__(256) void KPlan(IndexView ix, BatchView bt, WorkVie
         const int64_t w0 = bt.qoff[q];
         const int n = static_cast<int>(bt.qoff[q + 1] - w0);
         const float thr = bt.thr[q];
+        const int64_t slotOut = static_cast<int64_t>(q) * blocks + blk;
+        uint2 *segs = wk.segs + slotOut * kMaxSegs;
         uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
         if (lane < n)
             info = bt.words[w0 + lane];
         uint32_t lo = 0, hi = 0;
-        float ub = 0.0F;
         const int32_t sl = static_cast<int32_t>(info.z);
         const int64_t g0 = static_cast<int64_t>(blk) * granulesPerBlock;
         const int64_t g1 = min(g0 + granulesPerBlock, ix.granules);
@@ -243,9 +249,6 @@ __global__ __launch_bounds__(256) void KPlan(IndexView ix, BatchView bt, WorkVie
             const uint32_t *d = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
             lo = info.x + d[g0];
             hi = info.x + d[g1];
-            if (thr > 0.0F && hi > lo)
-                for (int64_t g = g0; g < g1; ++g)
-                    ub = fmaxf(ub, ix.bmax[static_cast<int64_t>(sl) * ix.granules + g]);
         }
         // Short lists: the whole wave searches each one in turn.
         unsigned long long shortMask = __ballot(lane < n && info.y > 0 && sl < 0);
@@ -258,20 +261,76 @@ __global__ __launch_bounds__(256) void KPlan(IndexView ix, BatchView bt, WorkVie
             if (lane == j) {
                 lo = a;
                 hi = b;
-                ub = __uint_as_float(info.w);
             }
         }
-        if (lane < n)
-            wk.slices[static_cast<int64_t>(blocks) * w0 + static_cast<int64_t>(blk) * n + lane] = make_uint2(lo, hi);
-        if (thr > 0.0F) {
-            // Sum in word order on one lane so the bound rounds the way scores do.
-            const float contrib = hi > lo ? ub : 0.0F;
-            float bound = 0.0F;
-            for (int j = 0; j < n; ++j)
-                bound += __shfl(contrib, j);
+        if (thr <= 0.0F) {
+            if (lane < n)
+                segs[lane] = make_uint2(lo, ((hi - lo) << 8) | static_cast<uint32_t>(lane));
             if (lane == 0)
-                wk.bound[static_cast<int64_t>(q) * blocks + blk] = bound;
+                wk.segCount[slotOut] = static_cast<uint32_t>(n);
+            continue;
         }
+        // Granule bounds (lane g: granule g0 + g), summed in word order.
+        const int gpb = static_cast<int>(g1 - g0);
+        float ub = 0.0F;
+        for (int j = 0; j < n; ++j) {
+            const uint32_t jlo = __shfl(lo, j), jhi = __shfl(hi, j);
+            const int32_t jsl = static_cast<int32_t>(__shfl(info.z, j));
+            const float jmax = __uint_as_float(__shfl(info.w, j));
+            if (jhi > jlo && lane < gpb)
+                ub += jsl >= 0 ? ix.bmax[static_cast<int64_t>(jsl) * ix.granules + g0 + lane] : jmax;
+        }
+        const unsigned long long active = __ballot(lane < gpb && ub >= thr);
+        uint32_t nseg = 0;
+        if (active) {
+            // Runs of active granules: [start bit, first inactive bit after it).
+            bool overflow = false;
+            for (int j = 0; j < n && !overflow; ++j) {
+                const uint32_t jlo = __shfl(lo, j), jhi = __shfl(hi, j);
+                if (jhi <= jlo)
+                    continue;
+                const int32_t jsl = static_cast<int32_t>(__shfl(info.z, j));
+                if (jsl < 0) {
+                    // Short word: its whole block slice.
+                    if (nseg >= static_cast<uint32_t>(kMaxSegs)) {
+                        overflow = true;
+                        break;
+                    }
+                    if (lane == 0)
+                        segs[nseg] = make_uint2(jlo, ((jhi - jlo) << 8) | static_cast<uint32_t>(j));
+                    ++nseg;
+                    continue;
+                }
+                const uint32_t *d = ix.dir + static_cast<int64_t>(jsl) * (ix.granules + 1);
+                const uint32_t first = __shfl(info.x, j);
+                const uint32_t dv = lane <= gpb ? d[g0 + lane] : 0u; // lane g: postings before granule g
+                unsigned long long runs = active & ~(active << 1);      // run starts
+                while (runs) {
+                    const int ga = __ffsll(static_cast<long long>(runs)) - 1;
+                    runs &= runs - 1;
+                    const unsigned long long after = ~active & (~0ULL << ga);
+                    const int gb = after ? min(__ffsll(static_cast<long long>(after)) - 1, gpb) : gpb;
+                    const uint32_t s0 = first + __shfl(dv, ga), s1 = first + __shfl(dv, gb);
+                    if (s1 > s0) {
+                        if (nseg >= static_cast<uint32_t>(kMaxSegs)) {
+                            overflow = true;
+                            break;
+                        }
+                        if (lane == 0)
+                            segs[nseg] = make_uint2(s0, ((s1 - s0) << 8) | static_cast<uint32_t>(j));
+                        ++nseg;
+                    }
+                }
+            }
+            if (overflow) {
+                // Too fragmented: whole block slices instead.
+                if (lane < n)
+                    segs[lane] = make_uint2(lo, ((hi - lo) << 8) | static_cast<uint32_t>(lane));
+                nseg = static_cast<uint32_t>(n);
+            }
+        }
+        if (lane == 0)
+            wk.segCount[slotOut] = nseg;
     }
 }
 
@@ -288,7 +347,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     constexpr int kRows = B / (4 * kThreads); // float4 rows per thread
     constexpr int U = kLoadsPerThread;        // 256-posting columns per round
     __shared__ __align__(16) float acc[B];
-    __shared__ uint32_t sLo[kMaxTerms], sPre[kMaxTerms + 1], sCol[kMaxTerms + 1];
+    __shared__ uint32_t sLo[kMaxSegs], sWord[kMaxSegs], sPre[kMaxSegs + 1], sCol[kMaxSegs + 1];
     __shared__ uint2 sRound[U];
     __shared__ uint32_t sRed[kWaves];
     __shared__ uint32_t sShared[4];
@@ -307,21 +366,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         return static_cast<int>(kPilotStride * (j / (kPilotStride - 1)) + j % (kPilotStride - 1) + 1);
     };
     auto queryOf = [&](int64_t t) { return static_cast<int>((t < firstTasks ? t : t - firstTasks) % nq); };
-    // This lane's word slice of a task (lane j: word j of the task's query).
-    auto sliceOf = [&](int64_t t) {
+    // This lane's posting segment of a task (lane s: segment s) and the task's segment count.
+    auto segOf = [&](int64_t t, uint32_t &count) {
         uint2 v = make_uint2(0u, 0u);
+        count = 0;
         if (t < total) {
-            const int tq = queryOf(t);
-            const int64_t tw0 = bt.qoff[tq];
-            const int tn = static_cast<int>(bt.qoff[tq + 1] - tw0);
-            if (lane < tn)
-                v = wk.slices[static_cast<int64_t>(blocks) * tw0 + static_cast<int64_t>(blockOf(t)) * tn + lane];
+            const int64_t at = static_cast<int64_t>(queryOf(t)) * blocks + blockOf(t);
+            count = wk.segCount[at];
+            v = wk.segs[at * kMaxSegs + lane];
         }
         return v;
     };
-    uint2 nextSlice = make_uint2(0u, 0u);
-    if (wave == 0)
-        nextSlice = sliceOf(blockIdx.x);
     unsigned long long scanned = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0, emitting = 0, exactTasks = 0;
 #ifdef BM25MI_PHASES
     // Cycle counters of thread 0 per phase: setup, postings, selection, rest.
@@ -346,17 +401,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         const int blk = blockOf(task);
         const float thr = bt.thr[q];
         const int32_t base = blk * B;
-        const int64_t w0 = bt.qoff[q];
-        const int n = static_cast<int>(bt.qoff[q + 1] - w0);
         const int32_t k = bt.k[q];
         __syncthreads(); // previous task is done with LDS
 
         if (wave == 0) {
-            // This block's slice of each query word (lane j: word j; loaded during the previous
-            // task), then prefixes of slice lengths and of 256-posting column counts.
-            const uint2 sl = nextSlice;
-            nextSlice = sliceOf(task + gridDim.x);
-            const uint32_t len = sl.y - sl.x;
+            // This task's posting segments (lane s: segment s; loaded during the previous
+            // task), then prefixes of segment lengths and of 256-posting column counts.
+            uint32_t nseg;
+            const uint2 sg = segOf(task, nseg);
+            const uint32_t len = lane < static_cast<int>(nseg) ? (sg.y >> 8) : 0u;
             uint32_t x = len, c = (len + kThreads - 1) / kThreads;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -366,8 +419,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                     c += z;
                 }
             }
-            if (lane < n) {
-                sLo[lane] = sl.x;
+            if (lane < static_cast<int>(nseg)) {
+                sLo[lane] = sg.x;
+                sWord[lane] = sg.y & 0xFFu;
                 sPre[lane + 1] = x;
                 sCol[lane + 1] = c;
             }
@@ -376,7 +430,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                 sPre[0] = 0;
                 sCol[0] = 0;
                 sShared[0] = max(max(__atomic_load_n(&wk.tau[q], __ATOMIC_RELAXED), hb), 1u);
-                sShared[2] = (thr > 0.0F && wk.bound[static_cast<int64_t>(q) * blocks + blk] < thr) ? 0u : 1u;
+                sShared[2] = (thr > 0.0F && nseg == 0) ? 0u : 1u;
+                sShared[3] = nseg;
             }
         }
         float4 *acc4 = reinterpret_cast<float4 *>(acc);
@@ -386,7 +441,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         __syncthreads();
         BM25MI_PHASE(0);
         if (sShared[2] == 0) {
-            ++skipped; // the block's score upper bound is below the threshold
+            ++skipped; // every granule's score upper bound is below the threshold
             continue;
         }
         ++scored;
@@ -399,21 +454,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         // column) before any add, then adds them column by column with a barrier where the
         // word changes. Within one word each doc appears once, so lanes never collide, and
         // every doc's score is summed in query-word order.
-        const uint32_t columns = sCol[n];
+        const uint32_t nseg = sShared[3];
+        const uint32_t columns = sCol[nseg];
         if (tid == 0)
-            scanned += sPre[n];
+            scanned += sPre[nseg];
         for (uint32_t r0 = 0; r0 < columns; r0 += U) {
             if (wave == 0 && lane < U) {
                 const uint32_t col = r0 + lane;
                 uint2 entry = make_uint2(0u, 0xFFFF0000u);
                 if (col < columns) {
-                    int w = 0;
-                    while (sCol[w + 1] <= col)
-                        ++w;
-                    const uint32_t i = col - sCol[w];
-                    const uint32_t len = sPre[w + 1] - sPre[w];
-                    entry = make_uint2(sLo[w] + i * kThreads, min(static_cast<uint32_t>(kThreads), len - i * kThreads) |
-                                                                  (static_cast<uint32_t>(w) << 16));
+                    int sg = 0;
+                    while (sCol[sg + 1] <= col)
+                        ++sg;
+                    const uint32_t i = col - sCol[sg];
+                    const uint32_t len = sPre[sg + 1] - sPre[sg];
+                    entry = make_uint2(sLo[sg] + i * kThreads, min(static_cast<uint32_t>(kThreads), len - i * kThreads) |
+                                                                   (sWord[sg] << 16));
                 }
                 sRound[lane] = entry;
             }

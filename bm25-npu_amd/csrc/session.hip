@@ -70,7 +70,7 @@ public:
     // Device memory.
     void *dPost = nullptr, *dToff = nullptr, *dSlot = nullptr, *dDir = nullptr, *dBmax = nullptr, *dTmax = nullptr;
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr, *dSlices = nullptr, *dBound = nullptr;
+    void *dHist = nullptr, *dSegs = nullptr, *dSegCount = nullptr;
     // Pinned host memory: request packet (copied by the graph) and result block (written by the device).
     void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     size_t packetBytes = 0, resultBytes = 0;
@@ -134,7 +134,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     device = cfg.device_id;
 
     // Host-side tables: impacts, directory, block maxima.
-    granule = std::min(1024, blockDocs);
+    granule = kGranule;
     granulesPerBlock = blockDocs / granule;
     blocks = static_cast<int32_t>((n + blockDocs - 1) / blockDocs);
     Layout layout;
@@ -233,8 +233,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     dPool = Alloc<unsigned long long>(q * poolPerQuery, "alloc candidate pools");
     dCount = Alloc<uint32_t>(q, "alloc pool counts");
     dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
-    dSlices = Alloc<uint2>(static_cast<size_t>(blocks) * std::max<int64_t>(1, cfg.max_words), "alloc slice table");
-    dBound = Alloc<float>(q * blocks, "alloc block bounds");
+    dSegs = Alloc<uint2>(q * blocks * kMaxSegs, "alloc segment table");
+    dSegCount = Alloc<uint32_t>(q * blocks, "alloc segment counts");
     dTau = Alloc<uint32_t>(q, "alloc thresholds");
     dStats = Alloc<unsigned long long>(kStats, "alloc stats");
 
@@ -252,8 +252,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
                   static_cast<uint32_t *>(dHist),
                   static_cast<uint32_t *>(dTau),            static_cast<unsigned long long *>(dStats),
                   reinterpret_cast<int32_t *>(rs + resStatus), reinterpret_cast<float *>(rs + resScores),
-                  reinterpret_cast<int32_t *>(rs + resIds), static_cast<uint2 *>(dSlices),
-                  static_cast<float *>(dBound)};
+                  reinterpret_cast<int32_t *>(rs + resIds), static_cast<uint2 *>(dSegs),
+                  static_cast<uint32_t *>(dSegCount)};
     LaunchInit(wk, cfg.max_queries, stream);
     BM25MI_HIP(hipStreamSynchronize(stream), "initialise counters");
     if (cfg.use_graph)
@@ -444,7 +444,7 @@ void Session::Release() noexcept
         stream = nullptr;
     }
     for (void **p : {&dPost, &dToff, &dSlot, &dDir, &dBmax, &dTmax, &dPacket, &dPool, &dCount, &dTau, &dStats,
-                     &dHist, &dSlices, &dBound})
+                     &dHist, &dSegs, &dSegCount})
         if (*p) {
             (void)hipFree(*p);
             *p = nullptr;
