@@ -1,4 +1,5 @@
-"""Builds libbm25mi.so for gfx950 in-tree (bm25-npu_amd/lib/) with hipcc.
+"""Builds libbm25mi.so for gfx950 in-tree (bm25-npu_amd/lib/) with hipcc, plus the
+copy_calibration program used to scale rocprofv3 byte counters.
 
 Usage: python bm25-npu_amd/build.py [--jobs N]
 """
@@ -49,6 +50,13 @@ def build(jobs=8, lib_dir=None):
     if r.returncode:
         raise RuntimeError(f'link failed:\n{r.stdout}\n{r.stderr}')
     os.replace(tmp, target)
+    # Counter-calibration copy program (a plain copy kernel of known size).
+    exe = LIB / 'copy_calibration'
+    cmd = [HIPCC] + COMMON + DEVICE + ['-o', str(exe) + '.tmp', str(CSRC / 'copy_calibration.hip')]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(f'compile copy_calibration failed:\n{r.stdout}\n{r.stderr}')
+    os.replace(str(exe) + '.tmp', exe)
     return target
 
 
