@@ -55,11 +55,24 @@ def cpu_model():
     return 'unknown'
 
 
-def algorithmic_bytes(session, offsets, terms, k, stats):
-    """Bytes the main scoring launch must move: 8 B per posting it streams ({doc, impact};
-    the pilot pass streams the rest), the query packet and the top-K output."""
+def algorithmic_bytes(offsets, terms, k, stats):
+    """Bytes a Search must move: 8 B per posting of the query terms ({doc ID, impact}), the
+    query packet (8 B per query offset, 4 B per term) and the top-K output (8 B per row)."""
     nq = len(offsets) - 1
-    return 8 * (stats['postings_scanned'] - stats['pilot_postings']) + 8 * (nq + 1) + 4 * len(terms) + 8 * nq * k
+    return 8 * stats['postings_total'] + 8 * (nq + 1) + 4 * len(terms) + 8 * nq * k
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of the scoring kernel from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy_calibration program's readings), when
+    they cover this kernel and workload; else None."""
+    path = ROOT / 'profiles' / 'kscore_synthetic10m_q8x3_k10.json'
+    if not path.exists() or (args.corpus, args.queries, args.terms, args.k) != ('synthetic10m', 8, 3, 10):
+        return None, None
+    rec = json.loads(path.read_text())
+    if rec.get('kernel') != kernel:
+        return None, None
+    return rec['traffic_bytes_per_launch'], rec['source']
 
 
 def main():
@@ -71,7 +84,7 @@ def main():
     ap.add_argument('--queries', type=int, default=8)
     ap.add_argument('--terms', type=int, default=3)
     ap.add_argument('--k', type=int, default=10)
-    ap.add_argument('--block-docs', type=int, default=8192)
+    ap.add_argument('--block-docs', type=int, default=0, help='docs per scoring task (0 = library default)')
     ap.add_argument('--grid', type=int, default=0, help='persistent scoring workgroups (0 = fill the GPU)')
     ap.add_argument('--cpu-calls', type=int, default=10, help='timed CPU oracle calls (A/B alternating)')
     ap.add_argument('--no-cpu', action='store_true')
@@ -111,14 +124,14 @@ def main():
     log(f'timed {args.steps} Searches: median {statistics.median(samples):.4f} ms')
 
     # Scoring-kernel time with HIP events on the launch stream (direct launches).
-    kernel_ms, bytes_per, pilot_ms = [], [], []
+    kernel_ms, bytes_per, final_ms = [], [], []
     for step in range(max(20, min(args.steps, 100))):
         offs, terms = packed[step % 2]
         session.search(packed[step % 2], args.k, timed=True)
         st = session.stats()
         kernel_ms.append(st['score_kernel_ms'])
-        pilot_ms.append(st['pilot_ms'])
-        bytes_per.append(algorithmic_bytes(session, offs, terms, args.k, st))
+        final_ms.append(st['final_ms'])
+        bytes_per.append(algorithmic_bytes(offs, terms, args.k, st))
     k_ms = statistics.mean(kernel_ms)
     b_avg = statistics.mean(bytes_per)
     achieved = b_avg / (k_ms * 1e-3) / 1e9
@@ -131,6 +144,7 @@ def main():
     cus, lds_block, lds_cu = C.c_int32(), C.c_int32(), C.c_int32()
     bm25mi._lib.check(bm25mi._lib.load().bm25mi_device_info(0, C.byref(cus), C.byref(lds_block), C.byref(lds_cu)))
 
+    traffic, traffic_note = measured_traffic(f'KScore<{st["block_docs"]}>', args)
     record = {
         'metric': 'Q8/K10 full-Search ms @10M docs; QPS; % HBM roofline; learned vs plain',
         'value': args.queries * args.steps / total_s,
@@ -151,16 +165,18 @@ def main():
                    'queries': args.queries, 'terms_per_query': args.terms, 'k': args.k,
                    'block_docs': args.block_docs, 'k1': index.k1, 'b': index.b},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'KScore', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg},
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'traffic_source': traffic_note,
+                     'kernel': f'KScore<{st["block_docs"]}>', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg,
+                     'launch_waves': st['score_grid']},
         'postings_per_search': st['postings_total'],
         'search_breakdown': {
             'device_ms_graph': statistics.mean(g['device_ms'] for g in graph_stats),
             'host_call_ms': statistics.mean(g['host_ms'] for g in graph_stats),
             'candidates_per_search': statistics.mean(g['candidates'] for g in graph_stats),
             'tasks_per_search': st['blocks_scored'] + st['blocks_skipped'],
-            'pilot_and_threshold_ms': statistics.mean(pilot_ms),
-            'pilot_postings': st['pilot_postings'],
+            'final_kernel_ms': statistics.mean(final_ms),
+            'postings_streamed': st['postings_scanned'],
+            'postings_in_range': st['postings_in_range'],
             'emitting_tasks': st['emitting_blocks'],
             'exact_tasks': st['exact_blocks'],
         },

@@ -27,6 +27,7 @@ class Config(C.Structure):
         ('k1', C.c_double),
         ('use_graph', C.c_int32),
         ('grid_blocks', C.c_int32),
+        ('task_cost', C.c_int32),
     ]
 
 
@@ -42,11 +43,16 @@ class Stats(C.Structure):
         ('candidates', C.c_int64),
         ('fallbacks', C.c_int32),
         ('queries', C.c_int32),
-        ('pilot_ms', C.c_double),
-        ('pilot_postings', C.c_int64),
+        ('final_ms', C.c_double),
+        ('postings_in_range', C.c_int64),
         ('emitting_blocks', C.c_int64),
         ('exact_blocks', C.c_int64),
-        ('phase_cycles', C.c_int64 * 8),
+        ('score_grid', C.c_int32),
+        ('block_docs', C.c_int32),
+        ('phase_cycles', C.c_int64 * 4),
+        ('wave_busy_max', C.c_int64),
+        ('wave_span', C.c_int64),
+        ('wave_start_spread', C.c_int64),
     ]
 
     def as_dict(self):

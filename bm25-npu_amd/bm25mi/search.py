@@ -29,8 +29,8 @@ def _is_int(value, low, high):
 class Session:
     """A resident index on one GPU. All searches go through libbm25mi."""
 
-    def __init__(self, index, max_queries=8, max_words=24, max_k=10, block_docs=8192, k1=None, device=0,
-                 use_graph=True, grid_blocks=0):
+    def __init__(self, index, max_queries=8, max_words=24, max_k=10, block_docs=0, k1=None, device=0,
+                 use_graph=True, grid_blocks=0, task_cost=0):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.bm25mi_config_default(C.byref(cfg))
@@ -42,6 +42,7 @@ class Session:
         cfg.k1 = index.k1 if k1 is None else k1
         cfg.use_graph = 1 if use_graph else 0
         cfg.grid_blocks = grid_blocks
+        cfg.task_cost = task_cost
         self.config = cfg
         self.num_docs, self.vocab = index.num_docs, index.vocab
         handle = C.c_void_p()
@@ -188,7 +189,7 @@ class Bm25Config:
     maxQueryWords: int = 24
     topK: int = 10
     k1: float = 1.2
-    blockDocs: int = 8192
+    blockDocs: int = 0
 
 
 @dataclass

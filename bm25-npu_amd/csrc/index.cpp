@@ -9,6 +9,7 @@
 // rounding, so the value is bit-identical to the reference's.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 
 #include "common.h"
 #include "layout.h"
@@ -122,6 +123,19 @@ void BuildLayout(const int64_t *offsets, int64_t vocab, const int32_t *docs, con
             }
             while (g <= out.granules)
                 dir[g++] = static_cast<uint32_t>(e - b);
+        }
+    });
+    out.topOff.assign(static_cast<size_t>(vocab + 1), 0);
+    for (int64_t t = 0; t < vocab; ++t) {
+        const int64_t df = offsets[t + 1] - offsets[t];
+        out.topOff[t + 1] = out.topOff[t] + std::min<int64_t>(df, df >= longMin ? kTopLong : kTopShort);
+    }
+    out.topVal.assign(static_cast<size_t>(out.topOff[vocab]), 0.0F);
+    ParallelChunks(vocab, Threads(threads), [&](int, int64_t lo, int64_t hi) {
+        for (int64_t t = lo; t < hi; ++t) {
+            float *top = out.topVal.data() + out.topOff[t];
+            const int64_t r = out.topOff[t + 1] - out.topOff[t];
+            std::partial_sort_copy(impact + offsets[t], impact + offsets[t + 1], top, top + r, std::greater<float>());
         }
     });
 }

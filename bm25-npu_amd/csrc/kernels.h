@@ -5,54 +5,58 @@
 
 namespace bm25mi {
 
-constexpr int kThreads = 256;              // scoring workgroup: 4 wave64
-constexpr int kWaves = kThreads / 64;
-constexpr int kMaxTerms = 64;              // terms per query (fixed_query_packet.h:32)
-constexpr int kMaxSegs = 64;               // posting segments per (query, block) task
+constexpr int kWave = 64;                  // CDNA wavefront
+constexpr int kMaxTerms = 64;              // terms per query (fixed_query_packet.h:32): one lane per term
 constexpr int kGranule = 256;              // docs per directory / block-max granule
-constexpr int kLongMin = 4096;             // lists this long use the block directory; shorter
-                                           // ones a two-round 64-ary in-kernel search (64*64)
-constexpr int kFinalThreads = 1024;        // selection workgroup (threshold and final kernels)
-constexpr int kSortCap = 16384;            // pool entries a selection workgroup sorts in LDS (128 KB)
-constexpr int kPilotStride = 16;           // every 16th block is scored first (spreads the first round)
-constexpr int kEmitCap = 512;              // pool entries per block reserved beyond K (sizing only)
+constexpr int kGranuleShift = 8;
+constexpr int kLongMin = 4096;             // lists this long use the granule directory; shorter ones
+                                           // are located through the posting samples
+constexpr int kSampleShift = 4;            // samples[i] = doc ID of posting 16 * i (global CSC index)
+constexpr int kFinalThreads = 1024;        // selection workgroup of the final kernel
+constexpr int kSortCap = 16384;            // pool entries the final kernel sorts in LDS (128 KB)
+constexpr int kTaskCost = 4096;            // estimated fixed cost of a scoring task, in postings
+#ifndef BM25MI_COLUMNS
+#define BM25MI_COLUMNS 8
+#endif
+constexpr int kColumns = BM25MI_COLUMNS;   // 64-posting columns a wave keeps in flight per round
+constexpr int kStats = 16;
 // Score histogram of kept candidates, per query: bucket = (score bits >> 19) - kHistBase,
 // i.e. 16 buckets per octave over [2^-4, 2^8) (lower scores fall in bucket 0, higher ones
 // in the last bucket, which only makes the bound it yields smaller).
 constexpr int kHistBins = 192;
 constexpr uint32_t kHistBase = (127u - 4u) << 4;
-#ifndef BM25MI_LOADS
-#define BM25MI_LOADS 8
-#endif
-constexpr int kLoadsPerThread = BM25MI_LOADS; // posting loads a thread keeps in flight per round
-constexpr int kStats = 16;
-constexpr int kCachedQueries = 256;      // per-query headers kept in LDS by the scoring kernel
+
+// Candidates a block may append: all of them when at most this many score at or above the
+// query's floor, else a cut keeping between K and this many. It sizes the candidate pools.
+__host__ __device__ constexpr int32_t BlockKeepCap(int32_t k, int32_t blockDocs)
+{
+    const int32_t c = 2 * k > 64 ? 2 * k : 64;
+    return c < blockDocs ? c : blockDocs;
+}
 
 // Stats slots (unsigned long long).
 enum {
-    kStatPostings = 0,
-    kStatScored = 1,
-    kStatSkipped = 2,
+    kStatPostings = 0, // postings streamed (including masked window postings of short lists)
+    kStatScored = 1,   // (query, block) tasks scored
+    kStatSkipped = 2,  // tasks skipped: no postings, or every granule bound below the threshold
     kStatCandidates = 3,
     kStatOverflow = 4,
     kStatEmitting = 5, // tasks that appended at least one candidate
-    kStatExact = 6,    // tasks that needed the exact K-th-score bisection
-    kStatPilot = 7,    // postings streamed by the pilot pass
-    kStatPhase = 8     // 8..15: scoring-kernel cycles per phase, main then pilot pass (BM25MI_PHASES builds)
+    kStatExact = 6,    // tasks that needed a cut (more candidates than they may append)
+    kStatUseful = 7,   // postings inside their task's doc range (the plain-path work)
+    kStatPhase = 8,    // 8..11: scoring time per phase (locate, stream and add, select, append)
+    kStatMax = 12      // 12..15 (maxima over waves): busy time, end, ~start, start
+                       // (8..15 in measurement builds, -DBM25MI_PHASES, only; 100 MHz ticks)
 };
 
 // Read-only resident index.
 struct IndexView {
     const uint2 *post;      // {doc ID, impact bits} per posting, CSC order
-    const int64_t *toff;    // term offsets [vocab+1]
-    const int32_t *slot;    // long-term slot or -1 [vocab]
+    const int32_t *samples; // doc ID of every 16th posting (global CSC index)
     const uint32_t *dir;    // [longTerms * (granules+1)]
     const float *bmax;      // [longTerms * granules]
-    const float *tmax;      // [vocab]
-    int64_t vocab;
     int64_t granules;
     int32_t ndocs;
-    int32_t granuleShift;
 };
 
 // One Search batch, laid out in a single device packet uploaded per Search.
@@ -61,11 +65,16 @@ struct BatchView {
     const int64_t *qoff;    // [maxQ+1]
     const int32_t *k;       // [maxQ]
     const float *thr;       // [maxQ] pruning threshold (0 = plain)
+    const uint32_t *tau0;   // [maxQ] lower bound of the K-th score known before the Search
+                            // (score bits): the largest K-th term impact over the query's terms
     const int64_t *poolOff; // [maxQ+1]
     const int64_t *outOff;  // [maxQ+1]
     // Per query word, resolved on the host from the term ID: {first posting, posting
     // count (0 for IDs >= vocab), long-term slot or -1, largest impact bits}.
     const uint4 *words;     // [maxWords]
+    // Scoring wave w takes tasks [waveStart[w], waveStart[w+1]) of the query-major task
+    // order (task = q * blocks + blk).
+    const uint32_t *waveStart; // [grid+1]
 };
 
 // Per-Search device scratch and results.
@@ -74,17 +83,10 @@ struct WorkView {
     uint32_t *count;          // [maxQ] pool fill
     uint32_t *hist;           // [maxQ * kHistBins] histogram of kept candidates' scores
     uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
-    unsigned long long *stats;// [kStats]
+    unsigned long long *stats;// [grid][kStats]: one counter row per scoring wave
     int32_t *status;          // [maxQ] 0 ok, 1 fewer than K survivors under a threshold
     float *scores;            // [maxQ*maxK]
     int32_t *ids;             // [maxQ*maxK]
-    // Posting slice {first, end} of every (query word, block), written by the planning
-    // kernel: entry blocks * qoff[q] + blk * n_q + j for word j of query q.
-    // Posting segments of every (query, block) task, written by the planning kernel:
-    // task (q, blk) owns segs[(q * blocks + blk) * kMaxSegs ...], {first posting,
-    // (count << 8) | word}, in word order; segCount[q * blocks + blk] of them.
-    uint2 *segs;
-    uint32_t *segCount;
 };
 
 } // namespace bm25mi

@@ -24,7 +24,16 @@ struct Layout {
     std::vector<float> termMean; // mean impact of each term (0 for empty lists)
     std::vector<uint32_t> dir;
     std::vector<float> blockMax;
+    // The largest impacts of each term in descending order: min(df, 1024) of them for long
+    // lists, min(df, 64) for short ones, term t at topVal[topOff[t] ...]. A doc's score is
+    // at least each of its impacts (FP32 sums of non-negative values never decrease), so the
+    // K-th largest impact of any query term is a lower bound of the query's K-th score.
+    std::vector<int64_t> topOff;
+    std::vector<float> topVal;
 };
+
+constexpr int32_t kTopLong = 1024;
+constexpr int32_t kTopShort = 64;
 
 void ValidateIndex(const int64_t *offsets, int64_t vocab, const int32_t *docs, const float *tf, int64_t postings,
                    const float *norm, int64_t ndocs, const float *idf, int threads);

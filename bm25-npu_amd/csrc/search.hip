@@ -9,20 +9,19 @@
 //     a 32-way tiered merge of sorted K-lists.
 //
 // Design here (one GPU, HBM-bound scan/scatter work, no MFMA):
-//   k_score  - persistent workgroups walk (query, doc-block) tasks. Each task streams the
-//              query terms' posting slices for its block (from a block directory, or a
-//              two-round 64-ary wave search for short lists) with 8-byte {doc, impact}
-//              loads and adds impacts into LDS FP32 accumulators, term by term in query
-//              order with a barrier between terms, so every doc's score is the same
-//              sequential FP32 sum as the CPU reference (cpu_index_search.h:176-189).
-//              Then it keeps the block's exact top-K among docs scoring at least a
-//              per-query running lower bound of the global K-th score, and appends them
-//              as 64-bit keys (score bits << 32 | ~doc) to the query's candidate pool.
-//   k_final  - one workgroup per query selects the exact top-K of its pool by a 64-step
-//              bisection on the composite key (unique per doc, so ties need no extra
-//              pass), bitonic-sorts it (score desc, doc asc: topk_heap.h:11-16) and
-//              writes it to pinned host memory. Fewer than K positive docs are padded
-//              with the lowest-ID zero-score docs, as the reference does.
+//   KScore  - every wave is its own workgroup and walks (query, doc-block) tasks on its own,
+//             with no barriers: it locates the query terms' postings in its block (granule
+//             directory for long lists, posting samples for short ones), streams them with
+//             8-byte {doc, impact} loads and adds impacts into its LDS FP32 accumulator row
+//             in query-term order, so every doc's score is the same sequential FP32 sum as
+//             the CPU reference (cpu_index_search.h:176-189). It keeps the docs scoring at
+//             least a per-query running lower bound of the global K-th score, cut to at most
+//             BlockKeepCap(K) per block, as 64-bit keys (score bits << 32 | ~doc) in the
+//             query's candidate pool.
+//   KFinal  - one workgroup per query selects the exact top-K of its pool (composite keys are
+//             unique per doc, so ties need no extra pass), bitonic-sorts it (score desc, doc
+//             asc: topk_heap.h:11-16) and writes it to pinned host memory. Fewer than K
+//             positive docs are padded with the lowest-ID zero-score docs, as the reference does.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -99,36 +98,6 @@ template <int NT> __device__ __forceinline__ uint32_t BlockExclusive(uint32_t v,
     return before + x - v;
 }
 
-// lower_bound of a and of b (a <= b) among doc IDs post[lo, hi), hi - lo < 4096.
-// Executed by one full wave: round 1 samples every ceil(len/64)-th posting, round 2
-// resolves the <= 63-posting window each sample count points at.
-__device__ __forceinline__ void WaveBounds(const uint2 *post, uint32_t lo, uint32_t hi, int32_t a, int32_t b,
-                                           uint32_t &ra, uint32_t &rb)
-{
-    const uint32_t lane = __lane_id();
-    const uint32_t len = hi - lo;
-    const uint32_t step = (len + 63) / 64;
-    const uint32_t p = lo + lane * step;
-    const bool valid = lane * step < len;
-    const int32_t v = valid ? static_cast<int32_t>(post[p].x) : 0;
-    const uint32_t ca = __popcll(__ballot(valid && v < a));
-    const uint32_t cb = __popcll(__ballot(valid && v < b));
-    uint32_t r0a = lo, r1a = lo, r0b = lo, r1b = lo;
-    if (ca > 0) {
-        r0a = lo + (ca - 1) * step + 1;
-        r1a = min(lo + ca * step, hi);
-    }
-    if (cb > 0) {
-        r0b = lo + (cb - 1) * step + 1;
-        r1b = min(lo + cb * step, hi);
-    }
-    const bool va = r0a + lane < r1a, vb = r0b + lane < r1b;
-    const int32_t xa = va ? static_cast<int32_t>(post[r0a + lane].x) : 0;
-    const int32_t xb = vb ? static_cast<int32_t>(post[r0b + lane].x) : 0;
-    ra = r0a + __popcll(__ballot(va && xa < a));
-    rb = r0b + __popcll(__ballot(vb && xb < b));
-}
-
 // Wave-aggregated append: returns this lane's slot (counter += lanes with pred), or
 // 0xFFFFFFFF when pred is false. Every lane of the wave calls.
 __device__ __forceinline__ uint32_t WaveAppend(bool pred, uint32_t *counter)
@@ -176,21 +145,19 @@ __device__ __forceinline__ uint32_t HistEdge(uint32_t bucket)
     return bucket == 0 ? 1u : (bucket + kHistBase) << 19;
 }
 
+constexpr int kHistPerLane = (kHistBins + 63) / 64;
+
 // Lower bound of a query's K-th score from its kept-candidate histogram, computed by one
-// wave: the edge of the highest bucket b with at least k counted scores at or above it
-// (those k candidates score at least that edge). 1 when fewer than k are counted.
-__device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t k)
+// wave from the bins it holds (lane l: bins l*kHistPerLane ...): the edge of the highest
+// bucket b with at least k counted scores at or above it (those k candidates score at least
+// that edge). 1 when fewer than k are counted.
+__device__ __forceinline__ uint32_t WaveHistBoundRegs(const uint32_t (&h)[kHistPerLane], uint32_t k)
 {
-    constexpr int kPer = (kHistBins + 63) / 64;
     const int lane = static_cast<int>(__lane_id());
-    uint32_t h[kPer];
     uint32_t sum = 0;
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int b = lane * kPer + i;
-        h[i] = b < kHistBins ? __atomic_load_n(hist + b, __ATOMIC_RELAXED) : 0u;
+    for (int i = 0; i < kHistPerLane; ++i)
         sum += h[i];
-    }
     // above = counts in buckets of higher lanes (suffix over lanes).
     uint32_t x = sum;
 #pragma unroll
@@ -202,484 +169,465 @@ __device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t
     uint32_t above = x - sum;
     uint32_t best = 0; // bucket + 1, or 0
 #pragma unroll
-    for (int i = kPer - 1; i >= 0; --i) {
+    for (int i = kHistPerLane - 1; i >= 0; --i) {
         above += h[i];
         if (best == 0 && above >= k)
-            best = static_cast<uint32_t>(lane * kPer + i) + 1;
+            best = static_cast<uint32_t>(lane * kHistPerLane + i) + 1;
     }
     const uint32_t top = WaveMax(best);
     return top ? HistEdge(top - 1) : 1u;
 }
 
-// Planning kernel, one wave per (query, block) task. Finds the posting slice of every
-// query word inside the block (block directory for long lists, a two-round 64-ary search
-// for short ones) and writes the task's posting segments:
-// * plain queries: one segment per word (its block slice);
-// * threshold queries: an upper bound of every 256-doc granule's scores (per-word granule
-//   maxima summed in word order; short words count their largest impact), then per word
-//   only the runs of granules whose bound reaches the threshold. Docs in other granules
-//   can only collect partial sums below the threshold, so they are never kept. A task with
-//   no such granule gets no segment; one needing more than kMaxSegs keeps whole slices.
-__global__ __launch_bounds__(256) void KPlan(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t blockDocs,
-                                             int32_t granulesPerBlock)
+__device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t k)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / 64);
-    const int32_t nq = bt.header[0];
-    const int64_t total = static_cast<int64_t>(nq) * blocks;
-    for (int64_t task = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); task < total;
-         task += waves) {
-        const int q = static_cast<int>(task % nq);
-        const int blk = static_cast<int>(task / nq);
-        const int32_t base = blk * blockDocs;
-        const int32_t end = min(base + blockDocs, ix.ndocs);
-        const int64_t w0 = bt.qoff[q];
-        const int n = static_cast<int>(bt.qoff[q + 1] - w0);
-        const float thr = bt.thr[q];
-        const int64_t slotOut = static_cast<int64_t>(q) * blocks + blk;
-        uint2 *segs = wk.segs + slotOut * kMaxSegs;
-        uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-        if (lane < n)
-            info = bt.words[w0 + lane];
-        uint32_t lo = 0, hi = 0;
-        const int32_t sl = static_cast<int32_t>(info.z);
-        const int64_t g0 = static_cast<int64_t>(blk) * granulesPerBlock;
-        const int64_t g1 = min(g0 + granulesPerBlock, ix.granules);
-        if (info.y > 0 && sl >= 0) {
-            const uint32_t *d = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
-            lo = info.x + d[g0];
-            hi = info.x + d[g1];
-        }
-        // Short lists: the whole wave searches each one in turn.
-        unsigned long long shortMask = __ballot(lane < n && info.y > 0 && sl < 0);
-        while (shortMask) {
-            const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
-            shortMask &= shortMask - 1;
-            const uint32_t first = __shfl(info.x, j), count = __shfl(info.y, j);
-            uint32_t a, b;
-            WaveBounds(ix.post, first, first + count, base, end, a, b);
-            if (lane == j) {
-                lo = a;
-                hi = b;
-            }
-        }
-        if (thr <= 0.0F) {
-            if (lane < n)
-                segs[lane] = make_uint2(lo, ((hi - lo) << 8) | static_cast<uint32_t>(lane));
-            if (lane == 0)
-                wk.segCount[slotOut] = static_cast<uint32_t>(n);
-            continue;
-        }
-        // Granule bounds (lane g: granule g0 + g), summed in word order.
-        const int gpb = static_cast<int>(g1 - g0);
-        float ub = 0.0F;
-        for (int j = 0; j < n; ++j) {
-            const uint32_t jlo = __shfl(lo, j), jhi = __shfl(hi, j);
-            const int32_t jsl = static_cast<int32_t>(__shfl(info.z, j));
-            const float jmax = __uint_as_float(__shfl(info.w, j));
-            if (jhi > jlo && lane < gpb)
-                ub += jsl >= 0 ? ix.bmax[static_cast<int64_t>(jsl) * ix.granules + g0 + lane] : jmax;
-        }
-        const unsigned long long active = __ballot(lane < gpb && ub >= thr);
-        uint32_t nseg = 0;
-        if (active) {
-            // Runs of active granules: [start bit, first inactive bit after it).
-            bool overflow = false;
-            for (int j = 0; j < n && !overflow; ++j) {
-                const uint32_t jlo = __shfl(lo, j), jhi = __shfl(hi, j);
-                if (jhi <= jlo)
-                    continue;
-                const int32_t jsl = static_cast<int32_t>(__shfl(info.z, j));
-                if (jsl < 0) {
-                    // Short word: its whole block slice.
-                    if (nseg >= static_cast<uint32_t>(kMaxSegs)) {
-                        overflow = true;
-                        break;
-                    }
-                    if (lane == 0)
-                        segs[nseg] = make_uint2(jlo, ((jhi - jlo) << 8) | static_cast<uint32_t>(j));
-                    ++nseg;
-                    continue;
-                }
-                const uint32_t *d = ix.dir + static_cast<int64_t>(jsl) * (ix.granules + 1);
-                const uint32_t first = __shfl(info.x, j);
-                const uint32_t dv = lane <= gpb ? d[g0 + lane] : 0u; // lane g: postings before granule g
-                unsigned long long runs = active & ~(active << 1);      // run starts
-                while (runs) {
-                    const int ga = __ffsll(static_cast<long long>(runs)) - 1;
-                    runs &= runs - 1;
-                    const unsigned long long after = ~active & (~0ULL << ga);
-                    const int gb = after ? min(__ffsll(static_cast<long long>(after)) - 1, gpb) : gpb;
-                    const uint32_t s0 = first + __shfl(dv, ga), s1 = first + __shfl(dv, gb);
-                    if (s1 > s0) {
-                        if (nseg >= static_cast<uint32_t>(kMaxSegs)) {
-                            overflow = true;
-                            break;
-                        }
-                        if (lane == 0)
-                            segs[nseg] = make_uint2(s0, ((s1 - s0) << 8) | static_cast<uint32_t>(j));
-                        ++nseg;
-                    }
-                }
-            }
-            if (overflow) {
-                // Too fragmented: whole block slices instead.
-                if (lane < n)
-                    segs[lane] = make_uint2(lo, ((hi - lo) << 8) | static_cast<uint32_t>(lane));
-                nseg = static_cast<uint32_t>(n);
-            }
-        }
-        if (lane == 0)
-            wk.segCount[slotOut] = nseg;
+    uint32_t h[kHistPerLane];
+#pragma unroll
+    for (int i = 0; i < kHistPerLane; ++i) {
+        const int b = static_cast<int>(__lane_id()) * kHistPerLane + i;
+        h[i] = b < kHistBins ? __hip_atomic_load(hist + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
     }
+    return WaveHistBoundRegs(h, k);
 }
 
-// The scoring kernel; persistent workgroups walk (query, block) tasks, every
-// kPilotStride-th block first so that the first round samples the whole corpus. Each task
-// keeps its block's top-K among docs scoring at least the query's lower bound tau and
-// counts them in the query's score histogram. tau is the larger of the best block K-th
-// score seen so far and the histogram bound (the K-th best kept score, to 1/16 octave),
-// so after the first round few docs qualify and most tasks keep nothing.
-template <int B>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt,
-                                                                                          WorkView wk, int32_t blocks)
+__device__ __forceinline__ unsigned long long WaveSum64(unsigned long long v)
 {
-    constexpr int kRows = B / (4 * kThreads); // float4 rows per thread
-    constexpr int U = kLoadsPerThread;        // 256-posting columns per round
-    __shared__ __align__(16) float acc[B];
-    __shared__ uint32_t sLo[kMaxSegs], sWord[kMaxSegs], sPre[kMaxSegs + 1], sCol[kMaxSegs + 1];
-    __shared__ uint2 sRound[U];
-    __shared__ uint32_t sRed[kWaves];
-    __shared__ uint32_t sShared[4];
-    __shared__ unsigned long long sTop[kThreads]; // small exact selections (also per-thread maxima)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int32_t nq = bt.header[0];
-    const int32_t firstBlocks = (blocks + kPilotStride - 1) / kPilotStride;
-    const int64_t firstTasks = static_cast<int64_t>(nq) * firstBlocks;
-    const int64_t total = static_cast<int64_t>(nq) * blocks;
-    // Task order: blocks 0, s, 2s, ... for every query, then the other blocks.
-    auto blockOf = [&](int64_t t) {
-        if (t < firstTasks)
-            return static_cast<int>(t / nq) * kPilotStride;
-        const int64_t j = (t - firstTasks) / nq;
-        return static_cast<int>(kPilotStride * (j / (kPilotStride - 1)) + j % (kPilotStride - 1) + 1);
-    };
-    auto queryOf = [&](int64_t t) { return static_cast<int>((t < firstTasks ? t : t - firstTasks) % nq); };
-    // This lane's posting segment of a task (lane s: segment s) and the task's segment count.
-    auto segOf = [&](int64_t t, uint32_t &count) {
-        uint2 v = make_uint2(0u, 0u);
-        count = 0;
-        if (t < total) {
-            const int64_t at = static_cast<int64_t>(queryOf(t)) * blocks + blockOf(t);
-            count = wk.segCount[at];
-            v = wk.segs[at * kMaxSegs + lane];
+
+__device__ __forceinline__ uint32_t ReadLane(uint32_t v, int lane)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+}
+
+// k-th largest (1 <= k <= 64) of one value per lane: a bitonic sort across the 64 lanes
+// through shuffles, descending, then lane k-1.
+__device__ __forceinline__ uint32_t WaveKthLargest(uint32_t v, uint32_t k)
+{
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t size = 2; size <= 64; size <<= 1)
+#pragma unroll
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), static_cast<int>(stride)));
+            const bool lower = (lane & stride) == 0, desc = (lane & size) == 0;
+            v = lower == desc ? max(v, o) : min(v, o);
         }
-        return v;
-    };
-    unsigned long long scanned = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0, emitting = 0, exactTasks = 0;
+    return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(k - 1)));
+}
+
+constexpr int Log2(int x) { return x <= 1 ? 0 : 1 + Log2(x / 2); }
+
+// The scoring kernel. Every wave is its own workgroup and walks (query, block) tasks of B
+// docs independently, without barriers, so the waves of a CU overlap one another's memory
+// round trips. Tasks are ordered every kPilotStride-th block of every query first, so the
+// first tasks sample the whole corpus and the per-query lower bounds rise early.
+//
+// Per task:
+//  1. Locate each query word's postings in the block (lane j: word j). Long lists read two
+//     granule-directory entries; short lists (< kLongMin postings) count the posting samples
+//     below the block's ends and take the window of postings between the neighbouring
+//     samples (at most 15 postings outside the block on each side, masked when added).
+//     Under a threshold, lanes g < B/256 first sum the words' granule maxima in word order
+//     (an upper bound of every doc's FP32 score in granule g); granules below the threshold
+//     are cut from both ends of the block, and a block with none left is skipped.
+//  2. Stream the postings in rounds of kColumns 64-posting columns (one 8-byte load per lane
+//     per column, all issued before any add) and add each impact into the wave's LDS
+//     accumulator row. Columns run in query-word order and a doc occurs once per word, so
+//     each doc's score is the sequential FP32 sum in query order (cpu_index_search.h:176-189)
+//     in the wave's own program order.
+//  3. Keep the docs scoring at least the query's running lower bound of its K-th score
+//     (tau, or the score-histogram bound). More than BlockKeepCap(K) of them are cut to
+//     between K and that many: at the K-th largest per-lane maximum when K <= 64, else by a
+//     bisection over unique (score, doc) keys. The cut is a lower bound of the query's K-th
+//     score and raises tau. Kept docs go to the query's candidate pool as 64-bit keys
+//     (score bits << 32 | ~doc) and into its score histogram.
+template <int B>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
+{
+    constexpr int kVec = B / (4 * kWave);   // float4 accumulators per lane
+    constexpr int kIdxBits = Log2(B);
+    constexpr int kGpb = B / kGranule;      // granules per block
+    constexpr int U = kColumns;
+    constexpr int kSampleRounds = (kLongMin >> kSampleShift) / kWave;
+    static_assert(kGpb <= kWave && kVec >= 1, "block size");
+    __shared__ __align__(16) float acc[B];
+    __shared__ uint32_t sHist[kHistBins]; // this wave's appends per histogram bucket
+
+    const uint32_t lane = __lane_id();
+    unsigned long long scanned = 0, useful = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0, emitting = 0,
+                       cuts = 0;
 #ifdef BM25MI_PHASES
-    // Cycle counters of thread 0 per phase: setup, postings, selection, rest.
-    unsigned long long phase[4] = {0, 0, 0, 0};
-    unsigned long long stamp = clock64();
-#define BM25MI_PHASE(i)                                                                                                \
+    // Shader-clock cycles per phase (descriptors, postings, selection, append), measurement builds only.
+    unsigned long long phase[4] = {0, 0, 0, 0}, stamp = wall_clock64();
+    const unsigned long long waveStart = stamp;
+    int cur = 0;
+#define BM25MI_PHASE(next)                                                                                             \
     do {                                                                                                               \
-        if (tid == 0) {                                                                                                \
-            const unsigned long long now_ = clock64();                                                                 \
-            phase[i] += now_ - stamp;                                                                                  \
-            stamp = now_;                                                                                              \
-        }                                                                                                              \
+        const unsigned long long now_ = wall_clock64();                                                                     \
+        phase[cur] += now_ - stamp;                                                                                    \
+        stamp = now_;                                                                                                  \
+        cur = (next);                                                                                                  \
     } while (0)
 #else
-#define BM25MI_PHASE(i)                                                                                                \
+#define BM25MI_PHASE(next)                                                                                             \
     do {                                                                                                               \
     } while (0)
 #endif
 
-    for (int64_t task = blockIdx.x; task < total; task += gridDim.x) {
-        const int q = queryOf(task);
-        const int blk = blockOf(task);
-        const float thr = bt.thr[q];
+    // This wave's tasks: a contiguous range of the query-major task order (task = q * blocks
+    // + blk), cut by the host so that every wave gets the same estimated work.
+    const uint32_t tEnd = bt.waveStart[blockIdx.x + 1];
+    // Per-query state, reloaded only when the task's query changes.
+    int qCur = -1, n = 0;
+    float thr = 0.0F;
+    uint32_t k = 1, tau0q = 0;
+    uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+    // Long-list cursor (plain queries): a task's slice starts where the previous block's
+    // ended, and the next block's end is loaded one task ahead (lane j: word j).
+    uint32_t hiPrev = 0, hiNext = 0;
+    uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / hiNext were prepared for
+    for (uint32_t task = bt.waveStart[blockIdx.x]; task < tEnd; ++task) {
+        BM25MI_PHASE(0);
+        const int q = static_cast<int>(task / static_cast<uint32_t>(blocks));
+        const int blk = static_cast<int>(task - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks));
         const int32_t base = blk * B;
-        const int32_t k = bt.k[q];
-        __syncthreads(); // previous task is done with LDS
+        const int32_t end = min(base + B, ix.ndocs);
+        if (q != qCur) {
+            qCur = q;
+            const int64_t w0 = bt.qoff[q];
+            n = static_cast<int>(bt.qoff[q + 1] - w0);
+            thr = bt.thr[q];
+            k = static_cast<uint32_t>(bt.k[q]);
+            tau0q = bt.tau0[q];
+            info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+            if (static_cast<int>(lane) < n)
+                info = bt.words[w0 + lane];
+        }
 
-        if (wave == 0) {
-            // This task's posting segments (lane s: segment s; loaded during the previous
-            // task), then prefixes of segment lengths and of 256-posting column counts.
-            uint32_t nseg;
-            const uint2 sg = segOf(task, nseg);
-            const uint32_t len = lane < static_cast<int>(nseg) ? (sg.y >> 8) : 0u;
-            uint32_t x = len, c = (len + kThreads - 1) / kThreads;
+        // Lower-bound inputs: loaded now, used after the postings.
+        const uint32_t *histq = wk.hist + static_cast<int64_t>(q) * kHistBins;
+        uint32_t h[kHistPerLane];
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o), z = __shfl_up(c, o);
-                if (lane >= o) {
-                    x += y;
-                    c += z;
-                }
+        for (int r = 0; r < kHistPerLane; ++r) {
+            const uint32_t bin = lane * kHistPerLane + r;
+            h[r] = bin < kHistBins ? __hip_atomic_load(histq + bin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                   : 0u;
+        }
+        // Stale values are still lower bounds: these only ever grow during a Search.
+        const uint32_t tauq =
+            max(__hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), tau0q);
+
+        const int64_t g0 = static_cast<int64_t>(blk) * kGpb;
+        const int gpb = static_cast<int>(min(static_cast<int64_t>(kGpb), ix.granules - g0));
+        int ga = 0, gb = gpb; // active granules [ga, gb) of the block
+        if (thr > 0.0F) {
+            float ub = 0.0F;
+            for (int j = 0; j < n; ++j) {
+                if (ReadLane(info.y, j) == 0)
+                    continue;
+                const int32_t sl = static_cast<int32_t>(ReadLane(info.z, j));
+                const float mx = __uint_as_float(ReadLane(info.w, j));
+                if (static_cast<int>(lane) < gpb)
+                    ub += sl >= 0 ? ix.bmax[static_cast<int64_t>(sl) * ix.granules + g0 + lane] : mx;
             }
-            if (lane < static_cast<int>(nseg)) {
-                sLo[lane] = sg.x;
-                sWord[lane] = sg.y & 0xFFu;
-                sPre[lane + 1] = x;
-                sCol[lane + 1] = c;
+            const unsigned long long active = __ballot(static_cast<int>(lane) < gpb && ub >= thr);
+            if (!active) {
+                ++skipped;
+                continue;
             }
-            const uint32_t hb = WaveHistBound(wk.hist + static_cast<int64_t>(q) * kHistBins, static_cast<uint32_t>(k));
-            if (lane == 0) {
-                sPre[0] = 0;
-                sCol[0] = 0;
-                sShared[0] = max(max(__atomic_load_n(&wk.tau[q], __ATOMIC_RELAXED), hb), 1u);
-                sShared[2] = (thr > 0.0F && nseg == 0) ? 0u : 1u;
-                sShared[3] = nseg;
+            ga = __ffsll(static_cast<long long>(active)) - 1;
+            gb = 64 - __clzll(static_cast<long long>(active));
+        }
+        const int32_t a = base + ga * kGranule;
+        const int32_t b = min(base + gb * kGranule, end);
+
+        // 1. Posting range of each word in [a, b) (lane j: word j).
+        uint32_t lo = 0, len = 0;
+        const int32_t sl = static_cast<int32_t>(info.z);
+        const bool isLong = info.y > 0 && sl >= 0;
+        const uint32_t *dirRow = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
+        if (isLong) {
+            if (cursorTask == task) {
+                lo = hiPrev;
+                len = hiNext - lo;
+            } else {
+                lo = info.x + dirRow[g0 + ga];
+                len = info.x + dirRow[g0 + gb] - lo;
             }
         }
-        float4 *acc4 = reinterpret_cast<float4 *>(acc);
+        hiPrev = lo + len;
+        // Next block's end, loaded now for the next task when it is this query's next block.
+        cursorTask = 0xFFFFFFFFu;
+        if (thr <= 0.0F && task + 1 < tEnd && blk + 1 < blocks) {
+            cursorTask = task + 1;
+            if (isLong)
+                hiNext = info.x + dirRow[min(g0 + 2 * kGpb, ix.granules)];
+        }
+        unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
+        while (shortMask) {
+            const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
+            shortMask &= shortMask - 1;
+            const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
+            const uint32_t i0 = (first + 15u) >> kSampleShift, i1 = (first + cnt + 15u) >> kSampleShift;
+            uint32_t ca = 0, cb = 0;
 #pragma unroll
-        for (int r = 0; r < kRows; ++r)
-            acc4[r * kThreads + tid] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
-        __syncthreads();
-        BM25MI_PHASE(0);
-        if (sShared[2] == 0) {
-            ++skipped; // every granule's score upper bound is below the threshold
+            for (int r = 0; r < kSampleRounds; ++r) {
+                const uint32_t i = i0 + r * kWave + lane;
+                if (i < i1) {
+                    const int32_t s = ix.samples[i];
+                    ca += s < a ? 1u : 0u;
+                    cb += s < b ? 1u : 0u;
+                }
+            }
+            ca = WaveSum(ca);
+            cb = WaveSum(cb);
+            const uint32_t ws = ca > 0 ? ((i0 + ca - 1) << kSampleShift) + 1 : first;
+            const uint32_t we = cb < i1 - i0 ? (i0 + cb) << kSampleShift : first + cnt;
+            if (static_cast<int>(lane) == j) {
+                lo = ws;
+                len = we > ws ? we - ws : 0u;
+            }
+        }
+        // Column prefix: lane j holds the end of word j's 64-posting columns; lanes past the
+        // last word hold ~0 so that column c belongs to word popcount(ballot(end <= c)).
+        const uint32_t myCols = (len + kWave - 1) / kWave;
+        uint32_t colEnd = myCols;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(colEnd), o));
+            if (lane >= static_cast<uint32_t>(o))
+                colEnd += y;
+        }
+        const uint32_t columns = ReadLane(colEnd, kWave - 1);
+        if (columns == 0) {
+            ++skipped; // no query word has postings here: every score is 0
             continue;
         }
         ++scored;
-        uint32_t tau = sShared[0];
-        if (thr > 0.0F)
-            tau = max(tau, __float_as_uint(thr));
+        const uint32_t colStart = colEnd - myCols;
+        if (static_cast<int>(lane) >= n)
+            colEnd = 0xFFFFFFFFu;
 
-        // Term-ordered FP32 accumulation. Each word's slice is cut into 256-posting
-        // columns; a round issues up to U columns' loads (one 8-byte load per thread per
-        // column) before any add, then adds them column by column with a barrier where the
-        // word changes. Within one word each doc appears once, so lanes never collide, and
-        // every doc's score is summed in query-word order.
-        const uint32_t nseg = sShared[3];
-        const uint32_t columns = sCol[nseg];
-        if (tid == 0)
-            scanned += sPre[nseg];
+        // 2. Stream and add, word by word in query order.
+        BM25MI_PHASE(1);
+        float4 *acc4 = reinterpret_cast<float4 *>(acc);
+        const uint32_t span = static_cast<uint32_t>(b - a), shift = static_cast<uint32_t>(a - base);
+        uint32_t inRange = 0;
         for (uint32_t r0 = 0; r0 < columns; r0 += U) {
-            if (wave == 0 && lane < U) {
-                const uint32_t col = r0 + lane;
-                uint2 entry = make_uint2(0u, 0xFFFF0000u);
-                if (col < columns) {
-                    int sg = 0;
-                    while (sCol[sg + 1] <= col)
-                        ++sg;
-                    const uint32_t i = col - sCol[sg];
-                    const uint32_t len = sPre[sg + 1] - sPre[sg];
-                    entry = make_uint2(sLo[sg] + i * kThreads, min(static_cast<uint32_t>(kThreads), len - i * kThreads) |
-                                                                   (sWord[sg] << 16));
-                }
-                sRound[lane] = entry;
-            }
-            __syncthreads();
+            // Lanes past a column's end keep doc 0xFFFFFFFF, which the range test drops.
             uint2 e[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint2 c = sRound[u];
-                if (static_cast<uint32_t>(tid) < (c.y & 0xFFFFu))
-                    e[u] = ix.post[c.x + tid];
-            }
-            // Unused columns carry word 0xFFFF and count 0: no barrier, no add. The barrier
-            // condition reads LDS values shared by the whole workgroup.
-            uint32_t word = sRound[0].y >> 16;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t meta = sRound[u].y;
-                const uint32_t w = meta >> 16;
-                if (w != 0xFFFFu && w != word) {
-                    __syncthreads();
-                    word = w;
+                e[u] = make_uint2(0xFFFFFFFFu, 0u);
+                const uint32_t c = r0 + u;
+                if (c < columns) {
+                    const int j = __popcll(__ballot(colEnd <= c));
+                    const uint32_t off = (c - ReadLane(colStart, j)) << 6;
+                    if (lane + off < ReadLane(len, j))
+                        e[u] = ix.post[ReadLane(lo, j) + off + lane];
                 }
-                if (static_cast<uint32_t>(tid) < (meta & 0xFFFFu))
-                    acc[static_cast<int32_t>(e[u].x) - base] += __uint_as_float(e[u].y);
             }
-            __syncthreads();
+            if (r0 == 0) {
+#pragma unroll
+                for (int r = 0; r < kVec; ++r)
+                    acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+            }
+            // Read-add-write each column in order. When the whole round lies in one word, its
+            // docs are distinct, so all reads may go first (no column can see another's write).
+            const uint32_t cLast = min(r0 + U, columns) - 1;
+            if (__popcll(__ballot(colEnd <= r0)) == __popcll(__ballot(colEnd <= cLast))) {
+                float v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = e[u].x - static_cast<uint32_t>(a);
+                    v[u] = rel < span ? acc[rel + shift] : 0.0F;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = e[u].x - static_cast<uint32_t>(a);
+                    if (rel < span) {
+                        acc[rel + shift] = v[u] + __uint_as_float(e[u].y);
+                        ++inRange;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = e[u].x - static_cast<uint32_t>(a);
+                    if (rel < span) {
+                        acc[rel + shift] += __uint_as_float(e[u].y);
+                        ++inRange;
+                    }
+                }
+            }
         }
-        BM25MI_PHASE(1);
+        uint32_t loaded = 0;
+#pragma unroll 1
+        for (int w = 0; w < n; ++w)
+            loaded += ReadLane(len, w);
+        scanned += loaded;
+        useful += inRange;
 
-        // Block selection. Scores are re-read from LDS in each pass (8 x 16-byte reads per
-        // thread) so they do not occupy registers across the passes.
+        // 3. Selection.
+        BM25MI_PHASE(2);
+        uint32_t floor = max(tauq, WaveHistBoundRegs(h, k));
+        floor = max(floor, 1u);
+        if (thr > 0.0F)
+            floor = max(floor, __float_as_uint(thr));
         const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
-        auto countIf = [&](auto pred) {
-            uint32_t c = 0;
-#pragma unroll 2
-            for (int r = 0; r < kRows; ++r) {
-                const uint4 v = bits4[r * kThreads + tid];
-                const uint32_t e = r * 4 * kThreads + tid * 4;
-                c += (pred(v.x, e) ? 1u : 0u) + (pred(v.y, e + 1) ? 1u : 0u) + (pred(v.z, e + 2) ? 1u : 0u) +
-                     (pred(v.w, e + 3) ? 1u : 0u);
-            }
-            return c;
-        };
-        auto countGe = [&](uint32_t x) {
-            return BlockSum<kThreads>(countIf([x](uint32_t b, uint32_t) { return b >= x; }), sRed);
-        };
-        auto keyOf = [base](uint32_t b, uint32_t e) {
-            return (static_cast<unsigned long long>(b) << 32) |
-                   static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e));
-        };
-        unsigned long long *pool = wk.pool + bt.poolOff[q];
-        uint32_t *hist = wk.hist + static_cast<int64_t>(q) * kHistBins;
-        const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
-        const uint32_t kk = static_cast<uint32_t>(k);
-        uint32_t floor = tau;
         uint32_t mine = 0, mx = 0;
-#pragma unroll 2
-        for (int r = 0; r < kRows; ++r) {
-            const uint4 v = bits4[r * kThreads + tid];
+#pragma unroll
+        for (int r = 0; r < kVec; ++r) {
+            const uint4 v = bits4[r * kWave + lane];
             mine += (v.x >= floor ? 1u : 0u) + (v.y >= floor ? 1u : 0u) + (v.z >= floor ? 1u : 0u) +
                     (v.w >= floor ? 1u : 0u);
             mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
         }
-        uint32_t candidates = BlockSum<kThreads>(mine, sRed);
-        if (candidates > 0) {
-            uint32_t cut = floor, tieIdx = B; // keep bits > cut, or bits == cut at index <= tieIdx
-            uint32_t kth = 0;                 // the block's K-th score bits when it has more than K
-            bool sorted = false;
-            if (candidates > kk && kk <= static_cast<uint32_t>(kThreads)) {
-                if (candidates > static_cast<uint32_t>(kThreads)) {
-                    // Raise the floor to the kk-th largest per-thread maximum: kk docs reach
-                    // it, so the block's kk-th score is at least that.
-                    sTop[tid] = mx;
-                    __syncthreads();
-                    BitonicDesc<kThreads>(sTop, kThreads);
-                    floor = max(floor, static_cast<uint32_t>(sTop[kk - 1]));
-                    __syncthreads();
-                    mine = countIf([floor](uint32_t b, uint32_t) { return b >= floor; });
-                    candidates = BlockSum<kThreads>(mine, sRed);
-                }
-                if (candidates <= static_cast<uint32_t>(kThreads)) {
-                    // Few enough to sort: compact their composite keys and sort them; the
-                    // first kk are the block's top-kk.
-                    uint32_t count = 0;
-                    uint32_t at = BlockExclusive<kThreads>(mine, sRed, &count);
-#pragma unroll 1
-                    for (int r = 0; r < kRows; ++r) {
-                        const uint4 v = bits4[r * kThreads + tid];
-                        const uint32_t e = r * 4 * kThreads + tid * 4;
-                        if (v.x >= floor)
-                            sTop[at++] = keyOf(v.x, e);
-                        if (v.y >= floor)
-                            sTop[at++] = keyOf(v.y, e + 1);
-                        if (v.z >= floor)
-                            sTop[at++] = keyOf(v.z, e + 2);
-                        if (v.w >= floor)
-                            sTop[at++] = keyOf(v.w, e + 3);
-                    }
-                    __syncthreads();
-                    if (static_cast<uint32_t>(tid) >= count)
-                        sTop[tid] = 0ULL;
-                    __syncthreads();
-                    BitonicDesc<kThreads>(sTop, kThreads);
-                    kth = static_cast<uint32_t>(sTop[kk - 1] >> 32);
-                    sorted = true;
-                }
-            }
-            if (!sorted && candidates > kk) {
-                // General exact cut: kk-th largest score bits by bisection,
-                // count(>= lo) >= kk > count(>= hi), then ties by lowest doc ID.
-                mx = BlockMax<kThreads>(mx, sRed);
-                uint32_t lo = floor, hi = mx + 1;
-                while (hi - lo > 1) {
-                    const uint32_t mid = lo + (hi - lo) / 2;
-                    if (countGe(mid) >= kk)
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                cut = lo;
-                kth = cut;
-                const uint32_t need = kk - countGe(lo + 1);
-                const uint32_t c0 = cut;
-                const uint32_t ties = BlockSum<kThreads>(countIf([c0](uint32_t b, uint32_t) { return b == c0; }), sRed);
-                if (ties > need) {
-                    // Smallest local index x with #{ties at index <= x} >= need.
-                    int32_t a = -1, z = B - 1; // count(<= a) < need <= count(<= z)
-                    while (z - a > 1) {
-                        const int32_t mid = a + (z - a) / 2;
-                        const uint32_t c = countIf([c0, mid](uint32_t b, uint32_t e) {
-                            return b == c0 && static_cast<int32_t>(e) <= mid;
-                        });
-                        if (BlockSum<kThreads>(c, sRed) >= need)
-                            z = mid;
-                        else
-                            a = mid;
-                    }
-                    tieIdx = static_cast<uint32_t>(z);
-                }
-            }
-            // Append the kept docs and count them in the histogram.
-            uint32_t kept;
-            if (sorted) {
-                kept = kk;
-                if (tid == 0)
-                    sShared[1] = atomicAdd(&wk.count[q], kk);
-                __syncthreads();
-                if (static_cast<uint32_t>(tid) < kk) {
-                    const uint32_t slot = sShared[1] + tid;
-                    if (static_cast<int64_t>(slot) < cap)
-                        pool[slot] = sTop[tid];
-                    else
-                        ++overflow;
-                    atomicAdd(hist + HistBucket(static_cast<uint32_t>(sTop[tid] >> 32)), 1u);
-                }
-            } else {
-                auto keepIt = [cut, tieIdx](uint32_t b, uint32_t e) { return b > cut || (b == cut && e <= tieIdx); };
-                mine = countIf(keepIt);
-                uint32_t before = BlockExclusive<kThreads>(mine, sRed, &kept);
-                if (tid == 0)
-                    sShared[1] = atomicAdd(&wk.count[q], kept);
-                __syncthreads();
-                if (mine > 0) {
-                    uint32_t slot = sShared[1] + before;
-#pragma unroll 1
-                    for (int r = 0; r < kRows; ++r) {
-                        const uint4 v = bits4[r * kThreads + tid];
-                        const uint32_t e = r * 4 * kThreads + tid * 4;
-                        const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
+        uint32_t kept = WaveSum(mine);
+        if (kept == 0)
+            continue;
+        // Keep elements whose key (bits << kIdxBits | (B-1-e)) is at least (tb, te).
+        uint32_t tb = floor, te = 0, tauNew = 0;
+        auto countKey = [&](uint32_t xb, uint32_t xe) {
+            uint32_t c = 0;
 #pragma unroll
-                        for (int cc = 0; cc < 4; ++cc)
-                            if (keepIt(vals[cc], e + cc)) {
-                                if (static_cast<int64_t>(slot) < cap)
-                                    pool[slot] = keyOf(vals[cc], e + cc);
-                                else
-                                    ++overflow;
-                                ++slot;
-                                atomicAdd(hist + HistBucket(vals[cc]), 1u);
-                            }
+            for (int r = 0; r < kVec; ++r) {
+                const uint4 v = bits4[r * kWave + lane];
+                const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
+                c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) +
+                     (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
+                     (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) +
+                     (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
+            }
+            return c;
+        };
+        const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
+        if (kept > cap) {
+            ++cuts;
+            if (k <= static_cast<uint32_t>(kWave)) {
+                const uint32_t f = max(WaveKthLargest(mx, k), floor);
+                mine = countKey(f, 0);
+                kept = WaveSum(mine);
+                tb = f;
+                if (kept <= cap)
+                    tauNew = f; // at least k docs of the block score >= f
+            }
+            if (kept > cap) {
+                // Bisection over composite keys: count(>= lo) > cap, count(>= hi) < k.
+                unsigned long long lok = static_cast<unsigned long long>(tb) << kIdxBits;
+                unsigned long long hik = (static_cast<unsigned long long>(WaveMax(mx)) + 1) << kIdxBits;
+                for (int it = 0; it < 64 && hik - lok > 1; ++it) {
+                    const unsigned long long mid = lok + (hik - lok) / 2;
+                    const uint32_t mb = static_cast<uint32_t>(mid >> kIdxBits);
+                    const uint32_t me = static_cast<uint32_t>(mid) & static_cast<uint32_t>(B - 1);
+                    const uint32_t cm = WaveSum(countKey(mb, me));
+                    if (cm > cap) {
+                        lok = mid;
+                    } else if (cm < k) {
+                        hik = mid;
+                    } else {
+                        lok = mid;
+                        break;
                     }
                 }
-            }
-            if (kth) {
-                ++exactTasks;
-                if (tid == 0)
-                    atomicMax(&wk.tau[q], kth); // the block's K-th score bounds the query's
-            }
-            if (tid == 0 && kept) {
-                emitted += kept;
-                ++emitting;
+                tb = static_cast<uint32_t>(lok >> kIdxBits);
+                te = static_cast<uint32_t>(lok) & static_cast<uint32_t>(B - 1);
+                mine = countKey(tb, te);
+                kept = WaveSum(mine);
+                if (kept >= k)
+                    tauNew = tb;
             }
         }
-        BM25MI_PHASE(2);
+
+        // Append the kept docs to the pool and count them in the histogram.
+        BM25MI_PHASE(3);
+        uint32_t before = mine;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
+            if (lane >= static_cast<uint32_t>(o))
+                before += y;
+        }
+        before -= mine;
+        uint32_t slot0 = 0;
+        if (lane == 0)
+            slot0 = atomicAdd(&wk.count[q], kept);
+        slot0 = ReadLane(slot0, 0);
+        unsigned long long *pool = wk.pool + bt.poolOff[q];
+        const int64_t capq = bt.poolOff[q + 1] - bt.poolOff[q];
+        uint32_t *histw = wk.hist + static_cast<int64_t>(q) * kHistBins;
+#pragma unroll
+        for (int r = 0; r < kHistPerLane; ++r)
+            if (lane * kHistPerLane + r < kHistBins)
+                sHist[lane * kHistPerLane + r] = 0;
+        if (mine) {
+            uint32_t slot = slot0 + before;
+#pragma unroll 1
+            for (int r = 0; r < kVec; ++r) {
+                const uint4 v = bits4[r * kWave + lane];
+                const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
+                const uint32_t e0 = (r * kWave + lane) * 4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint32_t x = vals[c], rk = static_cast<uint32_t>(B - 1) - (e0 + c);
+                    if (x > tb || (x == tb && rk >= te)) {
+                        if (static_cast<int64_t>(slot) < capq)
+                            pool[slot] = (static_cast<unsigned long long>(x) << 32) |
+                                         static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e0 + c));
+                        else
+                            ++overflow;
+                        ++slot;
+                        atomicAdd(sHist + HistBucket(x), 1u);
+                    }
+                }
+            }
+        }
+        // One global atomic per non-empty bucket of this append.
+#pragma unroll
+        for (int r = 0; r < kHistPerLane; ++r) {
+            const uint32_t bin = lane * kHistPerLane + r;
+            if (bin < kHistBins) {
+                const uint32_t c = sHist[bin];
+                if (c)
+                    atomicAdd(histw + bin, c);
+            }
+        }
+        if (lane == 0 && tauNew)
+            atomicMax(&wk.tau[q], tauNew);
+        emitted += kept;
+        ++emitting;
     }
-#ifdef BM25MI_PHASES
-    if (tid == 0)
-        for (int i = 0; i < 3; ++i)
-            atomicAdd(&wk.stats[kStatPhase + i], phase[i]);
-#endif
+    BM25MI_PHASE(0);
 #undef BM25MI_PHASE
-    if (tid == 0) {
-        atomicAdd(&wk.stats[kStatPostings], scanned);
-        atomicAdd(&wk.stats[kStatScored], scored);
-        atomicAdd(&wk.stats[kStatSkipped], skipped);
-        atomicAdd(&wk.stats[kStatCandidates], emitted);
-        atomicAdd(&wk.stats[kStatEmitting], emitting);
-        atomicAdd(&wk.stats[kStatExact], exactTasks);
+    // Counters go to this wave's own row (plain stores); the final kernel sums the rows.
+    useful = WaveSum64(useful);
+    overflow = WaveSum64(overflow);
+    if (lane == 0) {
+        unsigned long long *row = wk.stats + static_cast<int64_t>(blockIdx.x) * kStats;
+        row[kStatPostings] = scanned;
+        row[kStatScored] = scored;
+        row[kStatSkipped] = skipped;
+        row[kStatCandidates] = emitted;
+        row[kStatOverflow] = overflow;
+        row[kStatEmitting] = emitting;
+        row[kStatExact] = cuts;
+        row[kStatUseful] = useful;
+#ifdef BM25MI_PHASES
+        for (int i = 0; i < 4; ++i)
+            row[kStatPhase + i] = phase[i];
+        row[kStatMax] = phase[0] + phase[1] + phase[2] + phase[3]; // busy time
+        row[kStatMax + 1] = stamp;                                  // end
+        row[kStatMax + 2] = ~waveStart;                             // start (max of ~ = min)
+        row[kStatMax + 3] = waveStart;                              // latest start
+#else
+        for (int i = kStatPhase; i < kStats; ++i)
+            row[i] = 0;
+#endif
     }
-    if (overflow)
-        atomicAdd(&wk.stats[kStatOverflow], overflow);
 }
 
 // Initial state of the per-query counters; run once at session creation. After that
@@ -693,8 +641,6 @@ __global__ void KReset(WorkView wk, int32_t maxQ)
     }
     if (i < static_cast<int64_t>(maxQ) * kHistBins)
         wk.hist[i] = 0;
-    if (i < kStats)
-        wk.stats[i] = 0;
 }
 
 // The k-th largest of p unique keys in global memory (p > k), by a 64-step bisection
@@ -717,7 +663,8 @@ __device__ unsigned long long KthFromGlobal(const unsigned long long *keys, uint
 // final lower bound tau cannot be in the top-K and are dropped; the rest are sorted in
 // LDS (score desc, doc asc) when they fit, else reduced by a 64-step bisection first.
 // Fewer than K positive docs are padded with the lowest-ID zero-score docs.
-__global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView wk, unsigned long long *hostStats)
+__global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView wk, unsigned long long *hostStats,
+                                                         int32_t scoreGrid)
 {
     __shared__ unsigned long long sKeys[kSortCap];
     __shared__ unsigned long long sIds[1024];
@@ -725,14 +672,30 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     __shared__ uint32_t sN;
     const int q = blockIdx.x;
     const int32_t nq = bt.header[0];
+    const int tid = threadIdx.x;
+    {
+        // The scoring kernel has finished: this workgroup sums its share of the per-wave
+        // counter rows (64 threads per counter; counters 12..15 take the maximum) into its
+        // row of hostStats, which the host adds up.
+        static_assert(kFinalThreads == 64 * kStats, "one wave per counter");
+        const int stat = tid >> 6;
+        const int per = (scoreGrid + gridDim.x - 1) / gridDim.x;
+        const int r0 = q * per, r1 = min(scoreGrid, r0 + per);
+        unsigned long long acc = 0;
+        for (int r = r0 + (tid & 63); r < r1; r += 64) {
+            const unsigned long long v = wk.stats[static_cast<int64_t>(r) * kStats + stat];
+            acc = stat >= kStatMax ? max(acc, v) : acc + v;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long y = __shfl_xor(acc, o);
+            acc = stat >= kStatMax ? max(acc, y) : acc + y;
+        }
+        if ((tid & 63) == 0)
+            hostStats[static_cast<int64_t>(q) * kStats + stat] = acc;
+    }
     if (q >= nq)
         return;
-    const int tid = threadIdx.x;
-    if (q == 0 && tid < kStats) {
-        // The scoring kernel has finished: publish its counters and clear them.
-        hostStats[tid] = wk.stats[tid];
-        wk.stats[tid] = 0;
-    }
     const uint32_t k = static_cast<uint32_t>(bt.k[q]);
     const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
     const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(wk.count[q]), cap));
@@ -743,7 +706,7 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     if (tid < 64) {
         const uint32_t hb = WaveHistBound(hist, k);
         if (tid == 0)
-            sRed[0] = max(max(wk.tau[q], hb), 1u);
+            sRed[0] = max(max(max(wk.tau[q], hb), bt.tau0[q]), 1u);
     }
     __syncthreads();
     const uint32_t tau = sRed[0];
@@ -843,15 +806,9 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 {
     const int threads = 256;
-    const int grid = static_cast<int>((max(static_cast<int64_t>(maxQ) * kHistBins, static_cast<int64_t>(kStats)) +
+    const int grid = static_cast<int>((static_cast<int64_t>(maxQ) * kHistBins +
                                        threads - 1) / threads);
     hipLaunchKernelGGL(KReset, dim3(grid), dim3(threads), 0, stream, wk, maxQ);
-}
-
-void LaunchPlan(const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks, int32_t blockDocs,
-                int32_t granulesPerBlock, int32_t grid, hipStream_t stream)
-{
-    hipLaunchKernelGGL(KPlan, dim3(grid), dim3(256), 0, stream, ix, bt, wk, blocks, blockDocs, granulesPerBlock);
 }
 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
@@ -859,33 +816,33 @@ bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const 
 {
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kThreads), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
         return true;
     default:
         return false;
     }
 }
 
-// Resident scoring workgroups per CU for a block size (registers and LDS), or 0.
+// Resident scoring waves per CU for a block size (registers and LDS), or 0.
 int ScoreBlocksPerCu(int blockDocs)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (blockDocs) {
     case 2048:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<2048>, kThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<2048>, kWave, 0);
         break;
     case 4096:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<4096>, kThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<4096>, kWave, 0);
         break;
     case 8192:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<8192>, kThreads, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KScore<8192>, kWave, 0);
         break;
     default:
         break;
@@ -894,9 +851,9 @@ int ScoreBlocksPerCu(int blockDocs)
 }
 
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
-                 hipStream_t stream)
+                 int32_t scoreGrid, hipStream_t stream)
 {
-    hipLaunchKernelGGL(KFinal, dim3(maxQ), dim3(kFinalThreads), 0, stream, bt, wk, hostStats);
+    hipLaunchKernelGGL(KFinal, dim3(maxQ), dim3(kFinalThreads), 0, stream, bt, wk, hostStats, scoreGrid);
 }
 
 } // namespace bm25mi

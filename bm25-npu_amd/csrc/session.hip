@@ -30,13 +30,11 @@
 namespace bm25mi {
 
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
-void LaunchPlan(const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks, int32_t blockDocs,
-                int32_t granulesPerBlock, int32_t grid, hipStream_t stream);
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream);
 int ScoreBlocksPerCu(int blockDocs);
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
-                 hipStream_t stream);
+                 int32_t scoreGrid, hipStream_t stream);
 
 #define BM25MI_HIP(expr, what)                                                                                         \
     do {                                                                                                               \
@@ -59,22 +57,26 @@ class Session {
 public:
     bm25mi_config cfg{};
     int64_t ndocs = 0, vocab = 0, postings = 0;
-    int32_t blockDocs = 8192, granule = 1024, granulesPerBlock = 8, blocks = 0, grid = 0, cus = 0;
+    int32_t blockDocs = 2048, granule = kGranule, blocks = 0, grid = 0, cus = 0;
     int64_t poolPerQuery = 0;
+    int64_t taskCost = kTaskCost;
     std::vector<int64_t> toffHost;
     std::vector<float> idfHost, tmaxHost, tmeanHost;
     std::vector<int32_t> slotHost;
     std::vector<float> bmaxHost;
+    std::vector<int64_t> scratchCost;
+    std::vector<int64_t> topOffHost;
+    std::vector<float> topValHost;
     int64_t granules = 0;
 
     // Device memory.
-    void *dPost = nullptr, *dToff = nullptr, *dSlot = nullptr, *dDir = nullptr, *dBmax = nullptr, *dTmax = nullptr;
+    void *dPost = nullptr, *dSamples = nullptr, *dDir = nullptr, *dBmax = nullptr;
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr, *dSegs = nullptr, *dSegCount = nullptr;
+    void *dHist = nullptr;
     // Pinned host memory: request packet (copied by the graph) and result block (written by the device).
     void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     size_t packetBytes = 0, resultBytes = 0;
-    size_t offQoff = 0, offK = 0, offThr = 0, offPool = 0, offOut = 0, offTerms = 0;
+    size_t offQoff = 0, offK = 0, offThr = 0, offTau0 = 0, offPool = 0, offOut = 0, offTerms = 0, offWave = 0;
     size_t resStatus = 0, resScores = 0, resIds = 0;
 
     IndexView ix{};
@@ -84,7 +86,7 @@ public:
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int device = 0;
 
     bool failed = false, closed = false;
@@ -97,11 +99,50 @@ public:
     ~Session() { Release(); }
 
     void Release() noexcept;
-    // Candidate pool entries one query may need: every block keeps at most max(K, kEmitCap)
-    // docs, and no doc is kept twice.
+    // Candidate pool entries one query may need: every block appends at most
+    // BlockKeepCap(K) docs, and no doc is appended twice.
     int64_t PoolCapacity(int32_t k) const
     {
-        return std::min<int64_t>(static_cast<int64_t>(blocks) * std::max<int32_t>(k, kEmitCap), ndocs);
+        return std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs), ndocs);
+    }
+    // Score bits of the largest K-th term impact among the query's terms: every doc scores at
+    // least each of its impacts, so the query's K-th score is at least this (0 if unknown).
+    uint32_t StartBound(const int32_t *terms, int64_t n, int32_t k) const
+    {
+        float best = 0.0F;
+        for (int64_t w = 0; w < n; ++w) {
+            const int32_t t = terms[w];
+            if (t >= vocab || topOffHost[t + 1] - topOffHost[t] < k)
+                continue;
+            best = std::max(best, topValHost[topOffHost[t] + k - 1]);
+        }
+        uint32_t bits;
+        std::memcpy(&bits, &best, 4);
+        return bits;
+    }
+    // Cuts the query-major task order into `grid` contiguous ranges of equal estimated work,
+    // a task of query q costing taskCost + ceil(postings of q / blocks).
+    void PlanWaves(const int64_t *postingsPerQuery, int32_t nq, uint32_t *waveStart) const
+    {
+        std::vector<int64_t> cost(static_cast<size_t>(nq));
+        int64_t all = 0;
+        for (int32_t q = 0; q < nq; ++q) {
+            cost[q] = taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
+            all += cost[q] * blocks;
+        }
+        int32_t q = 0;
+        int64_t before = 0; // cost of queries < q
+        for (int32_t w = 0; w <= grid; ++w) {
+            const int64_t target = static_cast<int64_t>((static_cast<__int128>(all) * w) / grid);
+            while (q < nq && before + cost[q] * blocks <= target) {
+                before += cost[q] * blocks;
+                ++q;
+            }
+            int64_t task = static_cast<int64_t>(q) * blocks;
+            if (q < nq)
+                task += std::min<int64_t>(blocks, (target - before + cost[q] - 1) / cost[q]);
+            waveStart[w] = static_cast<uint32_t>(task);
+        }
     }
     void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
     void Run(bool timed);
@@ -126,7 +167,10 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(cfg.max_words >= 0 && cfg.max_words <= 16777216, "max_words must be in [0, 2^24]");
     Require(cfg.max_k >= 1 && cfg.max_k <= BM25MI_MAX_K, "max_k must be in [1, 1024]");
     Require(std::isfinite(cfg.k1) && cfg.k1 > 0, "k1 must be finite and > 0");
-    blockDocs = cfg.block_docs ? cfg.block_docs : 8192;
+    Require(cfg.task_cost >= 0, "task_cost must be >= 0");
+    if (cfg.task_cost > 0)
+        taskCost = cfg.task_cost;
+    blockDocs = cfg.block_docs ? cfg.block_docs : 2048;
     Require(blockDocs == 2048 || blockDocs == 4096 || blockDocs == 8192, "block_docs must be 2048, 4096 or 8192");
     ValidateIndex(offsets, v, docs, tf, m, norm, n, idf, 0);
     Require(m >= 1, "Index needs at least one posting");
@@ -134,8 +178,6 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     device = cfg.device_id;
 
     // Host-side tables: impacts, directory, block maxima.
-    granule = kGranule;
-    granulesPerBlock = blockDocs / granule;
     blocks = static_cast<int32_t>((n + blockDocs - 1) / blockDocs);
     Layout layout;
     std::vector<float> impact(static_cast<size_t>(m));
@@ -147,6 +189,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     tmaxHost = layout.termMax;
     tmeanHost = layout.termMean;
     slotHost = layout.termSlot;
+    topOffHost.swap(layout.topOff);
+    topValHost.swap(layout.topVal);
 
     int count = 0;
     BM25MI_HIP(hipGetDeviceCount(&count), "count devices");
@@ -158,11 +202,12 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     const int perCu = ScoreBlocksPerCu(blockDocs);
     Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
+    Require(static_cast<int64_t>(cfg.max_queries) * blocks < (int64_t(1) << 32), "Too many scoring tasks per Search");
 
     size_t freeBytes = 0, totalBytes = 0;
     BM25MI_HIP(hipMemGetInfo(&freeBytes, &totalBytes), "memory info");
     poolPerQuery = PoolCapacity(cfg.max_k);
-    const uint64_t known = static_cast<uint64_t>(m) * 8 + (v + 1) * 8 + v * 8 +
+    const uint64_t known = static_cast<uint64_t>(m) * 8 + static_cast<uint64_t>(m) / 4 + 4 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
                            static_cast<uint64_t>(cfg.max_queries) * poolPerQuery * 8;
     // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
@@ -186,12 +231,15 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
         dPost = Alloc<uint2>(m, "alloc postings");
         BM25MI_HIP(hipMemcpy(dPost, packed.data(), m * sizeof(uint2), hipMemcpyHostToDevice), "upload postings");
     }
-    dToff = Alloc<int64_t>(v + 1, "alloc term offsets");
-    BM25MI_HIP(hipMemcpy(dToff, offsets, (v + 1) * 8, hipMemcpyHostToDevice), "upload term offsets");
-    dSlot = Alloc<int32_t>(v, "alloc slots");
-    BM25MI_HIP(hipMemcpy(dSlot, layout.termSlot.data(), v * 4, hipMemcpyHostToDevice), "upload slots");
-    dTmax = Alloc<float>(v, "alloc term max");
-    BM25MI_HIP(hipMemcpy(dTmax, layout.termMax.data(), v * 4, hipMemcpyHostToDevice), "upload term max");
+    {
+        // Doc ID of every 16th posting: short lists are located through these samples.
+        const int64_t ns = (m + (1 << kSampleShift) - 1) >> kSampleShift;
+        std::vector<int32_t> samples(static_cast<size_t>(ns));
+        for (int64_t i = 0; i < ns; ++i)
+            samples[i] = docs[i << kSampleShift];
+        dSamples = Alloc<int32_t>(ns, "alloc posting samples");
+        BM25MI_HIP(hipMemcpy(dSamples, samples.data(), ns * 4, hipMemcpyHostToDevice), "upload posting samples");
+    }
     dDir = Alloc<uint32_t>(layout.dir.size(), "alloc directory");
     if (!layout.dir.empty())
         BM25MI_HIP(hipMemcpy(dDir, layout.dir.data(), layout.dir.size() * 4, hipMemcpyHostToDevice), "upload dir");
@@ -210,19 +258,23 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     at = Align(at + q * 4, 8);
     offThr = at;
     at = Align(at + q * 4, 8);
+    offTau0 = at;
+    at = Align(at + q * 4, 8);
     offPool = at;
     at = Align(at + (q + 1) * 8, 8);
     offOut = at;
     at = Align(at + (q + 1) * 8, 8);
     offTerms = at;
-    at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 16, 256);
+    at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 16, 16);
+    offWave = at;
+    at = Align(at + (static_cast<size_t>(grid) + 1) * 4, 256);
     packetBytes = at;
     BM25MI_HIP(hipHostMalloc(&hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
     std::memset(hPacket, 0, packetBytes);
     dPacket = Alloc<unsigned char>(packetBytes, "alloc device packet");
 
     // Result block in mapped pinned memory.
-    resStatus = kStats * 8;
+    resStatus = static_cast<size_t>(cfg.max_queries) * kStats * 8; // per final workgroup counter rows
     resScores = Align(resStatus + q * 4, 16);
     resIds = Align(resScores + q * cfg.max_k * 4, 16);
     resultBytes = Align(resIds + q * cfg.max_k * 4, 256);
@@ -233,27 +285,24 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     dPool = Alloc<unsigned long long>(q * poolPerQuery, "alloc candidate pools");
     dCount = Alloc<uint32_t>(q, "alloc pool counts");
     dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
-    dSegs = Alloc<uint2>(q * blocks * kMaxSegs, "alloc segment table");
-    dSegCount = Alloc<uint32_t>(q * blocks, "alloc segment counts");
     dTau = Alloc<uint32_t>(q, "alloc thresholds");
-    dStats = Alloc<unsigned long long>(kStats, "alloc stats");
+    dStats = Alloc<unsigned long long>(static_cast<size_t>(grid) * kStats, "alloc stats");
 
-    ix = IndexView{static_cast<const uint2 *>(dPost),  static_cast<const int64_t *>(dToff),
-                   static_cast<const int32_t *>(dSlot), static_cast<const uint32_t *>(dDir),
-                   static_cast<const float *>(dBmax),  static_cast<const float *>(dTmax),
-                   v, granules, static_cast<int32_t>(n), __builtin_ctz(static_cast<unsigned>(granule))};
+    ix = IndexView{static_cast<const uint2 *>(dPost), static_cast<const int32_t *>(dSamples),
+                   static_cast<const uint32_t *>(dDir), static_cast<const float *>(dBmax), granules,
+                   static_cast<int32_t>(n)};
     auto *pk = static_cast<unsigned char *>(dPacket);
     bt = BatchView{reinterpret_cast<const int32_t *>(pk),           reinterpret_cast<const int64_t *>(pk + offQoff),
                    reinterpret_cast<const int32_t *>(pk + offK),    reinterpret_cast<const float *>(pk + offThr),
+                   reinterpret_cast<const uint32_t *>(pk + offTau0),
                    reinterpret_cast<const int64_t *>(pk + offPool), reinterpret_cast<const int64_t *>(pk + offOut),
-                   reinterpret_cast<const uint4 *>(pk + offTerms)};
+                   reinterpret_cast<const uint4 *>(pk + offTerms), reinterpret_cast<const uint32_t *>(pk + offWave)};
     auto *rs = static_cast<unsigned char *>(hResultDev);
     wk = WorkView{static_cast<unsigned long long *>(dPool), static_cast<uint32_t *>(dCount),
                   static_cast<uint32_t *>(dHist),
                   static_cast<uint32_t *>(dTau),            static_cast<unsigned long long *>(dStats),
                   reinterpret_cast<int32_t *>(rs + resStatus), reinterpret_cast<float *>(rs + resScores),
-                  reinterpret_cast<int32_t *>(rs + resIds), static_cast<uint2 *>(dSegs),
-                  static_cast<uint32_t *>(dSegCount)};
+                  reinterpret_cast<int32_t *>(rs + resIds)};
     LaunchInit(wk, cfg.max_queries, stream);
     BM25MI_HIP(hipStreamSynchronize(stream), "initialise counters");
     if (cfg.use_graph)
@@ -265,14 +314,11 @@ void Session::Enqueue(bool timed)
 {
     BM25MI_HIP(hipMemcpyAsync(dPacket, hPacket, packetBytes, hipMemcpyHostToDevice, stream), "packet H2D");
     if (timed)
-        BM25MI_HIP(hipEventRecord(ev[4], stream), "record plan start");
-    LaunchPlan(ix, bt, wk, blocks, blockDocs, granulesPerBlock, cus * 8, stream);
-    if (timed)
         BM25MI_HIP(hipEventRecord(ev[1], stream), "record score start");
     Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, stream), "Unsupported block size");
     if (timed)
         BM25MI_HIP(hipEventRecord(ev[2], stream), "record score end");
-    LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(hResultDev), stream);
+    LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(hResultDev), grid, stream);
 }
 
 void Session::Capture()
@@ -307,11 +353,11 @@ void Session::Run(bool timed)
     if (timed) {
         BM25MI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]), "score kernel time");
         last.score_kernel_ms = ms;
-        BM25MI_HIP(hipEventElapsedTime(&ms, ev[4], ev[1]), "plan time");
-        last.pilot_ms = ms;
+        BM25MI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]), "final time");
+        last.final_ms = ms;
     } else {
         last.score_kernel_ms = -1;
-        last.pilot_ms = -1;
+        last.final_ms = -1;
     }
 }
 
@@ -341,6 +387,7 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
     auto *qoff = reinterpret_cast<int64_t *>(pk + offQoff);
     auto *kk = reinterpret_cast<int32_t *>(pk + offK);
     auto *thr = reinterpret_cast<float *>(pk + offThr);
+    auto *tau0 = reinterpret_cast<uint32_t *>(pk + offTau0);
     auto *pool = reinterpret_cast<int64_t *>(pk + offPool);
     auto *out = reinterpret_cast<int64_t *>(pk + offOut);
     qoff[0] = pool[0] = out[0] = 0;
@@ -351,26 +398,36 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
         thr[q] = query.thr ? query.thr[q] : 0.0F;
         pool[q + 1] = pool[q] + PoolCapacity(query.k[q]);
         out[q + 1] = out[q] + query.k[q];
+        tau0[q] = StartBound(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q]);
     }
     auto *info = reinterpret_cast<uint4 *>(pk + offTerms);
-    for (int64_t w = 0; w < words; ++w) {
-        const int32_t t = query.terms[w];
-        if (t < vocab) {
-            const int64_t len = toffHost[t + 1] - toffHost[t];
-            uint32_t ub;
-            std::memcpy(&ub, &tmaxHost[t], 4);
-            info[w] = make_uint4(static_cast<uint32_t>(toffHost[t]), static_cast<uint32_t>(len),
-                                 static_cast<uint32_t>(slotHost[t]), ub);
-            total += len;
-        } else {
-            info[w] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+    std::vector<int64_t> &perQuery = scratchCost;
+    perQuery.assign(static_cast<size_t>(nq), 0);
+    for (int32_t q = 0; q < nq; ++q)
+        for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w) {
+            const int32_t t = query.terms[w];
+            if (t < vocab) {
+                const int64_t len = toffHost[t + 1] - toffHost[t];
+                uint32_t ub;
+                std::memcpy(&ub, &tmaxHost[t], 4);
+                info[w] = make_uint4(static_cast<uint32_t>(toffHost[t]), static_cast<uint32_t>(len),
+                                     static_cast<uint32_t>(slotHost[t]), ub);
+                total += len;
+                perQuery[q] += len;
+            } else {
+                info[w] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+            }
         }
-    }
+    PlanWaves(perQuery.data(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
 
     try {
         Run(timed);
         auto *res = static_cast<unsigned char *>(hResult);
-        const auto *stats = reinterpret_cast<const unsigned long long *>(res);
+        const auto *rows = reinterpret_cast<const unsigned long long *>(res);
+        unsigned long long stats[kStats] = {};
+        for (int32_t r = 0; r < cfg.max_queries; ++r)
+            for (int i = 0; i < kStats; ++i)
+                stats[i] = i >= kStatMax ? std::max(stats[i], rows[r * kStats + i]) : stats[i] + rows[r * kStats + i];
         const auto *st = reinterpret_cast<const int32_t *>(res + resStatus);
         const auto *sc = reinterpret_cast<const float *>(res + resScores);
         const auto *id = reinterpret_cast<const int32_t *>(res + resIds);
@@ -393,13 +450,18 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
             std::memcpy(outScores + out[q], sc + out[q], (out[q + 1] - out[q]) * 4);
             std::memcpy(outIds + out[q], id + out[q], (out[q + 1] - out[q]) * 4);
         }
-        last.postings_scanned = static_cast<int64_t>(stats[kStatPostings] + stats[kStatPilot]);
-        for (int i = 0; i < 8; ++i)
-            last.phase_cycles[i] = static_cast<int64_t>(stats[kStatPhase + i]);
-        last.pilot_postings = static_cast<int64_t>(stats[kStatPilot]);
+        last.postings_scanned = static_cast<int64_t>(stats[kStatPostings]);
+        last.postings_in_range = static_cast<int64_t>(stats[kStatUseful]);
         last.emitting_blocks = static_cast<int64_t>(stats[kStatEmitting]);
         last.exact_blocks = static_cast<int64_t>(stats[kStatExact]);
         last.postings_total = total;
+        last.score_grid = grid;
+        for (int i = 0; i < 4; ++i)
+            last.phase_cycles[i] = static_cast<int64_t>(stats[kStatPhase + i]);
+        last.wave_busy_max = static_cast<int64_t>(stats[kStatMax]);
+        last.wave_span = static_cast<int64_t>(stats[kStatMax + 1] - ~stats[kStatMax + 2]);
+        last.wave_start_spread = static_cast<int64_t>(stats[kStatMax + 3] - ~stats[kStatMax + 2]);
+        last.block_docs = blockDocs;
         last.blocks_scored = static_cast<int64_t>(stats[kStatScored]);
         last.blocks_skipped = static_cast<int64_t>(stats[kStatSkipped]);
         last.candidates = static_cast<int64_t>(stats[kStatCandidates]);
@@ -443,8 +505,7 @@ void Session::Release() noexcept
         (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
-    for (void **p : {&dPost, &dToff, &dSlot, &dDir, &dBmax, &dTmax, &dPacket, &dPool, &dCount, &dTau, &dStats,
-                     &dHist, &dSegs, &dSegCount})
+    for (void **p : {&dPost, &dSamples, &dDir, &dBmax, &dPacket, &dPool, &dCount, &dTau, &dStats, &dHist})
         if (*p) {
             (void)hipFree(*p);
             *p = nullptr;
@@ -475,7 +536,7 @@ extern "C" void bm25mi_config_default(bm25mi_config *cfg)
     cfg->max_queries = 8;
     cfg->max_words = 24;
     cfg->max_k = 10;
-    cfg->block_docs = 8192;
+    cfg->block_docs = 2048;
     cfg->k1 = 1.2;
     cfg->use_graph = 1;
     cfg->grid_blocks = 0;
@@ -552,7 +613,7 @@ static int SearchCommon(bm25mi_session *s, const Query &query, float *scores, in
                     if (first.score_kernel_ms >= 0 && ss.last.score_kernel_ms >= 0)
                         ss.last.score_kernel_ms += first.score_kernel_ms;
                     ss.last.postings_scanned += first.postings_scanned;
-                    ss.last.pilot_postings += first.pilot_postings;
+                    ss.last.postings_in_range += first.postings_in_range;
                     ss.last.postings_total = first.postings_total;
                     ss.last.blocks_scored += first.blocks_scored;
                     ss.last.blocks_skipped += first.blocks_skipped;

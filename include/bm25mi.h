@@ -60,30 +60,37 @@ typedef struct {
     int32_t max_queries;     /* queryCount: largest batch one Search accepts */
     int64_t max_words;       /* maxQueryWords: total term IDs per batch */
     int32_t max_k;           /* topK upper bound; each Search names its own K <= max_k */
-    int32_t block_docs;      /* docs per scoring block: 2048, 4096, 8192 or 16384; 0 = 8192 */
+    int32_t block_docs;      /* docs per scoring task (one wave): 2048, 4096 or 8192; 0 = 2048 */
     double k1;               /* BM25 k1; impact = idf*(tf*(k1+1)) / (tf+norm) in FP32 */
     int32_t use_graph;       /* 1: replay one captured hipGraph per Search (default) */
-    int32_t grid_blocks;     /* persistent scoring workgroups; 0 = as many as fit the CUs */
+    int32_t grid_blocks;     /* persistent scoring waves; 0 = as many as fit the CUs */
+    int32_t task_cost;       /* estimated fixed cost of one scoring task in postings, used to give
+                                every wave equal work; 0 = library default */
 } bm25mi_config;
 
 /* Per-Search counters, filled by bm25mi_last_stats. */
 typedef struct {
     double device_ms;          /* HIP-event time of the device work of the last Search */
-    double score_kernel_ms;    /* HIP-event time of the scoring kernel alone */
+    double score_kernel_ms;    /* HIP-event time of the scoring kernel alone (timed Search only, else -1) */
     double host_ms;            /* wall time of the whole bm25mi_search* call */
-    int64_t postings_scanned;  /* postings streamed from HBM (pilot and main passes) */
+    int64_t postings_scanned;  /* postings streamed from HBM by the scoring kernel */
     int64_t postings_total;    /* postings of the query terms (plain-path work) */
     int64_t blocks_scored;     /* (query, block) tasks scored */
-    int64_t blocks_skipped;    /* (query, block) tasks skipped by a threshold */
+    int64_t blocks_skipped;    /* (query, block) tasks skipped (no postings, or below a threshold) */
     int64_t candidates;        /* entries written to the candidate pools */
     int32_t fallbacks;         /* queries rerun on the plain path */
     int32_t queries;
-    double pilot_ms;           /* HIP-event time of planning, pilot pass and threshold (timed Search only) */
-    int64_t pilot_postings;    /* postings streamed by the pilot pass */
+    double final_ms;           /* HIP-event time of the final selection kernel (timed Search only, else -1) */
+    int64_t postings_in_range; /* streamed postings inside their task's doc range */
     int64_t emitting_blocks;   /* scoring tasks that appended candidates */
-    int64_t exact_blocks;      /* tasks that needed an exact block top-K selection */
-    int64_t phase_cycles[8];   /* scoring cycles per phase (setup, postings, selection, sample) of the
-                                  main then the pilot pass; zero unless built with -DBM25MI_PHASES */
+    int64_t exact_blocks;      /* tasks that cut their candidates to the per-block cap */
+    int32_t score_grid;        /* scoring waves launched (persistent grid) */
+    int32_t block_docs;        /* docs per scoring task */
+    int64_t phase_cycles[4];   /* scoring-kernel shader cycles summed over waves per phase: locate postings,
+                                  stream and add, select, append; zero unless built with -DBM25MI_PHASES */
+    int64_t wave_busy_max;     /* -DBM25MI_PHASES builds: longest busy time of a scoring wave (100 MHz ticks), */
+    int64_t wave_span;         /* first wave start to last wave end, */
+    int64_t wave_start_spread; /* and first to last wave start */
 } bm25mi_stats;
 
 void bm25mi_config_default(bm25mi_config *cfg);
