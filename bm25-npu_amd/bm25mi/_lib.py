@@ -85,6 +85,7 @@ SIGNATURES = {
     'bm25mi_search_pruned': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
     'bm25mi_search_timed': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
     'bm25mi_query_features': (C.c_int, [P, P, P, i32, P, P]),
+    'bm25mi_query_bounds': (C.c_int, [P, P, P, i32, P, P, P]),
     'bm25mi_last_stats': (C.c_int, [P, P]),
     'bm25mi_destroy': (C.c_int, [P]),
     'bm25mi_last_error': (i64, [P, C.c_char_p, C.c_size_t]),
@@ -99,6 +100,10 @@ SIGNATURES = {
     'bm25mi_mlp_forward': (C.c_int, [P, P, i32, i32, P, P]),
     'bm25mi_mlp_backward': (C.c_int, [P, P, P, i32, i32, P, P, P]),
     'bm25mi_mlp_adam': (C.c_int, [P, P, P, P, i64, f32, f32, f32, f32, i32, P]),
+    'bm25mi_mlp_init': (C.c_int, [i32, C.c_uint64, P]),
+    'bm25mi_mlp_gradient': (C.c_int, [P, P, P, i32, i32, P, P]),
+    'bm25mi_mlp_predict': (C.c_int, [P, P, i32, i32, P]),
+    'bm25mi_mlp_train': (C.c_int, [P, P, P, i32, i32, i32, i32, f32, C.c_uint64, P]),
 }
 
 _lib = None

@@ -123,6 +123,16 @@ class Session:
                                                    _lib.ptr(ks), _lib.ptr(out)), self.handle)
         return out
 
+    def bounds(self, queries, k):
+        """(lower, upper) float32[Q]: the K-th largest term impact (a lower bound of each
+        query's K-th score) and the FP32 sum of term maxima (an upper bound of any score)."""
+        offsets, terms, nq, ks = self._prepare(queries, k)
+        lower = np.empty(nq, np.float32)
+        upper = np.empty(nq, np.float32)
+        _lib.check(self._lib.bm25mi_query_bounds(self.handle, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
+                                                 _lib.ptr(lower), _lib.ptr(upper)), self.handle)
+        return lower, upper
+
     def stats(self):
         st = _lib.Stats()
         _lib.check(self._lib.bm25mi_last_stats(self.handle, C.byref(st)), self.handle)

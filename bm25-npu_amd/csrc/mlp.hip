@@ -11,6 +11,9 @@
 //   adam      elementwise Adam update with bias correction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -308,5 +311,185 @@ extern "C" int bm25mi_mlp_adam(float *params, const float *grad, float *m, float
         hipLaunchKernelGGL(KMlpAdam, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), params, grad, m, v,
                            count, lr, beta1, beta2, eps, c1, c2);
         BM25MI_MLP_HIP(hipGetLastError(), "Adam launch");
+    });
+}
+
+// ---------------------------------------------------------------------------------------
+// Host-pointer entry points: device buffers are owned here, so callers need no GPU
+// allocator of their own.
+namespace {
+
+struct DeviceBuffer {
+    float *p = nullptr;
+    explicit DeviceBuffer(size_t n)
+    {
+        BM25MI_MLP_HIP(hipMalloc(&p, std::max<size_t>(1, n) * sizeof(float)), "MLP buffer alloc");
+    }
+    ~DeviceBuffer()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    void Upload(const float *src, size_t n, hipStream_t s)
+    {
+        BM25MI_MLP_HIP(hipMemcpyAsync(p, src, n * sizeof(float), hipMemcpyHostToDevice, s), "MLP upload");
+    }
+    void Download(float *dst, size_t n, hipStream_t s)
+    {
+        BM25MI_MLP_HIP(hipMemcpyAsync(dst, p, n * sizeof(float), hipMemcpyDeviceToHost, s), "MLP download");
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { BM25MI_MLP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "MLP stream"); }
+    ~Stream()
+    {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
+inline uint64_t SplitMix64(uint64_t &x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+inline double Uniform01(uint64_t &x) { return static_cast<double>(SplitMix64(x) >> 11) * (1.0 / 9007199254740992.0); }
+
+void CheckShapes(const float *x, const float *y, int64_t n, int32_t in)
+{
+    Require(x && y && n >= 1 && n <= (int64_t(1) << 30) && in >= 1 && in <= kMaxIn, "Invalid MLP data shape");
+}
+
+} // namespace
+
+// He-uniform initial weights (bound sqrt(6 / fan_in)), zero biases, from a seeded stream.
+extern "C" int bm25mi_mlp_init(int32_t in_features, uint64_t seed, float *params_out)
+{
+    return Invoke([&] {
+        Require(params_out && in_features >= 1 && in_features <= kMaxIn, "Invalid MLP init");
+        const int in = in_features;
+        float *W1 = params_out, *B1 = W1 + kH * in, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH, *B3 = W3 + kH;
+        uint64_t st = seed;
+        auto fill = [&](float *w, int count, int fanIn) {
+            const double bound = std::sqrt(6.0 / fanIn);
+            for (int i = 0; i < count; ++i)
+                w[i] = static_cast<float>((2.0 * Uniform01(st) - 1.0) * bound);
+        };
+        fill(W1, kH * in, in);
+        fill(W2, kH * kH, kH);
+        fill(W3, kH, kH);
+        std::fill(B1, B1 + kH, 0.0F);
+        std::fill(B2, B2 + kH, 0.0F);
+        B3[0] = 0.0F;
+    });
+}
+
+// Mean-squared-error gradient of params on (x, target), computed on the GPU, returned to host.
+extern "C" int bm25mi_mlp_gradient(const float *params, const float *x, const float *target, int32_t n,
+                                   int32_t in_features, float *grad_out, float *loss_out)
+{
+    return Invoke([&] {
+        CheckShapes(x, target, n, in_features);
+        Require(params && grad_out, "Null MLP gradient arguments");
+        const int64_t count = MlpParams(in_features);
+        Stream st;
+        DeviceBuffer dp(count), dx(static_cast<size_t>(n) * in_features), dt(n), dg(count), dl(1);
+        dp.Upload(params, count, st.s);
+        dx.Upload(x, static_cast<size_t>(n) * in_features, st.s);
+        dt.Upload(target, n, st.s);
+        BM25MI_MLP_HIP(hipMemsetAsync(dg.p, 0, count * sizeof(float), st.s), "MLP gradient clear");
+        const int rc = bm25mi_mlp_backward(dp.p, dx.p, dt.p, n, in_features, dg.p, dl.p, st.s);
+        if (rc != BM25MI_OK)
+            throw DeviceError(0, "MLP backward failed");
+        dg.Download(grad_out, count, st.s);
+        float loss = 0;
+        dl.Download(&loss, 1, st.s);
+        BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP gradient sync");
+        if (loss_out)
+            *loss_out = loss;
+    });
+}
+
+// y[n] = MLP(x[n, in]) on the GPU, host arrays in and out.
+extern "C" int bm25mi_mlp_predict(const float *params, const float *x, int32_t n, int32_t in_features, float *y_out)
+{
+    return Invoke([&] {
+        CheckShapes(x, y_out, n, in_features);
+        Require(params, "Null MLP parameters");
+        const int64_t count = MlpParams(in_features);
+        Stream st;
+        DeviceBuffer dp(count), dx(static_cast<size_t>(n) * in_features), dy(n);
+        dp.Upload(params, count, st.s);
+        dx.Upload(x, static_cast<size_t>(n) * in_features, st.s);
+        const int rc = bm25mi_mlp_forward(dp.p, dx.p, n, in_features, dy.p, st.s);
+        if (rc != BM25MI_OK)
+            throw DeviceError(0, "MLP forward failed");
+        dy.Download(y_out, n, st.s);
+        BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP predict sync");
+    });
+}
+
+// Minibatch Adam training on the GPU. params (in/out) start from the caller's values. The
+// sample order of every epoch is a seeded shuffle; epoch_loss[e] (optional) receives the
+// mean minibatch loss of epoch e.
+extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *target, int32_t n, int32_t in_features,
+                                int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss)
+{
+    return Invoke([&] {
+        CheckShapes(x, target, n, in_features);
+        Require(params && epochs >= 1 && epochs <= 100000 && batch >= 1 && batch <= n && std::isfinite(lr) && lr > 0,
+                "Invalid MLP training arguments");
+        const int in = in_features;
+        const int64_t count = MlpParams(in);
+        Stream st;
+        DeviceBuffer dp(count), dg(count), dm(count), dv(count), dx(static_cast<size_t>(n) * in), dt(n), dl(1);
+        dp.Upload(params, count, st.s);
+        BM25MI_MLP_HIP(hipMemsetAsync(dm.p, 0, count * sizeof(float), st.s), "Adam m clear");
+        BM25MI_MLP_HIP(hipMemsetAsync(dv.p, 0, count * sizeof(float), st.s), "Adam v clear");
+        std::vector<int32_t> order(n);
+        for (int32_t i = 0; i < n; ++i)
+            order[i] = i;
+        std::vector<float> xs(static_cast<size_t>(n) * in), ts(n);
+        uint64_t rng = seed;
+        int32_t step = 0;
+        for (int e = 0; e < epochs; ++e) {
+            for (int32_t i = n - 1; i > 0; --i) // Fisher-Yates
+                std::swap(order[i], order[static_cast<int32_t>(SplitMix64(rng) % static_cast<uint64_t>(i + 1))]);
+            BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP epoch sync"); // previous epoch done with xs
+            for (int32_t i = 0; i < n; ++i) {
+                std::memcpy(&xs[static_cast<size_t>(i) * in], &x[static_cast<size_t>(order[i]) * in], in * sizeof(float));
+                ts[i] = target[order[i]];
+            }
+            dx.Upload(xs.data(), xs.size(), st.s);
+            dt.Upload(ts.data(), ts.size(), st.s);
+            double sum = 0;
+            int batches = 0;
+            for (int32_t b0 = 0; b0 + batch <= n; b0 += batch) {
+                BM25MI_MLP_HIP(hipMemsetAsync(dg.p, 0, count * sizeof(float), st.s), "MLP grad clear");
+                if (bm25mi_mlp_backward(dp.p, dx.p + static_cast<size_t>(b0) * in, dt.p + b0, batch, in, dg.p, dl.p,
+                                        st.s) != BM25MI_OK)
+                    throw DeviceError(0, "MLP backward failed");
+                if (bm25mi_mlp_adam(dp.p, dg.p, dm.p, dv.p, count, lr, 0.9F, 0.999F, 1e-8F, ++step, st.s) != BM25MI_OK)
+                    throw DeviceError(0, "Adam step failed");
+                float loss = 0;
+                dl.Download(&loss, 1, st.s);
+                BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP step sync");
+                sum += loss;
+                ++batches;
+            }
+            if (epoch_loss)
+                epoch_loss[e] = static_cast<float>(sum / std::max(1, batches));
+        }
+        dp.Download(params, count, st.s);
+        BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP train sync");
     });
 }

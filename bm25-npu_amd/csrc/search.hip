@@ -184,7 +184,7 @@ __device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t
 #pragma unroll
     for (int i = 0; i < kHistPerLane; ++i) {
         const int b = static_cast<int>(__lane_id()) * kHistPerLane + i;
-        h[i] = b < kHistBins ? __hip_atomic_load(hist + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        h[i] = b < kHistBins ? __hip_atomic_load(hist + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0u;
     }
     return WaveHistBoundRegs(h, k);
 }
@@ -220,6 +220,135 @@ __device__ __forceinline__ uint32_t WaveKthLargest(uint32_t v, uint32_t k)
 }
 
 constexpr int Log2(int x) { return x <= 1 ? 0 : 1 + Log2(x / 2); }
+
+// The rare part of a task's selection, kept out of line so that its registers do not
+// constrain the streaming loop: cut the kept docs to BlockKeepCap(K) when there are more,
+// append them to the query's pool as composite keys, count them in the score histogram
+// and raise tau by the cut. kept/mine are the wave and lane counts of docs scoring at least
+// floor, mx the lane's largest score bits.
+struct SelectResult {
+    uint32_t kept, cut, overflow;
+};
+
+template <int B>
+__device__ __attribute__((noinline)) SelectResult SelectAppend(const float *acc, uint32_t *sHist,
+                                                               unsigned long long *pool, int64_t capq,
+                                                               uint32_t *count, uint32_t *histw, uint32_t *tau,
+                                                               int32_t base, uint32_t floor, uint32_t k, uint32_t kept,
+                                                               uint32_t mine, uint32_t mx)
+{
+    constexpr int kVec = B / (4 * kWave);
+    constexpr int kIdxBits = Log2(B);
+    const uint32_t lane = __lane_id();
+    const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
+    SelectResult res{0, 0, 0};
+    // Keep elements whose key (bits << kIdxBits | (B-1-e)) is at least (tb, te).
+    uint32_t tb = floor, te = 0, tauNew = 0;
+    auto countKey = [&](uint32_t xb, uint32_t xe) {
+        uint32_t c = 0;
+#pragma unroll 1
+        for (int r = 0; r < kVec; ++r) {
+            const uint4 v = bits4[r * kWave + lane];
+            const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
+            c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) +
+                 (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
+                 (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) +
+                 (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
+        }
+        return c;
+    };
+    const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
+    if (kept > cap) {
+        res.cut = 1;
+        if (k <= static_cast<uint32_t>(kWave)) {
+            const uint32_t f = max(WaveKthLargest(mx, k), floor);
+            mine = countKey(f, 0);
+            kept = WaveSum(mine);
+            tb = f;
+            if (kept <= cap)
+                tauNew = f; // at least k docs of the block score >= f
+        }
+        if (kept > cap) {
+            // Bisection over composite keys: count(>= lo) > cap, count(>= hi) < k.
+            unsigned long long lok = static_cast<unsigned long long>(tb) << kIdxBits;
+            unsigned long long hik = (static_cast<unsigned long long>(WaveMax(mx)) + 1) << kIdxBits;
+            for (int it = 0; it < 64 && hik - lok > 1; ++it) {
+                const unsigned long long mid = lok + (hik - lok) / 2;
+                const uint32_t mb = static_cast<uint32_t>(mid >> kIdxBits);
+                const uint32_t me = static_cast<uint32_t>(mid) & static_cast<uint32_t>(B - 1);
+                const uint32_t cm = WaveSum(countKey(mb, me));
+                if (cm > cap) {
+                    lok = mid;
+                } else if (cm < k) {
+                    hik = mid;
+                } else {
+                    lok = mid;
+                    break;
+                }
+            }
+            tb = static_cast<uint32_t>(lok >> kIdxBits);
+            te = static_cast<uint32_t>(lok) & static_cast<uint32_t>(B - 1);
+            mine = countKey(tb, te);
+            kept = WaveSum(mine);
+            if (kept >= k)
+                tauNew = tb;
+        }
+    }
+
+    // Append the kept docs to the pool and count them in the histogram.
+    uint32_t before = mine;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
+        if (lane >= static_cast<uint32_t>(o))
+            before += y;
+    }
+    before -= mine;
+    uint32_t slot0 = 0;
+    if (lane == 0)
+        slot0 = atomicAdd(count, kept);
+    slot0 = ReadLane(slot0, 0);
+#pragma unroll
+    for (int r = 0; r < kHistPerLane; ++r)
+        if (lane * kHistPerLane + r < kHistBins)
+            sHist[lane * kHistPerLane + r] = 0;
+    if (mine) {
+        uint32_t slot = slot0 + before;
+#pragma unroll 1
+        for (int r = 0; r < kVec; ++r) {
+            const uint4 v = bits4[r * kWave + lane];
+            const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
+            const uint32_t e0 = (r * kWave + lane) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t x = vals[c], rk = static_cast<uint32_t>(B - 1) - (e0 + c);
+                if (x > tb || (x == tb && rk >= te)) {
+                    if (static_cast<int64_t>(slot) < capq)
+                        pool[slot] = (static_cast<unsigned long long>(x) << 32) |
+                                     static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e0 + c));
+                    else
+                        ++res.overflow;
+                    ++slot;
+                    atomicAdd(sHist + HistBucket(x), 1u);
+                }
+            }
+        }
+    }
+    // One global atomic per non-empty bucket of this append.
+#pragma unroll
+    for (int r = 0; r < kHistPerLane; ++r) {
+        const uint32_t bin = lane * kHistPerLane + r;
+        if (bin < kHistBins) {
+            const uint32_t c = sHist[bin];
+            if (c)
+                atomicAdd(histw + bin, c);
+        }
+    }
+    if (lane == 0 && tauNew)
+        atomicMax(tau, tauNew);
+    res.kept = kept;
+    return res;
+}
 
 // The scoring kernel. Every wave is its own workgroup and walks (query, block) tasks of B
 // docs independently, without barriers, so the waves of a CU overlap one another's memory
@@ -290,7 +419,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // ended, and the next block's end is loaded one task ahead (lane j: word j).
     uint32_t hiPrev = 0, hiNext = 0;
     uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / hiNext were prepared for
-    for (uint32_t task = bt.waveStart[blockIdx.x]; task < tEnd; ++task) {
+    int histQ = -1;
+    uint32_t histBound = 1;
+    const uint32_t taskFirst = bt.waveStart[blockIdx.x];
+    for (uint32_t task = taskFirst; task < tEnd; ++task) {
         BM25MI_PHASE(0);
         const int q = static_cast<int>(task / static_cast<uint32_t>(blocks));
         const int blk = static_cast<int>(task - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks));
@@ -308,59 +440,144 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 info = bt.words[w0 + lane];
         }
 
-        // Lower-bound inputs: loaded now, used after the postings.
-        const uint32_t *histq = wk.hist + static_cast<int64_t>(q) * kHistBins;
-        uint32_t h[kHistPerLane];
-#pragma unroll
-        for (int r = 0; r < kHistPerLane; ++r) {
-            const uint32_t bin = lane * kHistPerLane + r;
-            h[r] = bin < kHistBins ? __hip_atomic_load(histq + bin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                   : 0u;
+        // Lower bound of the query's K-th score: the histogram bound is refreshed when the
+        // query changes and every 8th task (these bounds only grow, so an older value is
+        // still a lower bound).
+        if (q != histQ || ((task - taskFirst) & 7) == 0) {
+            histQ = q;
+            histBound = WaveHistBound(wk.hist + static_cast<int64_t>(q) * kHistBins, k);
         }
-        // Stale values are still lower bounds: these only ever grow during a Search.
         const uint32_t tauq =
-            max(__hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), tau0q);
+            max(max(__hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), tau0q), histBound);
 
         const int64_t g0 = static_cast<int64_t>(blk) * kGpb;
         const int gpb = static_cast<int>(min(static_cast<int64_t>(kGpb), ix.granules - g0));
-        int ga = 0, gb = gpb; // active granules [ga, gb) of the block
-        if (thr > 0.0F) {
-            float ub = 0.0F;
-            for (int j = 0; j < n; ++j) {
-                if (ReadLane(info.y, j) == 0)
-                    continue;
-                const int32_t sl = static_cast<int32_t>(ReadLane(info.z, j));
-                const float mx = __uint_as_float(ReadLane(info.w, j));
-                if (static_cast<int>(lane) < gpb)
-                    ub += sl >= 0 ? ix.bmax[static_cast<int64_t>(sl) * ix.granules + g0 + lane] : mx;
-            }
-            const unsigned long long active = __ballot(static_cast<int>(lane) < gpb && ub >= thr);
-            if (!active) {
-                ++skipped;
-                continue;
-            }
-            ga = __ffsll(static_cast<long long>(active)) - 1;
-            gb = 64 - __clzll(static_cast<long long>(active));
-        }
-        const int32_t a = base + ga * kGranule;
-        const int32_t b = min(base + gb * kGranule, end);
-
-        // 1. Posting range of each word in [a, b) (lane j: word j).
-        uint32_t lo = 0, len = 0;
         const int32_t sl = static_cast<int32_t>(info.z);
         const bool isLong = info.y > 0 && sl >= 0;
         const uint32_t *dirRow = ix.dir + static_cast<int64_t>(sl) * (ix.granules + 1);
+        float4 *acc4 = reinterpret_cast<float4 *>(acc);
+
+        // Window of short list j (a doc-ascending run of < kLongMin postings) covering docs
+        // [lo, hi): postings between the samples that bracket lo and hi. Executed by the whole
+        // wave; lane j receives it.
+        auto shortWindow = [&](int j, int32_t wlo, int32_t whi, uint32_t &oLo, uint32_t &oLen) {
+            const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
+            const uint32_t i0 = (first + 15u) >> kSampleShift, i1 = (first + cnt + 15u) >> kSampleShift;
+            uint32_t ca = 0, cb = 0;
+#pragma unroll
+            for (int r = 0; r < kSampleRounds; ++r) {
+                const uint32_t i = i0 + r * kWave + lane;
+                if (i < i1) {
+                    const int32_t smp = ix.samples[i];
+                    ca += smp < wlo ? 1u : 0u;
+                    cb += smp < whi ? 1u : 0u;
+                }
+            }
+            ca = WaveSum(ca);
+            cb = WaveSum(cb);
+            const uint32_t ws = ca > 0 ? ((i0 + ca - 1) << kSampleShift) + 1 : first;
+            const uint32_t we = cb < i1 - i0 ? (i0 + cb) << kSampleShift : first + cnt;
+            if (static_cast<int>(lane) == j) {
+                oLo = ws;
+                oLen = we > ws ? we - ws : 0u;
+            }
+        };
+
+        // Zeroes the accumulator row and adds the postings of segments [segLo, segLo+segLen)
+        // (lane s: segment s, s < nseg, in query-word order) for docs in [base, end), in
+        // rounds of U 64-posting columns. Returns the postings streamed.
+        uint32_t inRange = 0;
+        auto streamAdd = [&](uint32_t segLo, uint32_t segLen, int nseg) {
+            const uint32_t myCols = static_cast<int>(lane) < nseg ? (segLen + kWave - 1) / kWave : 0u;
+            uint32_t colEnd = myCols;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(colEnd), o));
+                if (lane >= static_cast<uint32_t>(o))
+                    colEnd += y;
+            }
+            const uint32_t columns = ReadLane(colEnd, kWave - 1);
+            const uint32_t colStart = colEnd - myCols;
+            // Lanes past the last segment hold ~0: column c belongs to segment
+            // popcount(ballot(colEnd <= c)).
+            if (static_cast<int>(lane) >= nseg)
+                colEnd = 0xFFFFFFFFu;
+            if (columns == 0) {
+#pragma unroll
+                for (int r = 0; r < kVec; ++r)
+                    acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+            }
+            for (uint32_t r0 = 0; r0 < columns; r0 += U) {
+                // Lanes past a column's end keep doc 0xFFFFFFFF, which the range test drops.
+                uint2 e[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    e[u] = make_uint2(0xFFFFFFFFu, 0u);
+                    const uint32_t c = r0 + u;
+                    if (c < columns) {
+                        const int sg = __popcll(__ballot(colEnd <= c));
+                        const uint32_t off = (c - ReadLane(colStart, sg)) << 6;
+                        if (lane + off < ReadLane(segLen, sg))
+                            e[u] = ix.post[ReadLane(segLo, sg) + off + lane];
+                    }
+                }
+                if (r0 == 0) {
+#pragma unroll
+                    for (int r = 0; r < kVec; ++r)
+                        acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+                }
+                // Read-add-write each column in order. When the whole round lies in one
+                // segment, its docs are distinct, so all reads may go first (no column can
+                // see another's write).
+                const uint32_t cLast = min(r0 + U, columns) - 1;
+                const uint32_t span = static_cast<uint32_t>(end - base);
+                if (__popcll(__ballot(colEnd <= r0)) == __popcll(__ballot(colEnd <= cLast))) {
+                    float v[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t rel = e[u].x - static_cast<uint32_t>(base);
+                        v[u] = rel < span ? acc[rel] : 0.0F;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t rel = e[u].x - static_cast<uint32_t>(base);
+                        if (rel < span) {
+                            acc[rel] = v[u] + __uint_as_float(e[u].y);
+                            ++inRange;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t rel = e[u].x - static_cast<uint32_t>(base);
+                        if (rel < span) {
+                            acc[rel] += __uint_as_float(e[u].y);
+                            ++inRange;
+                        }
+                    }
+                }
+            }
+            uint32_t streamed = 0;
+#pragma unroll 1
+            for (int sg = 0; sg < nseg; ++sg)
+                streamed += ReadLane(segLen, sg);
+            return streamed;
+        };
+
+        // 1. Posting range of each word in the block (lane j: word j). Long lists: the
+        // directory, or on the plain path the cursor (this block starts where the previous one
+        // ended; the next block's end is loaded one task ahead). Short lists: sample windows.
+        uint32_t lo = 0, len = 0;
         if (isLong) {
             if (cursorTask == task) {
                 lo = hiPrev;
                 len = hiNext - lo;
             } else {
-                lo = info.x + dirRow[g0 + ga];
-                len = info.x + dirRow[g0 + gb] - lo;
+                lo = info.x + dirRow[g0];
+                len = info.x + dirRow[g0 + gpb] - lo;
             }
         }
         hiPrev = lo + len;
-        // Next block's end, loaded now for the next task when it is this query's next block.
         cursorTask = 0xFFFFFFFFu;
         if (thr <= 0.0F && task + 1 < tEnd && blk + 1 < blocks) {
             cursorTask = task + 1;
@@ -371,116 +588,138 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         while (shortMask) {
             const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
             shortMask &= shortMask - 1;
-            const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
-            const uint32_t i0 = (first + 15u) >> kSampleShift, i1 = (first + cnt + 15u) >> kSampleShift;
-            uint32_t ca = 0, cb = 0;
-#pragma unroll
-            for (int r = 0; r < kSampleRounds; ++r) {
-                const uint32_t i = i0 + r * kWave + lane;
-                if (i < i1) {
-                    const int32_t s = ix.samples[i];
-                    ca += s < a ? 1u : 0u;
-                    cb += s < b ? 1u : 0u;
-                }
-            }
-            ca = WaveSum(ca);
-            cb = WaveSum(cb);
-            const uint32_t ws = ca > 0 ? ((i0 + ca - 1) << kSampleShift) + 1 : first;
-            const uint32_t we = cb < i1 - i0 ? (i0 + cb) << kSampleShift : first + cnt;
-            if (static_cast<int>(lane) == j) {
-                lo = ws;
-                len = we > ws ? we - ws : 0u;
-            }
+            shortWindow(j, base, end, lo, len);
         }
-        // Column prefix: lane j holds the end of word j's 64-posting columns; lanes past the
-        // last word hold ~0 so that column c belongs to word popcount(ballot(end <= c)).
-        const uint32_t myCols = (len + kWave - 1) / kWave;
-        uint32_t colEnd = myCols;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(colEnd), o));
-            if (lane >= static_cast<uint32_t>(o))
-                colEnd += y;
-        }
-        const uint32_t columns = ReadLane(colEnd, kWave - 1);
-        if (columns == 0) {
+        if (WaveSum(len) == 0) {
             ++skipped; // no query word has postings here: every score is 0
             continue;
         }
-        ++scored;
-        const uint32_t colStart = colEnd - myCols;
-        if (static_cast<int>(lane) >= n)
-            colEnd = 0xFFFFFFFFu;
 
-        // 2. Stream and add, word by word in query order.
-        BM25MI_PHASE(1);
-        float4 *acc4 = reinterpret_cast<float4 *>(acc);
-        const uint32_t span = static_cast<uint32_t>(b - a), shift = static_cast<uint32_t>(a - base);
-        uint32_t inRange = 0;
-        for (uint32_t r0 = 0; r0 < columns; r0 += U) {
-            // Lanes past a column's end keep doc 0xFFFFFFFF, which the range test drops.
-            uint2 e[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                e[u] = make_uint2(0xFFFFFFFFu, 0u);
-                const uint32_t c = r0 + u;
-                if (c < columns) {
-                    const int j = __popcll(__ballot(colEnd <= c));
-                    const uint32_t off = (c - ReadLane(colStart, j)) << 6;
-                    if (lane + off < ReadLane(len, j))
-                        e[u] = ix.post[ReadLane(lo, j) + off + lane];
-                }
-            }
-            if (r0 == 0) {
-#pragma unroll
-                for (int r = 0; r < kVec; ++r)
-                    acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
-            }
-            // Read-add-write each column in order. When the whole round lies in one word, its
-            // docs are distinct, so all reads may go first (no column can see another's write).
-            const uint32_t cLast = min(r0 + U, columns) - 1;
-            if (__popcll(__ballot(colEnd <= r0)) == __popcll(__ballot(colEnd <= cLast))) {
-                float v[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t rel = e[u].x - static_cast<uint32_t>(a);
-                    v[u] = rel < span ? acc[rel + shift] : 0.0F;
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t rel = e[u].x - static_cast<uint32_t>(a);
-                    if (rel < span) {
-                        acc[rel + shift] = v[u] + __uint_as_float(e[u].y);
-                        ++inRange;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t rel = e[u].x - static_cast<uint32_t>(a);
-                    if (rel < span) {
-                        acc[rel + shift] += __uint_as_float(e[u].y);
-                        ++inRange;
-                    }
-                }
+        // Threshold path (MaxScore on granules). The host marked each word essential (bit 31
+        // of its max-impact word) unless the sum of the non-essential words' maxima stays
+        // below the threshold: a doc with no essential posting cannot reach it. Pass 0 adds
+        // only the essential words' impacts; a doc's score is at most that plus the sum of
+        // the non-essential words' maxima in its granule (FP32 addition is monotone; the
+        // threshold is lowered by 2^-16 relative to cover any summation order), so granules
+        // where no doc reaches the threshold are dead. Pass 1 scores the live granules exactly.
+        // The plain path makes the exact pass directly.
+        const bool ess = info.y > 0 && (info.w >> 31);
+        // With no non-essential word the bound pass would compute the exact scores: score
+        // directly (docs below the threshold are dropped by the selection floor).
+        const bool pruned = thr > 0.0F && __ballot(static_cast<int>(lane) < n && info.y > 0 && !ess) != 0;
+        uint32_t segLo = lo, segLen = pruned && !ess ? 0u : len;
+        int nseg = n;
+        float cg = 0.0F; // lane g < gpb: FP32 sum of the non-essential words' maxima in granule g
+        if (pruned) {
+            for (int j = 0; j < n; ++j) {
+                const uint32_t wj = ReadLane(info.w, j);
+                if (ReadLane(info.y, j) == 0 || (wj >> 31))
+                    continue;
+                const int32_t slj = static_cast<int32_t>(ReadLane(info.z, j));
+                if (static_cast<int>(lane) < gpb)
+                    cg += slj >= 0 ? ix.bmax[static_cast<int64_t>(slj) * ix.granules + g0 + lane]
+                                   : __uint_as_float(wj & 0x7FFFFFFFu);
             }
         }
-        uint32_t loaded = 0;
-#pragma unroll 1
-        for (int w = 0; w < n; ++w)
-            loaded += ReadLane(len, w);
-        scanned += loaded;
+        BM25MI_PHASE(1);
+        bool dead = false;
+        for (int pass = pruned ? 0 : 1; pass < 2; ++pass) {
+            // 2. Stream and add, word by word in query order.
+            scanned += streamAdd(segLo, segLen, nseg);
+            if (pass == 1)
+                break;
+            const float thrLive = thr * 0.99998474F; // thr * (1 - 2^-16)
+            unsigned long long live = 0;
+#pragma unroll
+            for (int r = 0; r < kVec; ++r) {
+                // Row r of the accumulators (docs r*256 ...) is granule r of the block.
+                const float4 v = acc4[r * kWave + lane];
+                const float m = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
+                const float c = __uint_as_float(ReadLane(__float_as_uint(cg), r));
+                if (__ballot(r < gpb && m + c >= thrLive))
+                    live |= 1ULL << r;
+            }
+            if (!live) {
+                dead = true;
+                break;
+            }
+            // Pass 1 segments: each long word's runs of live granules and each short word's
+            // window, in query order (lane s: segment s); more than 64 fall back to one span
+            // per word over [first live, last live] granules.
+            const int ga = __ffsll(static_cast<long long>(live)) - 1;
+            const int gz = 64 - __clzll(static_cast<long long>(live));
+            uint32_t sLo = 0, sLen = 0;
+            int ns = 0;
+            constexpr int kDirPerWord = kGpb + 1, kWordsPerLoad = kWave / kDirPerWord;
+            for (int j0 = 0; j0 < n && ns <= kWave; j0 += kWordsPerLoad) {
+                // Directory entries of up to kWordsPerLoad words in one load: lane
+                // (j - j0) * kDirPerWord + g holds word j's entry for granule g0 + g.
+                const int jl = min(j0 + static_cast<int>(lane) / kDirPerWord, kWave - 1);
+                const int gl = static_cast<int>(lane) % kDirPerWord;
+                const uint32_t ly = static_cast<uint32_t>(__shfl(static_cast<int>(info.y), jl));
+                const int32_t lsl = __shfl(static_cast<int>(info.z), jl);
+                uint32_t dv = 0;
+                if (static_cast<int>(lane) < kWordsPerLoad * kDirPerWord && jl < n && ly > 0 && lsl >= 0 && gl <= gpb)
+                    dv = ix.dir[static_cast<int64_t>(lsl) * (ix.granules + 1) + g0 + gl];
+                for (int j = j0; j < min(n, j0 + kWordsPerLoad); ++j) {
+                    if (ReadLane(info.y, j) == 0)
+                        continue;
+                    if (static_cast<int32_t>(ReadLane(info.z, j)) < 0) {
+                        const uint32_t wl = ReadLane(lo, j), wn = ReadLane(len, j);
+                        if (wn && ns < kWave && static_cast<int>(lane) == ns) {
+                            sLo = wl;
+                            sLen = wn;
+                        }
+                        ns += wn ? 1 : 0;
+                        continue;
+                    }
+                    const uint32_t first = ReadLane(info.x, j);
+                    const int at = (j - j0) * kDirPerWord;
+                    unsigned long long runs = live & ~(live << 1);
+                    while (runs) {
+                        const int r0 = __ffsll(static_cast<long long>(runs)) - 1;
+                        runs &= runs - 1;
+                        const unsigned long long after = ~live & (~0ULL << r0);
+                        const int r1 = after ? min(__ffsll(static_cast<long long>(after)) - 1, gpb) : gpb;
+                        const uint32_t s0 = first + ReadLane(dv, at + r0), s1 = first + ReadLane(dv, at + r1);
+                        if (s1 > s0) {
+                            if (ns < kWave && static_cast<int>(lane) == ns) {
+                                sLo = s0;
+                                sLen = s1 - s0;
+                            }
+                            ++ns;
+                        }
+                    }
+                }
+            }
+            if (ns > kWave) {
+                sLo = lo;
+                sLen = len;
+                if (isLong) {
+                    sLo = info.x + dirRow[g0 + ga];
+                    sLen = info.x + dirRow[g0 + gz] - sLo;
+                }
+                ns = n;
+            }
+            segLo = sLo;
+            segLen = sLen;
+            nseg = ns;
+        }
+        if (dead) {
+            ++skipped; // no doc of the block can reach the threshold
+            continue;
+        }
+        ++scored;
         useful += inRange;
 
         // 3. Selection.
         BM25MI_PHASE(2);
-        uint32_t floor = max(tauq, WaveHistBoundRegs(h, k));
-        floor = max(floor, 1u);
+        uint32_t floor = max(tauq, 1u);
         if (thr > 0.0F)
             floor = max(floor, __float_as_uint(thr));
         const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
         uint32_t mine = 0, mx = 0;
-#pragma unroll
+#pragma unroll 2
         for (int r = 0; r < kVec; ++r) {
             const uint4 v = bits4[r * kWave + lane];
             mine += (v.x >= floor ? 1u : 0u) + (v.y >= floor ? 1u : 0u) + (v.z >= floor ? 1u : 0u) +
@@ -490,115 +729,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         uint32_t kept = WaveSum(mine);
         if (kept == 0)
             continue;
-        // Keep elements whose key (bits << kIdxBits | (B-1-e)) is at least (tb, te).
-        uint32_t tb = floor, te = 0, tauNew = 0;
-        auto countKey = [&](uint32_t xb, uint32_t xe) {
-            uint32_t c = 0;
-#pragma unroll
-            for (int r = 0; r < kVec; ++r) {
-                const uint4 v = bits4[r * kWave + lane];
-                const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
-                c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) +
-                     (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
-                     (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) +
-                     (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
-            }
-            return c;
-        };
-        const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
-        if (kept > cap) {
-            ++cuts;
-            if (k <= static_cast<uint32_t>(kWave)) {
-                const uint32_t f = max(WaveKthLargest(mx, k), floor);
-                mine = countKey(f, 0);
-                kept = WaveSum(mine);
-                tb = f;
-                if (kept <= cap)
-                    tauNew = f; // at least k docs of the block score >= f
-            }
-            if (kept > cap) {
-                // Bisection over composite keys: count(>= lo) > cap, count(>= hi) < k.
-                unsigned long long lok = static_cast<unsigned long long>(tb) << kIdxBits;
-                unsigned long long hik = (static_cast<unsigned long long>(WaveMax(mx)) + 1) << kIdxBits;
-                for (int it = 0; it < 64 && hik - lok > 1; ++it) {
-                    const unsigned long long mid = lok + (hik - lok) / 2;
-                    const uint32_t mb = static_cast<uint32_t>(mid >> kIdxBits);
-                    const uint32_t me = static_cast<uint32_t>(mid) & static_cast<uint32_t>(B - 1);
-                    const uint32_t cm = WaveSum(countKey(mb, me));
-                    if (cm > cap) {
-                        lok = mid;
-                    } else if (cm < k) {
-                        hik = mid;
-                    } else {
-                        lok = mid;
-                        break;
-                    }
-                }
-                tb = static_cast<uint32_t>(lok >> kIdxBits);
-                te = static_cast<uint32_t>(lok) & static_cast<uint32_t>(B - 1);
-                mine = countKey(tb, te);
-                kept = WaveSum(mine);
-                if (kept >= k)
-                    tauNew = tb;
-            }
-        }
-
-        // Append the kept docs to the pool and count them in the histogram.
         BM25MI_PHASE(3);
-        uint32_t before = mine;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
-            if (lane >= static_cast<uint32_t>(o))
-                before += y;
-        }
-        before -= mine;
-        uint32_t slot0 = 0;
-        if (lane == 0)
-            slot0 = atomicAdd(&wk.count[q], kept);
-        slot0 = ReadLane(slot0, 0);
-        unsigned long long *pool = wk.pool + bt.poolOff[q];
-        const int64_t capq = bt.poolOff[q + 1] - bt.poolOff[q];
-        uint32_t *histw = wk.hist + static_cast<int64_t>(q) * kHistBins;
-#pragma unroll
-        for (int r = 0; r < kHistPerLane; ++r)
-            if (lane * kHistPerLane + r < kHistBins)
-                sHist[lane * kHistPerLane + r] = 0;
-        if (mine) {
-            uint32_t slot = slot0 + before;
-#pragma unroll 1
-            for (int r = 0; r < kVec; ++r) {
-                const uint4 v = bits4[r * kWave + lane];
-                const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
-                const uint32_t e0 = (r * kWave + lane) * 4;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const uint32_t x = vals[c], rk = static_cast<uint32_t>(B - 1) - (e0 + c);
-                    if (x > tb || (x == tb && rk >= te)) {
-                        if (static_cast<int64_t>(slot) < capq)
-                            pool[slot] = (static_cast<unsigned long long>(x) << 32) |
-                                         static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e0 + c));
-                        else
-                            ++overflow;
-                        ++slot;
-                        atomicAdd(sHist + HistBucket(x), 1u);
-                    }
-                }
-            }
-        }
-        // One global atomic per non-empty bucket of this append.
-#pragma unroll
-        for (int r = 0; r < kHistPerLane; ++r) {
-            const uint32_t bin = lane * kHistPerLane + r;
-            if (bin < kHistBins) {
-                const uint32_t c = sHist[bin];
-                if (c)
-                    atomicAdd(histw + bin, c);
-            }
-        }
-        if (lane == 0 && tauNew)
-            atomicMax(&wk.tau[q], tauNew);
-        emitted += kept;
+        const SelectResult sel =
+            SelectAppend<B>(acc, sHist, wk.pool + bt.poolOff[q], bt.poolOff[q + 1] - bt.poolOff[q], wk.count + q,
+                            wk.hist + static_cast<int64_t>(q) * kHistBins, wk.tau + q, base, floor, k, kept, mine, mx);
+        cuts += sel.cut;
+        overflow += sel.overflow;
+        emitted += sel.kept;
         ++emitting;
     }
     BM25MI_PHASE(0);

@@ -65,6 +65,7 @@ public:
     std::vector<int32_t> slotHost;
     std::vector<float> bmaxHost;
     std::vector<int64_t> scratchCost;
+    std::vector<std::pair<float, int64_t>> scratchOrder;
     std::vector<int64_t> topOffHost;
     std::vector<float> topValHost;
     int64_t granules = 0;
@@ -418,6 +419,29 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
                 info[w] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
             }
         }
+    // Threshold queries: mark the essential words (bit 31 of the max-impact word). Words are
+    // non-essential, smallest maximum first, while the sum of their maxima stays below the
+    // threshold with a 2^-16 relative margin (any FP32 summation order of up to 64 terms
+    // stays within it): a doc with no essential posting then scores below the threshold.
+    for (int32_t q = 0; q < nq; ++q) {
+        if (!(thr[q] > 0.0F))
+            continue;
+        std::vector<std::pair<float, int64_t>> &order = scratchOrder;
+        order.clear();
+        for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w)
+            if (info[w].y > 0)
+                order.emplace_back(tmaxHost[query.terms[w]], w);
+        std::sort(order.begin(), order.end());
+        double sum = 0;
+        size_t cut = 0;
+        while (cut < order.size() && (sum + order[cut].first) * (1.0 + 1.0 / 65536) < static_cast<double>(thr[q]) *
+                                                                                       (1.0 - 1.0 / 65536)) {
+            sum += order[cut].first;
+            ++cut;
+        }
+        for (size_t i = cut; i < order.size(); ++i)
+            info[order[i].second].w |= 0x80000000u;
+    }
     PlanWaves(perQuery.data(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
 
     try {
@@ -663,6 +687,15 @@ extern "C" int bm25mi_search_timed(bm25mi_session *s, const int64_t *query_offse
     return SearchCommon(s, q, out_scores, out_ids, fallback_out, thresholds != nullptr, true);
 }
 
+// The K-th largest term impact among the query's terms (a lower bound of the K-th score).
+static double LowerBound(const Session &ss, const int32_t *terms, int64_t n, int32_t k)
+{
+    uint32_t bits = ss.StartBound(terms, n, k);
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
 // Per-query features of the learned K-th-score model. Every feature depends only on
 // index statistics of the query's terms (IDs >= vocab ignored, repeats counted).
 extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
@@ -678,6 +711,7 @@ extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_off
             const Session &ss = *s->impl;
             const double n = static_cast<double>(ss.ndocs);
             for (int32_t q = 0; q < queries; ++q) {
+                Require(k_per_query[q] >= 1, "K out of range");
                 std::vector<double> m;
                 double sumMean = 0, sumIdf = 0, maxIdf = 0, sumDf = 0, maxDf = 0, minDf = -1, density = 0,
                        logAll = 0;
@@ -712,6 +746,40 @@ extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_off
                     std::log(n), std::max(-40.0, logAll)};
                 for (int i = 0; i < BM25MI_FEATURES; ++i)
                     f[i] = static_cast<float>(vals[i]);
+            }
+        },
+        &s->error);
+}
+
+extern "C" int bm25mi_query_bounds(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                   int32_t queries, const int32_t *k_per_query, float *out_lower, float *out_upper)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            Require(s->impl && !s->impl->closed, "Session closed");
+            Require(query_offsets && k_per_query && queries >= 1 && (out_lower || out_upper), "Null bound arguments");
+            const Session &ss = *s->impl;
+            for (int32_t q = 0; q < queries; ++q) {
+                const int64_t n = query_offsets[q + 1] - query_offsets[q];
+                Require(n >= 0 && (n == 0 || query_terms), "Invalid query offsets");
+                const int32_t *t = query_terms + query_offsets[q];
+                for (int64_t w = 0; w < n; ++w)
+                    Require(t[w] >= 0, "Invalid public word ID");
+                Require(k_per_query[q] >= 1, "K out of range");
+                if (out_lower)
+                    out_lower[q] = static_cast<float>(LowerBound(ss, t, n, k_per_query[q]));
+                if (out_upper) {
+                    // Sequential FP32 sum of the term maxima in query order: FP32 addition is
+                    // monotone, so no doc's score exceeds it.
+                    float ub = 0.0F;
+                    for (int64_t w = 0; w < n; ++w)
+                        if (t[w] < ss.vocab)
+                            ub += ss.tmaxHost[t[w]];
+                    out_upper[q] = ub;
+                }
             }
         },
         &s->error);

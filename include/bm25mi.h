@@ -140,6 +140,13 @@ int bm25mi_search_timed(bm25mi_session *s, const int64_t *query_offsets, const i
 int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                           int32_t queries, const int32_t *k_per_query, float *out_features);
 
+/* Per query (no reference counterpart): out_lower[q] = the K-th largest impact of any of its
+ * terms, a lower bound of its K-th score (every doc scores at least each of its impacts);
+ * out_upper[q] = the FP32 sum of its terms' largest impacts in query order, an upper bound of
+ * every doc's score. Either output may be NULL. */
+int bm25mi_query_bounds(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                        int32_t queries, const int32_t *k_per_query, float *out_lower, float *out_upper);
+
 int bm25mi_last_stats(const bm25mi_session *s, bm25mi_stats *out);
 
 /* Close: idempotent; frees device memory. The handle is freed on success. */
@@ -211,6 +218,19 @@ int bm25mi_mlp_backward(const float *params, const float *x, const float *target
 /* Adam step over count parameters; state m, v are device arrays of count floats. */
 int bm25mi_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t count, float lr, float beta1,
                     float beta2, float eps, int32_t step, void *stream);
+
+/* Host-pointer forms (the library owns the device buffers). */
+/* He-uniform weights, zero biases, from a seeded stream; params_out float[param_count]. */
+int bm25mi_mlp_init(int32_t in_features, uint64_t seed, float *params_out);
+/* One GPU backward pass: grad_out = d/dparams (1/n) sum (y - target)^2, loss_out = that mean. */
+int bm25mi_mlp_gradient(const float *params, const float *x, const float *target, int32_t n, int32_t in_features,
+                        float *grad_out, float *loss_out);
+/* y_out[n] = MLP(x[n, in]) on the GPU. */
+int bm25mi_mlp_predict(const float *params, const float *x, int32_t n, int32_t in_features, float *y_out);
+/* Minibatch Adam (beta 0.9/0.999, eps 1e-8) on the GPU for `epochs` seeded shuffles of the data;
+ * params in/out; epoch_loss float[epochs] (optional) gets each epoch's mean minibatch loss. */
+int bm25mi_mlp_train(float *params, const float *x, const float *target, int32_t n, int32_t in_features,
+                     int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,112 @@
+"""The learned K-th-score model: the float64 CPU oracle pinned by finite differences, the
+GPU kernels (forward, gradient, Adam training) checked against it, and the learned pruned
+Search checked against the plain path (identical results, fewer postings streamed)."""
+
+import numpy as np
+import pytest
+
+import bm25mi
+from bm25mi import learned
+from oracle import mlp_oracle as mo
+
+
+def random_case(seed, n=300, n_in=16):
+    rng = np.random.default_rng(seed)
+    params = (rng.standard_normal(learned.param_count(n_in)) * 0.2).astype(np.float32)
+    x = rng.standard_normal((n, n_in)).astype(np.float32)
+    target = rng.standard_normal(n).astype(np.float32)
+    return params, x, target
+
+
+def test_oracle_gradient_matches_finite_differences():
+    params, x, target = random_case(1, n=40)
+    grad, _ = mo.gradient(params, x, target)
+    p = params.astype(np.float64)
+    rng = np.random.default_rng(2)
+    for i in rng.choice(len(p), size=60, replace=False):
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (mo.loss(p + e, x, target) - mo.loss(p - e, x, target)) / 2e-6
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-5 * abs(fd), i
+    # A planted mistake (a dropped ReLU mask on layer 2) is caught by the same check.
+    W1, b1, W2, b2, W3, b3 = mo.unpack(p, x.shape[1])
+    z1 = x @ W1.T + b1
+    h1 = np.maximum(z1, 0)
+    dy = 2.0 * ((np.maximum(h1 @ W2.T + b2, 0) @ W3 + b3) - target) / len(target)
+    wrong_gb2 = np.outer(dy, W3).sum(axis=0)
+    b2 = 64 * 16 + 64 + 64 * 64  # offset of b2 in the flat layout
+    assert not np.allclose(wrong_gb2, grad[b2:b2 + 64])
+
+
+def test_param_layout_and_init_are_seeded():
+    n = learned.param_count()
+    assert n == 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1
+    a, b, c = learned.init_params(3), learned.init_params(3), learned.init_params(4)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    W1 = a[:64 * 16]
+    assert np.abs(W1).max() <= np.sqrt(6 / 16) + 1e-6 and np.all(a[64 * 16:64 * 17] == 0)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_and_gradient_match_oracle(need_gpu):
+    for seed in (1, 2):
+        params, x, target = random_case(seed)
+        y = learned.forward(params, x)
+        assert np.allclose(y, mo.forward(params, x), rtol=1e-4, atol=1e-5)
+        g, loss = learned.gradient(params, x, target)
+        want, want_loss = mo.gradient(params, x, target)
+        # FP32 sums of ~300 products against float64: relative error per entry well below 1e-3.
+        assert np.allclose(g, want, rtol=1e-3, atol=1e-5 * np.abs(want).max())
+        assert abs(loss - want_loss) <= 1e-4 * want_loss
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_a_smooth_function(need_gpu):
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((4096, 16)).astype(np.float32)
+    target = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2]).astype(np.float32)
+    params, losses = learned.train(learned.init_params(0), x, target, epochs=40, batch=256, lr=3e-3, seed=1)
+    assert losses[-1] < 0.1 * losses[0]
+    assert mo.loss(params, x, target) < 0.05
+    # Same seed, same result: the GPU reduction order is fixed.
+    params2, _ = learned.train(learned.init_params(0), x, target, epochs=40, batch=256, lr=3e-3, seed=1)
+    assert np.array_equal(params, params2)
+
+
+@pytest.fixture(scope='module')
+def small_corpus():
+    corpus = bm25mi.generate(bm25mi.CorpusConfig(300_000, 200_000, 1.07, 56.0, seed=11))
+    return corpus, corpus.index()
+
+
+@pytest.mark.gpu
+def test_learned_threshold_prunes_and_keeps_results(small_corpus, need_gpu):
+    corpus, index = small_corpus
+    with bm25mi.Session(index, max_queries=64, max_words=64 * 12, max_k=100) as s:
+        qs, ks = learned.make_queries(corpus.offsets, 1536, seed=100)
+        data = learned.collect(s, qs, ks, batch=64)
+        model = learned.ThresholdModel(seed=0).fit(data, epochs=60, batch=128, lr=2e-3, seed=0)
+        assert model.losses[-1] < model.losses[0]
+        vq, vk = learned.make_queries(corpus.offsets, 512, seed=200)
+        model.calibrate(learned.collect(s, vq, vk, batch=64), fallback_rate=0.02)
+        scanned = {'plain': 0, 'learned': 0, 'lower': 0}
+        fallbacks = 0
+        for k in (10, 100):
+            for seed in range(3):
+                queries = learned.make_queries(corpus.offsets, 64, seed=300 + seed, k_choices=(k,))[0]
+                feats = s.features(queries, k)
+                lower, upper = s.bounds(queries, k)
+                thr = model.thresholds(feats, lower, upper)
+                mism, fb, plain, pruned = learned.compare(s, queries, k, thr)
+                assert mism == 0
+                fallbacks += int(fb.sum())
+                scanned['plain'] += plain['postings_scanned']
+                scanned['learned'] += pruned['postings_scanned']
+                # The method without the network: the provable lower bound alone as threshold.
+                mism, fb0, _, base = learned.compare(s, queries, k, lower.astype(np.float32))
+                assert mism == 0 and not fb0.any()
+                scanned['lower'] += base['postings_scanned']
+        assert fallbacks <= 0.1 * 6 * 64
+        # The trained threshold streams fewer postings than the plain path and than the
+        # provable lower bound used as the threshold without the network.
+        assert scanned['learned'] < min(scanned['lower'], scanned['plain']), scanned
