@@ -224,6 +224,7 @@ class AscendIndexBm25:
         self._mutex = threading.Lock()
         self._session = None
         self._k = 0
+        self._queries = 0
         self._error = ErrorInfo()
 
     def _invoke(self, op):
@@ -252,6 +253,7 @@ class AscendIndexBm25:
                                         max_k=config.topK, block_docs=config.blockDocs, k1=config.k1,
                                         device=config.deviceId)
                 self._k = config.topK
+                self._queries = config.queryCount
             return self._invoke(op)
 
     def Search(self, queries, result):
@@ -261,6 +263,8 @@ class AscendIndexBm25:
                     raise _lib.Bm25Error(APP_ERR_ILLEGAL_OPERATION, 'BM25 is not initialized')
                 if not (isinstance(queries, list) and all(isinstance(r, list) for r in queries)):
                     raise ValueError('Queries must be a list of lists of word IDs')
+                if len(queries) != self._queries:
+                    raise ValueError('Fixed session requires its declared query count')
                 for r in queries:
                     for w in r:
                         if not _is_int(w, 0, LIMIT - 1):

@@ -57,6 +57,16 @@ def build(jobs=8, lib_dir=None):
     if r.returncode:
         raise RuntimeError(f'compile copy_calibration failed:\n{r.stdout}\n{r.stderr}')
     os.replace(str(exe) + '.tmp', exe)
+    # Check program of the C++ facade (include/bm25mi_index.hpp), run by tests/test_facade.py.
+    check_src = HERE.parent / 'tests' / 'cpp' / 'facade_check.cpp'
+    if check_src.exists():
+        exe = LIB / 'facade_check'
+        cmd = ['g++', '-std=c++17', '-O2', '-Wall', '-I', str(HERE.parent / 'include'), str(check_src), '-o',
+               str(exe) + '.tmp', '-L', str(LIB), '-lbm25mi', '-Wl,-rpath,$ORIGIN']
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(f'compile facade_check failed:\n{r.stdout}\n{r.stderr}')
+        os.replace(str(exe) + '.tmp', exe)
     return target
 
 
