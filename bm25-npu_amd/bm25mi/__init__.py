@@ -10,9 +10,10 @@ from .corpus import CONFIGS, Corpus, CorpusConfig, draw_queries, generate, load_
     save_index, write_queries
 from .index import Index
 from .search import AscendIndexBm25, Bm25Config, Bm25SearchResult, SearchSession, Session
+from .serving import Server
 
 __all__ = [
     'AscendIndexBm25', 'Bm25Config', 'Bm25Error', 'Bm25SearchResult', 'CONFIGS', 'Corpus', 'CorpusConfig',
-    'Index', 'SearchSession', 'Session', 'draw_queries', 'generate', 'load_index', 'pack', 'read_queries',
+    'Index', 'SearchSession', 'Server', 'Session', 'draw_queries', 'generate', 'load_index', 'pack', 'read_queries',
     'save_index', 'write_queries', '_lib',
 ]

@@ -28,6 +28,8 @@ class Config(C.Structure):
         ('use_graph', C.c_int32),
         ('grid_blocks', C.c_int32),
         ('task_cost', C.c_int32),
+        ('slots', C.c_int32),
+        ('pool_entries', C.c_int64),
     ]
 
 
@@ -86,6 +88,12 @@ SIGNATURES = {
     'bm25mi_search_timed': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
     'bm25mi_query_features': (C.c_int, [P, P, P, i32, P, P]),
     'bm25mi_query_bounds': (C.c_int, [P, P, P, i32, P, P, P]),
+    'bm25mi_server_start': (C.c_int, [P, i32, P]),
+    'bm25mi_server_submit': (C.c_int, [P, P, i32, i32, P]),
+    'bm25mi_server_wait': (C.c_int, [P, C.c_uint64, P, P, P]),
+    'bm25mi_server_run_poisson': (C.c_int, [P, P, P, i32, P, f64, C.c_uint64, P, P, P, P]),
+    'bm25mi_server_counters': (C.c_int, [P, P, P, P]),
+    'bm25mi_server_stop': (C.c_int, [P]),
     'bm25mi_last_stats': (C.c_int, [P, P]),
     'bm25mi_destroy': (C.c_int, [P]),
     'bm25mi_last_error': (i64, [P, C.c_char_p, C.c_size_t]),

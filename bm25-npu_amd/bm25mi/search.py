@@ -30,7 +30,7 @@ class Session:
     """A resident index on one GPU. All searches go through libbm25mi."""
 
     def __init__(self, index, max_queries=8, max_words=24, max_k=10, block_docs=0, k1=None, device=0,
-                 use_graph=True, grid_blocks=0, task_cost=0):
+                 use_graph=True, grid_blocks=0, task_cost=0, slots=1, pool_entries=0):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.bm25mi_config_default(C.byref(cfg))
@@ -43,6 +43,8 @@ class Session:
         cfg.use_graph = 1 if use_graph else 0
         cfg.grid_blocks = grid_blocks
         cfg.task_cost = task_cost
+        cfg.slots = slots
+        cfg.pool_entries = pool_entries
         self.config = cfg
         self.num_docs, self.vocab = index.num_docs, index.vocab
         handle = C.c_void_p()

@@ -21,7 +21,7 @@ COMMON = ['-O3', '-fPIC', '-std=c++17', '-ffp-contract=off', '-fno-fast-math', '
           '-Wno-unused-command-line-argument']
 DEVICE = ['--offload-arch=gfx950', '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 EXTRA = os.environ.get('BM25MI_CFLAGS', '').split()
-SOURCES = ['search.hip', 'session.hip', 'mlp.hip', 'corpus.cpp', 'index.cpp']
+SOURCES = ['search.hip', 'session.hip', 'server.hip', 'mlp.hip', 'corpus.cpp', 'index.cpp']
 
 
 def compile_one(src):

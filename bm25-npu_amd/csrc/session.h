@@ -1,0 +1,150 @@
+// The resident-index session behind the C ABI, shared by the Search entry points
+// (session.hip) and the serving loop (server.hip).
+//
+// Mirrors bm25large::Session (FR/ascendhost/src/bm25/large_search.cpp:107-212, FR =
+// feature_retrieval/src/ascendfaiss): the index is validated and uploaded once; every
+// per-Search buffer is allocated up front; H2D packet copy -> kernels is captured into a
+// hipGraph; an empty bootstrap Search must return IDs 0..K-1 with score 0 (:184-187); a
+// request is validated whole before the reusable packet is touched
+// (FR/common/bm25/fixed_query_packet.h:26-49); results are checked the way Session::Receive
+// does (:64-85); a device error marks the session failed for good (:208-210).
+//
+// Differences by design: impacts are precomputed at create time; the directory covers only
+// long posting lists; results land in mapped pinned host memory written by the last kernel
+// (no D2H copy node); Q and K may vary per call up to the configured capacity, K per query;
+// and the per-Search state is replicated in `slots`, each with its own stream and graph, so
+// that the serving loop can keep several batches in flight.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace bm25mi {
+
+#define BM25MI_HIP(expr, what)                                                                                         \
+    do {                                                                                                               \
+        const hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+            throw DeviceError(static_cast<int64_t>(e_), std::string(what) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+struct Query {
+    const int64_t *offsets;
+    const int32_t *terms;
+    int32_t count;
+    const int32_t *k;   // per query
+    const float *thr;   // per query or nullptr
+};
+
+// One in-flight Search: request packet, candidate pools, per-query bound state, per-wave
+// counters, result block, stream and captured graph.
+struct Slot {
+    void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
+    void *dHist = nullptr;
+    void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
+    BatchView bt{};
+    WorkView wk{};
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // The launched batch: query count, output row offsets, plain-path postings, timing mode.
+    int32_t nq = 0;
+    std::vector<int64_t> out;
+    int64_t total = 0;
+    bool timed = false;
+    bm25mi_stats last{};
+};
+
+class Session {
+public:
+    bm25mi_config cfg{};
+    int64_t ndocs = 0, vocab = 0, postings = 0;
+    int32_t blockDocs = 2048, granule = kGranule, blocks = 0, grid = 0, cus = 0;
+    int64_t poolEntries = 0; // candidate-pool entries of each slot
+    int64_t taskCost = kTaskCost;
+    std::vector<int64_t> toffHost;
+    std::vector<float> idfHost, tmaxHost, tmeanHost;
+    std::vector<int32_t> slotHost;
+    std::vector<int64_t> topOffHost;
+    std::vector<float> topValHost;
+    int64_t granules = 0;
+
+    // Resident index in HBM.
+    void *dPost = nullptr, *dSamples = nullptr, *dDir = nullptr, *dBmax = nullptr;
+    IndexView ix{};
+    size_t packetBytes = 0, resultBytes = 0;
+    size_t offQoff = 0, offK = 0, offThr = 0, offTau0 = 0, offPool = 0, offOut = 0, offTerms = 0, offWave = 0;
+    size_t resStatus = 0, resScores = 0, resIds = 0;
+    std::vector<Slot> slots;
+    int device = 0;
+
+    bool failed = false, closed = false, serving = false;
+    bm25mi_stats last{};
+
+    Session(const int64_t *offsets, int64_t v, const int32_t *docs, const float *tf, int64_t m, const float *norm,
+            int64_t n, const float *idf, const bm25mi_config &config);
+    ~Session() { Release(); }
+    Session(const Session &) = delete;
+    Session &operator=(const Session &) = delete;
+
+    void Release() noexcept;
+    // Candidate pool entries one query may need: every block appends at most
+    // BlockKeepCap(K) docs, and no doc is appended twice.
+    int64_t PoolCapacity(int32_t k) const;
+    // Score bits of the largest K-th term impact among the query's terms: every doc scores at
+    // least each of its impacts, so the query's K-th score is at least this (0 if unknown).
+    uint32_t StartBound(const int32_t *terms, int64_t n, int32_t k) const;
+    // Cuts the query-major task order into `grid` contiguous ranges of equal estimated work,
+    // a task of query q costing taskCost + ceil(postings of q / blocks).
+    void PlanWaves(const int64_t *postingsPerQuery, int32_t nq, uint32_t *waveStart) const;
+    // Validates a whole request (throws InvalidArgument, touching nothing, when it is not
+    // acceptable), then writes it into the slot's packet.
+    void Pack(const Query &query, Slot &slot);
+    // Queues the slot's Search on its stream (graph replay, or direct launches with events
+    // around each kernel when timed). Returns at once.
+    void Launch(Slot &slot, bool timed);
+    // True once the slot's launched Search has finished on the device.
+    bool Ready(Slot &slot);
+    // Waits for the slot's Search, checks it and copies the results (rows of query q start at
+    // sum of the K of earlier queries); status[q] = 1 marks a threshold query with fewer than
+    // K survivors, whose rows are left untouched.
+    void Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *status);
+    // Pack + Launch + Complete on slot 0; any device failure marks the session failed.
+    void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
+
+private:
+    void InitSlot(Slot &slot);
+    void Capture(Slot &slot);
+    void Enqueue(Slot &slot, bool timed);
+    void Bootstrap();
+    template <class T> T *Alloc(size_t count, const char *what)
+    {
+        void *p = nullptr;
+        BM25MI_HIP(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)), what);
+        return static_cast<T *>(p);
+    }
+};
+
+void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
+bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
+                 int32_t grid, hipStream_t stream);
+int ScoreBlocksPerCu(int blockDocs);
+void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
+                 int32_t scoreGrid, hipStream_t stream);
+
+} // namespace bm25mi
+
+struct bm25mi_session {
+    std::unique_ptr<bm25mi::Session> impl;
+    bm25mi::ErrorInfo error;
+    std::mutex mutex;
+};

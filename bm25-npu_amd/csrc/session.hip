@@ -1,18 +1,5 @@
-// Host runtime of libbm25mi: the resident-index session behind the C ABI.
-//
-// Mirrors bm25large::Session (FR/ascendhost/src/bm25/large_search.cpp:107-212, FR =
-// feature_retrieval/src/ascendfaiss): the constructor validates and uploads the
-// index once, allocates every per-Search buffer, captures H2D packet copy ->
-// kernels into one hipGraph and runs an empty bootstrap Search that must return
-// IDs 0..K-1 with score 0 (:184-187); Search validates the whole request before
-// touching the reusable packet (FR/common/bm25/fixed_query_packet.h:26-49),
-// replays the graph, synchronises and checks the result the way Session::Receive
-// does (:64-85). A device error marks the session failed for good (:208-210).
-//
-// Differences by design: impacts are precomputed at create time; the directory
-// covers only long posting lists; results land in mapped pinned host memory
-// written by the last kernel, so no D2H copy node is needed; Q and K may vary
-// per call up to the configured capacity, and K may differ per query.
+// Host runtime of libbm25mi: the resident-index session behind the C ABI (declared in
+// session.h, which describes how it mirrors bm25large::Session of the reference).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,139 +13,53 @@
 #include "common.h"
 #include "kernels.h"
 #include "layout.h"
+#include "session.h"
 
 namespace bm25mi {
 
-void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
-bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream);
-int ScoreBlocksPerCu(int blockDocs);
-void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
-                 int32_t scoreGrid, hipStream_t stream);
-
-#define BM25MI_HIP(expr, what)                                                                                         \
-    do {                                                                                                               \
-        const hipError_t e_ = (expr);                                                                                  \
-        if (e_ != hipSuccess)                                                                                          \
-            throw DeviceError(static_cast<int64_t>(e_), std::string(what) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
 static size_t Align(size_t n, size_t a) { return (n + a - 1) / a * a; }
 
-struct Query {
-    const int64_t *offsets;
-    const int32_t *terms;
-    int32_t count;
-    const int32_t *k;   // per query
-    const float *thr;   // per query or nullptr
-};
+int64_t Session::PoolCapacity(int32_t k) const
+{
+    return std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs), ndocs);
+}
 
-class Session {
-public:
-    bm25mi_config cfg{};
-    int64_t ndocs = 0, vocab = 0, postings = 0;
-    int32_t blockDocs = 2048, granule = kGranule, blocks = 0, grid = 0, cus = 0;
-    int64_t poolPerQuery = 0;
-    int64_t taskCost = kTaskCost;
-    std::vector<int64_t> toffHost;
-    std::vector<float> idfHost, tmaxHost, tmeanHost;
-    std::vector<int32_t> slotHost;
-    std::vector<float> bmaxHost;
-    std::vector<int64_t> scratchCost;
-    std::vector<std::pair<float, int64_t>> scratchOrder;
-    std::vector<int64_t> topOffHost;
-    std::vector<float> topValHost;
-    int64_t granules = 0;
-
-    // Device memory.
-    void *dPost = nullptr, *dSamples = nullptr, *dDir = nullptr, *dBmax = nullptr;
-    void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr;
-    // Pinned host memory: request packet (copied by the graph) and result block (written by the device).
-    void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
-    size_t packetBytes = 0, resultBytes = 0;
-    size_t offQoff = 0, offK = 0, offThr = 0, offTau0 = 0, offPool = 0, offOut = 0, offTerms = 0, offWave = 0;
-    size_t resStatus = 0, resScores = 0, resIds = 0;
-
-    IndexView ix{};
-    BatchView bt{};
-    WorkView wk{};
-
-    hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    int device = 0;
-
-    bool failed = false, closed = false;
-    ErrorInfo error;
-    std::mutex mutex;
-    bm25mi_stats last{};
-
-    Session(const int64_t *offsets, int64_t v, const int32_t *docs, const float *tf, int64_t m, const float *norm,
-            int64_t n, const float *idf, const bm25mi_config &config);
-    ~Session() { Release(); }
-
-    void Release() noexcept;
-    // Candidate pool entries one query may need: every block appends at most
-    // BlockKeepCap(K) docs, and no doc is appended twice.
-    int64_t PoolCapacity(int32_t k) const
-    {
-        return std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs), ndocs);
+uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
+{
+    float best = 0.0F;
+    for (int64_t w = 0; w < n; ++w) {
+        const int32_t t = terms[w];
+        if (t >= vocab || topOffHost[t + 1] - topOffHost[t] < k)
+            continue;
+        best = std::max(best, topValHost[topOffHost[t] + k - 1]);
     }
-    // Score bits of the largest K-th term impact among the query's terms: every doc scores at
-    // least each of its impacts, so the query's K-th score is at least this (0 if unknown).
-    uint32_t StartBound(const int32_t *terms, int64_t n, int32_t k) const
-    {
-        float best = 0.0F;
-        for (int64_t w = 0; w < n; ++w) {
-            const int32_t t = terms[w];
-            if (t >= vocab || topOffHost[t + 1] - topOffHost[t] < k)
-                continue;
-            best = std::max(best, topValHost[topOffHost[t] + k - 1]);
+    uint32_t bits;
+    std::memcpy(&bits, &best, 4);
+    return bits;
+}
+
+void Session::PlanWaves(const int64_t *postingsPerQuery, int32_t nq, uint32_t *waveStart) const
+{
+    std::vector<int64_t> cost(static_cast<size_t>(nq));
+    int64_t all = 0;
+    for (int32_t q = 0; q < nq; ++q) {
+        cost[q] = taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
+        all += cost[q] * blocks;
+    }
+    int32_t q = 0;
+    int64_t before = 0; // cost of queries < q
+    for (int32_t w = 0; w <= grid; ++w) {
+        const int64_t target = static_cast<int64_t>((static_cast<__int128>(all) * w) / grid);
+        while (q < nq && before + cost[q] * blocks <= target) {
+            before += cost[q] * blocks;
+            ++q;
         }
-        uint32_t bits;
-        std::memcpy(&bits, &best, 4);
-        return bits;
+        int64_t task = static_cast<int64_t>(q) * blocks;
+        if (q < nq)
+            task += std::min<int64_t>(blocks, (target - before + cost[q] - 1) / cost[q]);
+        waveStart[w] = static_cast<uint32_t>(task);
     }
-    // Cuts the query-major task order into `grid` contiguous ranges of equal estimated work,
-    // a task of query q costing taskCost + ceil(postings of q / blocks).
-    void PlanWaves(const int64_t *postingsPerQuery, int32_t nq, uint32_t *waveStart) const
-    {
-        std::vector<int64_t> cost(static_cast<size_t>(nq));
-        int64_t all = 0;
-        for (int32_t q = 0; q < nq; ++q) {
-            cost[q] = taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
-            all += cost[q] * blocks;
-        }
-        int32_t q = 0;
-        int64_t before = 0; // cost of queries < q
-        for (int32_t w = 0; w <= grid; ++w) {
-            const int64_t target = static_cast<int64_t>((static_cast<__int128>(all) * w) / grid);
-            while (q < nq && before + cost[q] * blocks <= target) {
-                before += cost[q] * blocks;
-                ++q;
-            }
-            int64_t task = static_cast<int64_t>(q) * blocks;
-            if (q < nq)
-                task += std::min<int64_t>(blocks, (target - before + cost[q] - 1) / cost[q]);
-            waveStart[w] = static_cast<uint32_t>(task);
-        }
-    }
-    void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
-    void Run(bool timed);
-    void Bootstrap();
-
-private:
-    void Capture();
-    void Enqueue(bool timed);
-    template <class T> T *Alloc(size_t count, const char *what)
-    {
-        void *p = nullptr;
-        BM25MI_HIP(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)), what);
-        return static_cast<T *>(p);
-    }
-};
+}
 
 Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const float *tf, int64_t m,
                  const float *norm, int64_t n, const float *idf, const bm25mi_config &config)
@@ -169,6 +70,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(cfg.max_k >= 1 && cfg.max_k <= BM25MI_MAX_K, "max_k must be in [1, 1024]");
     Require(std::isfinite(cfg.k1) && cfg.k1 > 0, "k1 must be finite and > 0");
     Require(cfg.task_cost >= 0, "task_cost must be >= 0");
+    Require(cfg.slots >= 0 && cfg.slots <= 16, "slots must be in [0, 16]");
+    Require(cfg.pool_entries >= 0, "pool_entries must be >= 0");
     if (cfg.task_cost > 0)
         taskCost = cfg.task_cost;
     blockDocs = cfg.block_docs ? cfg.block_docs : 2048;
@@ -178,7 +81,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(cfg.max_k <= n, "max_k exceeds the document count");
     device = cfg.device_id;
 
-    // Host-side tables: impacts, directory, block maxima.
+    // Host-side tables: impacts, directory, block maxima, term top impacts.
     blocks = static_cast<int32_t>((n + blockDocs - 1) / blockDocs);
     Layout layout;
     std::vector<float> impact(static_cast<size_t>(m));
@@ -205,18 +108,19 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
     Require(static_cast<int64_t>(cfg.max_queries) * blocks < (int64_t(1) << 32), "Too many scoring tasks per Search");
 
+    // Candidate pools: by default room for max_queries queries of K = max_k in every slot;
+    // a smaller pool_entries budget makes callers admit fewer large-K queries per batch.
+    const int64_t fullPool = static_cast<int64_t>(cfg.max_queries) * PoolCapacity(cfg.max_k);
+    poolEntries = cfg.pool_entries > 0 ? std::min(cfg.pool_entries, fullPool) : fullPool;
+    Require(poolEntries >= PoolCapacity(cfg.max_k), "pool_entries cannot hold one query of K = max_k");
+    const int32_t nslots = cfg.slots > 0 ? cfg.slots : 1;
     size_t freeBytes = 0, totalBytes = 0;
     BM25MI_HIP(hipMemGetInfo(&freeBytes, &totalBytes), "memory info");
-    poolPerQuery = PoolCapacity(cfg.max_k);
     const uint64_t known = static_cast<uint64_t>(m) * 8 + static_cast<uint64_t>(m) / 4 + 4 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
-                           static_cast<uint64_t>(cfg.max_queries) * poolPerQuery * 8;
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(poolEntries) * 8;
     // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
     Require(known <= static_cast<uint64_t>(freeBytes) / 5 * 4, "Index, pools and arenas exceed 80% of free HBM");
-
-    BM25MI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create stream");
-    for (auto &e : ev)
-        BM25MI_HIP(hipEventCreate(&e), "create event");
 
     // Postings as {doc, impact bits}: one 8-byte load per posting.
     {
@@ -248,9 +152,11 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     if (!layout.blockMax.empty())
         BM25MI_HIP(hipMemcpy(dBmax, layout.blockMax.data(), layout.blockMax.size() * 4, hipMemcpyHostToDevice),
                    "upload block max");
-    bmaxHost.swap(layout.blockMax);
+    ix = IndexView{static_cast<const uint2 *>(dPost), static_cast<const int32_t *>(dSamples),
+                   static_cast<const uint32_t *>(dDir), static_cast<const float *>(dBmax), granules,
+                   static_cast<int32_t>(n)};
 
-    // Request packet.
+    // Request packet layout (the same in every slot).
     const size_t q = static_cast<size_t>(cfg.max_queries);
     size_t at = 16;
     offQoff = at;
@@ -270,106 +176,97 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     offWave = at;
     at = Align(at + (static_cast<size_t>(grid) + 1) * 4, 256);
     packetBytes = at;
-    BM25MI_HIP(hipHostMalloc(&hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
-    std::memset(hPacket, 0, packetBytes);
-    dPacket = Alloc<unsigned char>(packetBytes, "alloc device packet");
-
-    // Result block in mapped pinned memory.
-    resStatus = static_cast<size_t>(cfg.max_queries) * kStats * 8; // per final workgroup counter rows
+    // Result block layout: per final-workgroup counter rows, status, scores, IDs.
+    resStatus = q * kStats * 8;
     resScores = Align(resStatus + q * 4, 16);
     resIds = Align(resScores + q * cfg.max_k * 4, 16);
     resultBytes = Align(resIds + q * cfg.max_k * 4, 256);
-    BM25MI_HIP(hipHostMalloc(&hResult, resultBytes, hipHostMallocMapped), "alloc pinned result");
-    std::memset(hResult, 0, resultBytes);
-    BM25MI_HIP(hipHostGetDevicePointer(&hResultDev, hResult, 0), "map pinned result");
 
-    dPool = Alloc<unsigned long long>(q * poolPerQuery, "alloc candidate pools");
-    dCount = Alloc<uint32_t>(q, "alloc pool counts");
-    dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
-    dTau = Alloc<uint32_t>(q, "alloc thresholds");
-    dStats = Alloc<unsigned long long>(static_cast<size_t>(grid) * kStats, "alloc stats");
-
-    ix = IndexView{static_cast<const uint2 *>(dPost), static_cast<const int32_t *>(dSamples),
-                   static_cast<const uint32_t *>(dDir), static_cast<const float *>(dBmax), granules,
-                   static_cast<int32_t>(n)};
-    auto *pk = static_cast<unsigned char *>(dPacket);
-    bt = BatchView{reinterpret_cast<const int32_t *>(pk),           reinterpret_cast<const int64_t *>(pk + offQoff),
-                   reinterpret_cast<const int32_t *>(pk + offK),    reinterpret_cast<const float *>(pk + offThr),
-                   reinterpret_cast<const uint32_t *>(pk + offTau0),
-                   reinterpret_cast<const int64_t *>(pk + offPool), reinterpret_cast<const int64_t *>(pk + offOut),
-                   reinterpret_cast<const uint4 *>(pk + offTerms), reinterpret_cast<const uint32_t *>(pk + offWave)};
-    auto *rs = static_cast<unsigned char *>(hResultDev);
-    wk = WorkView{static_cast<unsigned long long *>(dPool), static_cast<uint32_t *>(dCount),
-                  static_cast<uint32_t *>(dHist),
-                  static_cast<uint32_t *>(dTau),            static_cast<unsigned long long *>(dStats),
-                  reinterpret_cast<int32_t *>(rs + resStatus), reinterpret_cast<float *>(rs + resScores),
-                  reinterpret_cast<int32_t *>(rs + resIds)};
-    LaunchInit(wk, cfg.max_queries, stream);
-    BM25MI_HIP(hipStreamSynchronize(stream), "initialise counters");
-    if (cfg.use_graph)
-        Capture();
+    slots.resize(static_cast<size_t>(nslots));
+    for (auto &slot : slots)
+        InitSlot(slot);
     Bootstrap();
 }
 
-void Session::Enqueue(bool timed)
+void Session::InitSlot(Slot &slot)
 {
-    BM25MI_HIP(hipMemcpyAsync(dPacket, hPacket, packetBytes, hipMemcpyHostToDevice, stream), "packet H2D");
-    if (timed)
-        BM25MI_HIP(hipEventRecord(ev[1], stream), "record score start");
-    Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, stream), "Unsupported block size");
-    if (timed)
-        BM25MI_HIP(hipEventRecord(ev[2], stream), "record score end");
-    LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(hResultDev), grid, stream);
+    const size_t q = static_cast<size_t>(cfg.max_queries);
+    BM25MI_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking), "create stream");
+    for (auto &e : slot.ev)
+        BM25MI_HIP(hipEventCreate(&e), "create event");
+    BM25MI_HIP(hipHostMalloc(&slot.hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
+    std::memset(slot.hPacket, 0, packetBytes);
+    slot.dPacket = Alloc<unsigned char>(packetBytes, "alloc device packet");
+    BM25MI_HIP(hipHostMalloc(&slot.hResult, resultBytes, hipHostMallocMapped), "alloc pinned result");
+    std::memset(slot.hResult, 0, resultBytes);
+    BM25MI_HIP(hipHostGetDevicePointer(&slot.hResultDev, slot.hResult, 0), "map pinned result");
+    slot.dPool = Alloc<unsigned long long>(static_cast<size_t>(poolEntries), "alloc candidate pools");
+    slot.dCount = Alloc<uint32_t>(q, "alloc pool counts");
+    slot.dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
+    slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
+    slot.dStats = Alloc<unsigned long long>(static_cast<size_t>(grid) * kStats, "alloc stats");
+    auto *pk = static_cast<unsigned char *>(slot.dPacket);
+    slot.bt = BatchView{reinterpret_cast<const int32_t *>(pk),
+                        reinterpret_cast<const int64_t *>(pk + offQoff),
+                        reinterpret_cast<const int32_t *>(pk + offK),
+                        reinterpret_cast<const float *>(pk + offThr),
+                        reinterpret_cast<const uint32_t *>(pk + offTau0),
+                        reinterpret_cast<const int64_t *>(pk + offPool),
+                        reinterpret_cast<const int64_t *>(pk + offOut),
+                        reinterpret_cast<const uint4 *>(pk + offTerms),
+                        reinterpret_cast<const uint32_t *>(pk + offWave)};
+    auto *rs = static_cast<unsigned char *>(slot.hResultDev);
+    slot.wk = WorkView{static_cast<unsigned long long *>(slot.dPool),
+                       static_cast<uint32_t *>(slot.dCount),
+                       static_cast<uint32_t *>(slot.dHist),
+                       static_cast<uint32_t *>(slot.dTau),
+                       static_cast<unsigned long long *>(slot.dStats),
+                       reinterpret_cast<int32_t *>(rs + resStatus),
+                       reinterpret_cast<float *>(rs + resScores),
+                       reinterpret_cast<int32_t *>(rs + resIds)};
+    LaunchInit(slot.wk, cfg.max_queries, slot.stream);
+    BM25MI_HIP(hipStreamSynchronize(slot.stream), "initialise counters");
+    if (cfg.use_graph)
+        Capture(slot);
 }
 
-void Session::Capture()
+void Session::Enqueue(Slot &slot, bool timed)
 {
-    BM25MI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    BM25MI_HIP(hipMemcpyAsync(slot.dPacket, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
+               "packet H2D");
+    if (timed)
+        BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
+    Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+    if (timed)
+        BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
+    LaunchFinal(slot.bt, slot.wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev), grid,
+                slot.stream);
+}
+
+void Session::Capture(Slot &slot)
+{
+    BM25MI_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
     try {
-        Enqueue(false);
+        Enqueue(slot, false);
     } catch (...) {
         hipGraph_t discarded = nullptr;
-        (void)hipStreamEndCapture(stream, &discarded);
+        (void)hipStreamEndCapture(slot.stream, &discarded);
         if (discarded)
             (void)hipGraphDestroy(discarded);
         throw;
     }
-    BM25MI_HIP(hipStreamEndCapture(stream, &graph), "end graph capture");
-    BM25MI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate graph");
+    BM25MI_HIP(hipStreamEndCapture(slot.stream, &slot.graph), "end graph capture");
+    BM25MI_HIP(hipGraphInstantiate(&slot.exec, slot.graph, nullptr, nullptr, 0), "instantiate graph");
 }
 
-void Session::Run(bool timed)
-{
-    BM25MI_HIP(hipEventRecord(ev[0], stream), "record start");
-    if (exec && !timed) {
-        BM25MI_HIP(hipGraphLaunch(exec, stream), "replay Search graph");
-    } else {
-        Enqueue(timed);
-    }
-    BM25MI_HIP(hipEventRecord(ev[3], stream), "record end");
-    BM25MI_HIP(hipStreamSynchronize(stream), "complete Search");
-    float ms = 0;
-    BM25MI_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]), "device time");
-    last.device_ms = ms;
-    if (timed) {
-        BM25MI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]), "score kernel time");
-        last.score_kernel_ms = ms;
-        BM25MI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]), "final time");
-        last.final_ms = ms;
-    } else {
-        last.score_kernel_ms = -1;
-        last.final_ms = -1;
-    }
-}
-
-void Session::Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed)
+void Session::Pack(const Query &query, Slot &slot)
 {
     // Validate the complete request before modifying the reusable packet.
     const int32_t nq = query.count;
     Require(nq >= 1 && nq <= cfg.max_queries, "Query batch exceeds declared capacity or is empty");
-    Require(query.offsets && query.k && outScores && outIds, "Null query or output array");
+    Require(query.offsets && query.k, "Null query or K array");
     Require(query.offsets[0] == 0, "query_offsets[0] must be 0");
-    int64_t words = 0;
+    int64_t words = 0, poolNeed = 0;
     for (int32_t q = 0; q < nq; ++q) {
         const int64_t len = query.offsets[q + 1] - query.offsets[q];
         Require(len >= 0 && len <= kMaxTerms && len <= cfg.max_words - words,
@@ -378,12 +275,14 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
         Require(query.k[q] >= 1 && query.k[q] <= cfg.max_k && query.k[q] <= ndocs, "K out of range");
         if (query.thr)
             Require(std::isfinite(query.thr[q]) && query.thr[q] >= 0, "Threshold must be finite and >= 0");
+        poolNeed += PoolCapacity(query.k[q]);
     }
+    Require(poolNeed <= poolEntries, "Batch needs more candidate-pool memory than a slot holds");
     Require(words == 0 || query.terms, "Null query terms");
     for (int64_t w = 0; w < words; ++w)
         Require(query.terms[w] >= 0 && query.terms[w] < INT32_MAX, "Invalid public word ID");
 
-    auto *pk = static_cast<unsigned char *>(hPacket);
+    auto *pk = static_cast<unsigned char *>(slot.hPacket);
     reinterpret_cast<int32_t *>(pk)[0] = nq;
     auto *qoff = reinterpret_cast<int64_t *>(pk + offQoff);
     auto *kk = reinterpret_cast<int32_t *>(pk + offK);
@@ -391,19 +290,20 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
     auto *tau0 = reinterpret_cast<uint32_t *>(pk + offTau0);
     auto *pool = reinterpret_cast<int64_t *>(pk + offPool);
     auto *out = reinterpret_cast<int64_t *>(pk + offOut);
+    slot.out.assign(static_cast<size_t>(nq) + 1, 0);
     qoff[0] = pool[0] = out[0] = 0;
-    int64_t total = 0;
     for (int32_t q = 0; q < nq; ++q) {
         qoff[q + 1] = query.offsets[q + 1];
         kk[q] = query.k[q];
         thr[q] = query.thr ? query.thr[q] : 0.0F;
         pool[q + 1] = pool[q] + PoolCapacity(query.k[q]);
         out[q + 1] = out[q] + query.k[q];
+        slot.out[q + 1] = out[q + 1];
         tau0[q] = StartBound(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q]);
     }
     auto *info = reinterpret_cast<uint4 *>(pk + offTerms);
-    std::vector<int64_t> &perQuery = scratchCost;
-    perQuery.assign(static_cast<size_t>(nq), 0);
+    std::vector<int64_t> perQuery(static_cast<size_t>(nq), 0);
+    int64_t total = 0;
     for (int32_t q = 0; q < nq; ++q)
         for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w) {
             const int32_t t = query.terms[w];
@@ -423,10 +323,10 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
     // non-essential, smallest maximum first, while the sum of their maxima stays below the
     // threshold with a 2^-16 relative margin (any FP32 summation order of up to 64 terms
     // stays within it): a doc with no essential posting then scores below the threshold.
+    std::vector<std::pair<float, int64_t>> order;
     for (int32_t q = 0; q < nq; ++q) {
         if (!(thr[q] > 0.0F))
             continue;
-        std::vector<std::pair<float, int64_t>> &order = scratchOrder;
         order.clear();
         for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w)
             if (info[w].y > 0)
@@ -443,54 +343,100 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
             info[order[i].second].w |= 0x80000000u;
     }
     PlanWaves(perQuery.data(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
+    slot.nq = nq;
+    slot.total = total;
+}
 
-    try {
-        Run(timed);
-        auto *res = static_cast<unsigned char *>(hResult);
-        const auto *rows = reinterpret_cast<const unsigned long long *>(res);
-        unsigned long long stats[kStats] = {};
-        for (int32_t r = 0; r < cfg.max_queries; ++r)
-            for (int i = 0; i < kStats; ++i)
-                stats[i] = i >= kStatMax ? std::max(stats[i], rows[r * kStats + i]) : stats[i] + rows[r * kStats + i];
-        const auto *st = reinterpret_cast<const int32_t *>(res + resStatus);
-        const auto *sc = reinterpret_cast<const float *>(res + resScores);
-        const auto *id = reinterpret_cast<const int32_t *>(res + resIds);
-        if (stats[kStatOverflow])
-            throw DeviceError(0, "Candidate pool overflow");
-        int32_t fallbacks = 0;
-        for (int32_t q = 0; q < nq; ++q) {
-            status[q] = st[q];
-            fallbacks += st[q] ? 1 : 0;
-            if (st[q])
-                continue;
-            // Receive checks of large_search.cpp:64-85.
-            for (int64_t r = out[q]; r < out[q + 1]; ++r) {
-                if (!(id[r] >= 0 && id[r] < ndocs && std::isfinite(sc[r]) && sc[r] >= 0))
-                    throw DeviceError(0, "Invalid real-document result");
-                if (r > out[q] &&
-                    !(sc[r - 1] > sc[r] || (sc[r - 1] == sc[r] && id[r - 1] < id[r])))
-                    throw DeviceError(0, "Strict global ranking failure");
-            }
-            std::memcpy(outScores + out[q], sc + out[q], (out[q + 1] - out[q]) * 4);
-            std::memcpy(outIds + out[q], id + out[q], (out[q + 1] - out[q]) * 4);
+void Session::Launch(Slot &slot, bool timed)
+{
+    slot.timed = timed;
+    BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
+    if (slot.exec && !timed)
+        BM25MI_HIP(hipGraphLaunch(slot.exec, slot.stream), "replay Search graph");
+    else
+        Enqueue(slot, timed);
+    BM25MI_HIP(hipEventRecord(slot.ev[3], slot.stream), "record end");
+}
+
+bool Session::Ready(Slot &slot)
+{
+    const hipError_t e = hipStreamQuery(slot.stream);
+    if (e == hipErrorNotReady)
+        return false;
+    BM25MI_HIP(e, "query Search stream");
+    return true;
+}
+
+void Session::Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *status)
+{
+    BM25MI_HIP(hipStreamSynchronize(slot.stream), "complete Search");
+    bm25mi_stats &st = slot.last;
+    float ms = 0;
+    BM25MI_HIP(hipEventElapsedTime(&ms, slot.ev[0], slot.ev[3]), "device time");
+    st.device_ms = ms;
+    st.score_kernel_ms = st.final_ms = -1;
+    if (slot.timed) {
+        BM25MI_HIP(hipEventElapsedTime(&ms, slot.ev[1], slot.ev[2]), "score kernel time");
+        st.score_kernel_ms = ms;
+        BM25MI_HIP(hipEventElapsedTime(&ms, slot.ev[2], slot.ev[3]), "final time");
+        st.final_ms = ms;
+    }
+    auto *res = static_cast<unsigned char *>(slot.hResult);
+    const auto *rows = reinterpret_cast<const unsigned long long *>(res);
+    unsigned long long stats[kStats] = {};
+    for (int32_t r = 0; r < cfg.max_queries; ++r)
+        for (int i = 0; i < kStats; ++i)
+            stats[i] = i >= kStatMax ? std::max(stats[i], rows[r * kStats + i]) : stats[i] + rows[r * kStats + i];
+    const auto *stq = reinterpret_cast<const int32_t *>(res + resStatus);
+    const auto *sc = reinterpret_cast<const float *>(res + resScores);
+    const auto *id = reinterpret_cast<const int32_t *>(res + resIds);
+    if (stats[kStatOverflow])
+        throw DeviceError(0, "Candidate pool overflow");
+    int32_t fallbacks = 0;
+    const auto &out = slot.out;
+    for (int32_t q = 0; q < slot.nq; ++q) {
+        status[q] = stq[q];
+        fallbacks += stq[q] ? 1 : 0;
+        if (stq[q])
+            continue;
+        // Receive checks of large_search.cpp:64-85.
+        for (int64_t r = out[q]; r < out[q + 1]; ++r) {
+            if (!(id[r] >= 0 && id[r] < ndocs && std::isfinite(sc[r]) && sc[r] >= 0))
+                throw DeviceError(0, "Invalid real-document result");
+            if (r > out[q] && !(sc[r - 1] > sc[r] || (sc[r - 1] == sc[r] && id[r - 1] < id[r])))
+                throw DeviceError(0, "Strict global ranking failure");
         }
-        last.postings_scanned = static_cast<int64_t>(stats[kStatPostings]);
-        last.postings_in_range = static_cast<int64_t>(stats[kStatUseful]);
-        last.emitting_blocks = static_cast<int64_t>(stats[kStatEmitting]);
-        last.exact_blocks = static_cast<int64_t>(stats[kStatExact]);
-        last.postings_total = total;
-        last.score_grid = grid;
-        for (int i = 0; i < 4; ++i)
-            last.phase_cycles[i] = static_cast<int64_t>(stats[kStatPhase + i]);
-        last.wave_busy_max = static_cast<int64_t>(stats[kStatMax]);
-        last.wave_span = static_cast<int64_t>(stats[kStatMax + 1] - ~stats[kStatMax + 2]);
-        last.wave_start_spread = static_cast<int64_t>(stats[kStatMax + 3] - ~stats[kStatMax + 2]);
-        last.block_docs = blockDocs;
-        last.blocks_scored = static_cast<int64_t>(stats[kStatScored]);
-        last.blocks_skipped = static_cast<int64_t>(stats[kStatSkipped]);
-        last.candidates = static_cast<int64_t>(stats[kStatCandidates]);
-        last.fallbacks = fallbacks;
-        last.queries = nq;
+        std::memcpy(outScores + out[q], sc + out[q], (out[q + 1] - out[q]) * 4);
+        std::memcpy(outIds + out[q], id + out[q], (out[q + 1] - out[q]) * 4);
+    }
+    st.postings_scanned = static_cast<int64_t>(stats[kStatPostings]);
+    st.postings_in_range = static_cast<int64_t>(stats[kStatUseful]);
+    st.emitting_blocks = static_cast<int64_t>(stats[kStatEmitting]);
+    st.exact_blocks = static_cast<int64_t>(stats[kStatExact]);
+    st.postings_total = slot.total;
+    st.score_grid = grid;
+    for (int i = 0; i < 4; ++i)
+        st.phase_cycles[i] = static_cast<int64_t>(stats[kStatPhase + i]);
+    st.wave_busy_max = static_cast<int64_t>(stats[kStatMax]);
+    st.wave_span = static_cast<int64_t>(stats[kStatMax + 1] - ~stats[kStatMax + 2]);
+    st.wave_start_spread = static_cast<int64_t>(stats[kStatMax + 3] - ~stats[kStatMax + 2]);
+    st.block_docs = blockDocs;
+    st.blocks_scored = static_cast<int64_t>(stats[kStatScored]);
+    st.blocks_skipped = static_cast<int64_t>(stats[kStatSkipped]);
+    st.candidates = static_cast<int64_t>(stats[kStatCandidates]);
+    st.fallbacks = fallbacks;
+    st.queries = slot.nq;
+}
+
+void Session::Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed)
+{
+    Require(outScores && outIds, "Null output array");
+    Slot &slot = slots[0];
+    Pack(query, slot); // input errors leave the session usable
+    try {
+        Launch(slot, timed);
+        Complete(slot, outScores, outIds, status);
+        last = slot.last;
     } catch (...) {
         failed = true;
         throw;
@@ -504,39 +450,55 @@ void Session::Bootstrap()
     std::vector<int32_t> ks(q, k), status(q);
     std::vector<float> scores(static_cast<size_t>(q) * k);
     std::vector<int32_t> ids(scores.size());
-    Query query{offsets.data(), nullptr, q, ks.data(), nullptr};
-    Search(query, scores.data(), ids.data(), status.data(), false);
-    for (size_t i = 0; i < ids.size(); ++i)
-        if (!(scores[i] == 0 && ids[i] == static_cast<int32_t>(i % k)))
-            throw DeviceError(0, "Empty bootstrap Search failed");
+    // Each query needs PoolCapacity(max_k); a smaller pool budget bootstraps fewer at a time.
+    const int32_t per = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(q, poolEntries / PoolCapacity(k))));
+    for (auto &slot : slots) {
+        for (int32_t q0 = 0; q0 < q; q0 += per) {
+            const int32_t nq = std::min(per, q - q0);
+            Query query{offsets.data(), nullptr, nq, ks.data(), nullptr};
+            Pack(query, slot);
+            Launch(slot, false);
+            Complete(slot, scores.data(), ids.data(), status.data());
+            for (size_t i = 0; i < static_cast<size_t>(nq) * k; ++i)
+                if (!(scores[i] == 0 && ids[i] == static_cast<int32_t>(i % k)))
+                    throw DeviceError(0, "Empty bootstrap Search failed");
+        }
+    }
 }
 
 void Session::Release() noexcept
 {
-    if (exec)
-        (void)hipGraphExecDestroy(exec);
-    if (graph)
-        (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-    for (auto &e : ev)
-        if (e) {
-            (void)hipEventDestroy(e);
-            e = nullptr;
+    for (auto &slot : slots) {
+        if (slot.exec)
+            (void)hipGraphExecDestroy(slot.exec);
+        if (slot.graph)
+            (void)hipGraphDestroy(slot.graph);
+        slot.exec = nullptr;
+        slot.graph = nullptr;
+        for (auto &e : slot.ev)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        if (slot.stream) {
+            (void)hipStreamSynchronize(slot.stream);
+            (void)hipStreamDestroy(slot.stream);
+            slot.stream = nullptr;
         }
-    if (stream) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipStreamDestroy(stream);
-        stream = nullptr;
+        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+        for (void **p : {&slot.hPacket, &slot.hResult})
+            if (*p) {
+                (void)hipHostFree(*p);
+                *p = nullptr;
+            }
     }
-    for (void **p : {&dPost, &dSamples, &dDir, &dBmax, &dPacket, &dPool, &dCount, &dTau, &dStats, &dHist})
+    for (void **p : {&dPost, &dSamples, &dDir, &dBmax})
         if (*p) {
             (void)hipFree(*p);
-            *p = nullptr;
-        }
-    for (void **p : {&hPacket, &hResult})
-        if (*p) {
-            (void)hipHostFree(*p);
             *p = nullptr;
         }
 }
@@ -544,12 +506,6 @@ void Session::Release() noexcept
 } // namespace bm25mi
 
 using namespace bm25mi;
-
-struct bm25mi_session {
-    std::unique_ptr<Session> impl;
-    ErrorInfo error;
-    std::mutex mutex;
-};
 
 extern "C" void bm25mi_config_default(bm25mi_config *cfg)
 {
@@ -564,6 +520,9 @@ extern "C" void bm25mi_config_default(bm25mi_config *cfg)
     cfg->k1 = 1.2;
     cfg->use_graph = 1;
     cfg->grid_blocks = 0;
+    cfg->task_cost = 0;
+    cfg->slots = 1;
+    cfg->pool_entries = 0;
 }
 
 extern "C" int bm25mi_create(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,
@@ -594,6 +553,8 @@ static int SearchCommon(bm25mi_session *s, const Query &query, float *scores, in
             const auto start = std::chrono::steady_clock::now();
             if (!s->impl || s->impl->closed || s->impl->failed)
                 throw InvalidState("BM25 session closed/failed");
+            if (s->impl->serving)
+                throw InvalidState("BM25 session is serving: submit through its server");
             Session &ss = *s->impl;
             std::vector<int32_t> status(std::max(0, query.count));
             ss.Search(query, scores, ids, status.data(), timed);
@@ -806,6 +767,8 @@ extern "C" int bm25mi_destroy(bm25mi_session *s)
         rc = Invoke(
             [&] {
                 if (s->impl) {
+                    if (s->impl->serving)
+                        throw InvalidState("Stop the session's server before Close");
                     s->impl->closed = true;
                     BM25MI_HIP(hipSetDevice(s->impl->device), "set device for close");
                     s->impl.reset();

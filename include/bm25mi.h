@@ -66,6 +66,10 @@ typedef struct {
     int32_t grid_blocks;     /* persistent scoring waves; 0 = as many as fit the CUs */
     int32_t task_cost;       /* estimated fixed cost of one scoring task in postings, used to give
                                 every wave equal work; 0 = library default */
+    int32_t slots;           /* in-flight Search contexts (buffers, stream, graph) for the server;
+                                0 or 1 = one */
+    int64_t pool_entries;    /* candidate-pool entries per slot; 0 = max_queries queries of K = max_k
+                                (a batch whose queries need more is rejected / split by the server) */
 } bm25mi_config;
 
 /* Per-Search counters, filled by bm25mi_last_stats. */
@@ -159,6 +163,33 @@ int64_t bm25mi_last_error(const bm25mi_session *s, char *buf, size_t size);
 
 /* Device properties of the session's GPU: compute units, LDS bytes per block. */
 int bm25mi_device_info(int32_t device_id, int32_t *cus, int32_t *lds_per_block, int32_t *lds_per_cu);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop with continuous batching (no reference counterpart). A server takes
+ * requests from any thread and keeps up to cfg.slots batches in flight on the session:
+ * whenever a slot is free it admits the oldest waiting requests, up to max_batch queries,
+ * the word capacity and the slot's candidate-pool memory (pool_entries), and launches them
+ * without waiting, so requests that arrive while others run are batched and overlap on the
+ * GPU. Results equal a Search of the same query alone. While a server runs, the session's
+ * own Search entry points and bm25mi_destroy return BM25MI_ERR_ILLEGAL_OPERATION.
+ * ------------------------------------------------------------------------- */
+typedef struct bm25mi_server bm25mi_server;
+int bm25mi_server_start(bm25mi_session *s, int32_t max_batch, bm25mi_server **out);
+/* Queues one query (n term IDs, top-K); *ticket identifies it for bm25mi_server_wait. */
+int bm25mi_server_submit(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, uint64_t *ticket);
+/* Blocks until the query is done; writes its K scores and doc IDs (either may be NULL) and
+ * the submit-to-completion latency in microseconds (may be NULL). */
+int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *scores, int32_t *ids, double *latency_us);
+/* Open-loop load: query i (terms offsets[i]..offsets[i+1], K ks[i]) is submitted at the i-th
+ * arrival of a seeded Poisson process of rate qps; latency_us[i] gets its latency, scores /
+ * ids (optional) its rows at offset sum_{j<i} ks[j], *elapsed_s the wall time of the run. */
+int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offsets, const int32_t *terms, int32_t n,
+                              const int32_t *ks, double qps, uint64_t seed, double *latency_us, float *scores,
+                              int32_t *ids, double *elapsed_s);
+/* Batches launched, queries served and the largest number of batches in flight so far. */
+int bm25mi_server_counters(bm25mi_server *srv, int64_t *batches, int64_t *queries, int32_t *max_inflight);
+/* Finishes the submitted queries and stops; NULL is a no-op. */
+int bm25mi_server_stop(bm25mi_server *srv);
 
 /* ---------------------------------------------------------------------------
  * Index builder (host C++, no GPU). The reference's Python builder is
