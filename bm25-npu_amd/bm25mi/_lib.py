@@ -91,8 +91,9 @@ SIGNATURES = {
     'bm25mi_server_start': (C.c_int, [P, i32, P]),
     'bm25mi_server_submit': (C.c_int, [P, P, i32, i32, P]),
     'bm25mi_server_wait': (C.c_int, [P, C.c_uint64, P, P, P]),
-    'bm25mi_server_run_poisson': (C.c_int, [P, P, P, i32, P, f64, C.c_uint64, P, P, P, P]),
-    'bm25mi_server_counters': (C.c_int, [P, P, P, P]),
+    'bm25mi_server_submit_pruned': (C.c_int, [P, P, i32, i32, f32, P]),
+    'bm25mi_server_run_poisson': (C.c_int, [P, P, P, i32, P, P, f64, C.c_uint64, P, P, P, P]),
+    'bm25mi_server_counters': (C.c_int, [P, P, P, P, P]),
     'bm25mi_server_stop': (C.c_int, [P]),
     'bm25mi_last_stats': (C.c_int, [P, P]),
     'bm25mi_destroy': (C.c_int, [P]),

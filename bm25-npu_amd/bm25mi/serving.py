@@ -31,11 +31,16 @@ class Server:
         _lib.check(self._lib.bm25mi_server_start(session.handle, max_batch, C.byref(handle)))
         self._h = handle
 
-    def submit(self, terms, k):
+    def submit(self, terms, k, threshold=None):
+        """Queues one query; with a threshold it runs pruned (and reruns plain if it must)."""
         terms = np.ascontiguousarray(terms, np.int32)
         ticket = C.c_uint64()
-        _lib.check(self._lib.bm25mi_server_submit(self._h, _lib.ptr(terms) if terms.size else None, len(terms), k,
-                                                  C.byref(ticket)))
+        tp = _lib.ptr(terms) if terms.size else None
+        if threshold is None:
+            _lib.check(self._lib.bm25mi_server_submit(self._h, tp, len(terms), k, C.byref(ticket)))
+        else:
+            _lib.check(self._lib.bm25mi_server_submit_pruned(self._h, tp, len(terms), k, float(threshold),
+                                                             C.byref(ticket)))
         return ticket.value
 
     def wait(self, ticket, k):
@@ -47,7 +52,7 @@ class Server:
                                                 C.byref(lat)))
         return scores, ids, lat.value
 
-    def run_poisson(self, queries, ks, qps, seed=0, keep_results=True):
+    def run_poisson(self, queries, ks, qps, seed=0, keep_results=True, thresholds=None):
         """Open-loop load: query i is submitted at the i-th arrival of a seeded Poisson process
         of rate qps. Returns latencies (us), sustained QPS, percentiles and, when asked, the
         concatenated result rows."""
@@ -55,6 +60,7 @@ class Server:
         offsets = np.ascontiguousarray(offsets, np.int64)
         terms = np.ascontiguousarray(terms, np.int32)
         ks = np.ascontiguousarray(ks, np.int32)
+        thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
         n = len(ks)
         lat = np.empty(n, np.float64)
         rows = int(ks.sum())
@@ -62,8 +68,8 @@ class Server:
         ids = np.empty(rows, np.int32) if keep_results else None
         elapsed = C.c_double()
         _lib.check(self._lib.bm25mi_server_run_poisson(self._h, _lib.ptr(offsets), _lib.ptr(terms), n, _lib.ptr(ks),
-                                                       float(qps), C.c_uint64(seed), _lib.ptr(lat), _lib.ptr(scores),
-                                                       _lib.ptr(ids), C.byref(elapsed)))
+                                                       _lib.ptr(thr), float(qps), C.c_uint64(seed), _lib.ptr(lat),
+                                                       _lib.ptr(scores), _lib.ptr(ids), C.byref(elapsed)))
         out = {'latency_us': lat, 'elapsed_s': elapsed.value, 'offered_qps': float(qps),
                'sustained_qps': n / elapsed.value, **percentiles(lat)}
         if keep_results:
@@ -71,9 +77,9 @@ class Server:
         return out
 
     def counters(self):
-        b, q, m = C.c_int64(), C.c_int64(), C.c_int32()
-        _lib.check(self._lib.bm25mi_server_counters(self._h, C.byref(b), C.byref(q), C.byref(m)))
-        return {'batches': b.value, 'queries': q.value, 'max_inflight': m.value}
+        b, q, m, f = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64()
+        _lib.check(self._lib.bm25mi_server_counters(self._h, C.byref(b), C.byref(q), C.byref(m), C.byref(f)))
+        return {'batches': b.value, 'queries': q.value, 'max_inflight': m.value, 'fallbacks': f.value}
 
     def stop(self):
         if self._h is not None and self._h.value:

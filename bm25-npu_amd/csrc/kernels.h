@@ -16,7 +16,7 @@ constexpr int kFinalThreads = 1024;        // selection workgroup of the final k
 constexpr int kSortCap = 16384;            // pool entries the final kernel sorts in LDS (128 KB)
 constexpr int kTaskCost = 4096;            // estimated fixed cost of a scoring task, in postings
 #ifndef BM25MI_COLUMNS
-#define BM25MI_COLUMNS 8
+#define BM25MI_COLUMNS 12
 #endif
 constexpr int kColumns = BM25MI_COLUMNS;   // 64-posting columns a wave keeps in flight per round
 constexpr int kStats = 16;

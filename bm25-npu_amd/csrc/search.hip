@@ -564,63 +564,97 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             return streamed;
         };
 
-        // 1. Posting range of each word in the block (lane j: word j). Long lists: the
-        // directory, or on the plain path the cursor (this block starts where the previous one
-        // ended; the next block's end is loaded one task ahead). Short lists: sample windows.
+        // Threshold path (MaxScore on granules). The host marked each word essential (bit 31
+        // of its max-impact word) unless the sum of the non-essential words' maxima stays
+        // below the threshold: a doc with no essential posting cannot reach it. With no
+        // non-essential word, the block is scored directly (docs below the threshold are
+        // dropped by the selection floor).
+        const bool ess = info.y > 0 && (info.w >> 31);
+        const bool pruned = thr > 0.0F && __ballot(static_cast<int>(lane) < n && info.y > 0 && !ess) != 0;
+        const float thrLive = thr * 0.99998474F; // thr * (1 - 2^-16): covers any summation order
         uint32_t lo = 0, len = 0;
-        if (isLong) {
-            if (cursorTask == task) {
-                lo = hiPrev;
-                len = hiNext - lo;
-            } else {
-                lo = info.x + dirRow[g0];
-                len = info.x + dirRow[g0 + gpb] - lo;
-            }
-        }
-        hiPrev = lo + len;
-        cursorTask = 0xFFFFFFFFu;
-        if (thr <= 0.0F && task + 1 < tEnd && blk + 1 < blocks) {
-            cursorTask = task + 1;
-            if (isLong)
-                hiNext = info.x + dirRow[min(g0 + 2 * kGpb, ix.granules)];
-        }
-        unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
+        float cg = 0.0F; // lane g < gpb: FP32 sum of the non-essential words' maxima in granule g
+        // 1. Posting range of each word in the block (lane j: word j). Short lists: sample
+        // windows (on the threshold path only the essential ones, needed by the pre-check).
+        unsigned long long shortMask =
+            __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0 && (!pruned || ess));
         while (shortMask) {
             const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
             shortMask &= shortMask - 1;
             shortWindow(j, base, end, lo, len);
         }
-        if (WaveSum(len) == 0) {
-            ++skipped; // no query word has postings here: every score is 0
-            continue;
-        }
-
-        // Threshold path (MaxScore on granules). The host marked each word essential (bit 31
-        // of its max-impact word) unless the sum of the non-essential words' maxima stays
-        // below the threshold: a doc with no essential posting cannot reach it. Pass 0 adds
-        // only the essential words' impacts; a doc's score is at most that plus the sum of
-        // the non-essential words' maxima in its granule (FP32 addition is monotone; the
-        // threshold is lowered by 2^-16 relative to cover any summation order), so granules
-        // where no doc reaches the threshold are dead. Pass 1 scores the live granules exactly.
-        // The plain path makes the exact pass directly.
-        const bool ess = info.y > 0 && (info.w >> 31);
-        // With no non-essential word the bound pass would compute the exact scores: score
-        // directly (docs below the threshold are dropped by the selection floor).
-        const bool pruned = thr > 0.0F && __ballot(static_cast<int>(lane) < n && info.y > 0 && !ess) != 0;
-        uint32_t segLo = lo, segLen = pruned && !ess ? 0u : len;
-        int nseg = n;
-        float cg = 0.0F; // lane g < gpb: FP32 sum of the non-essential words' maxima in granule g
         if (pruned) {
-            for (int j = 0; j < n; ++j) {
-                const uint32_t wj = ReadLane(info.w, j);
-                if (ReadLane(info.y, j) == 0 || (wj >> 31))
-                    continue;
-                const int32_t slj = static_cast<int32_t>(ReadLane(info.z, j));
-                if (static_cast<int>(lane) < gpb)
-                    cg += slj >= 0 ? ix.bmax[static_cast<int64_t>(slj) * ix.granules + g0 + lane]
-                                   : __uint_as_float(wj & 0x7FFFFFFFu);
+            // Pre-check: an FP32 upper bound of every doc's score in granule g, in query order:
+            // long words add their granule maximum, short words their term maximum (an
+            // essential one only when its window reaches into the block). A block where no
+            // granule reaches the threshold is skipped before any posting is read. The
+            // non-essential part is kept as cg for the pass below.
+            float ub = 0.0F;
+            constexpr int kWordsPerLoad = kWave / kGpb;
+            for (int j0 = 0; j0 < n; j0 += kWordsPerLoad) {
+                // Granule maxima of up to kWordsPerLoad words in one load (lane (j-j0)*kGpb + g).
+                const int jr = j0 + static_cast<int>(lane) / kGpb, gl = static_cast<int>(lane) % kGpb;
+                const int jl = min(jr, kWave - 1);
+                const uint32_t ly = static_cast<uint32_t>(__shfl(static_cast<int>(info.y), jl));
+                const int32_t lsl = __shfl(static_cast<int>(info.z), jl);
+                float bv = 0.0F;
+                if (jr < n && ly > 0 && lsl >= 0 && gl < gpb)
+                    bv = ix.bmax[static_cast<int64_t>(lsl) * ix.granules + g0 + gl];
+                for (int j = j0; j < min(n, j0 + kWordsPerLoad); ++j) {
+                    if (ReadLane(info.y, j) == 0)
+                        continue;
+                    const uint32_t wj = ReadLane(info.w, j);
+                    const bool essj = (wj >> 31) != 0;
+                    float c;
+                    if (static_cast<int32_t>(ReadLane(info.z, j)) >= 0)
+                        c = __shfl(bv, (j - j0) * kGpb + gl);
+                    else
+                        c = (!essj || ReadLane(len, j) > 0) ? __uint_as_float(wj & 0x7FFFFFFFu) : 0.0F;
+                    if (static_cast<int>(lane) < gpb) {
+                        ub += c;
+                        if (!essj)
+                            cg += c;
+                    }
+                }
+            }
+            if (!__ballot(static_cast<int>(lane) < gpb && ub >= thrLive)) {
+                ++skipped; // no doc of the block can reach the threshold
+                continue;
+            }
+            if (ess && isLong) {
+                lo = info.x + dirRow[g0];
+                len = info.x + dirRow[g0 + gpb] - lo;
+            }
+            cursorTask = 0xFFFFFFFFu;
+        } else {
+            // Long lists: the directory, or on the plain path the cursor (this block starts where
+            // the previous one ended; the next block's end is loaded one task ahead).
+            if (isLong) {
+                if (cursorTask == task) {
+                    lo = hiPrev;
+                    len = hiNext - lo;
+                } else {
+                    lo = info.x + dirRow[g0];
+                    len = info.x + dirRow[g0 + gpb] - lo;
+                }
+            }
+            hiPrev = lo + len;
+            cursorTask = 0xFFFFFFFFu;
+            if (thr <= 0.0F && task + 1 < tEnd && blk + 1 < blocks) {
+                cursorTask = task + 1;
+                if (isLong)
+                    hiNext = info.x + dirRow[min(g0 + 2 * kGpb, ix.granules)];
+            }
+            if (WaveSum(len) == 0) {
+                ++skipped; // no query word has postings here: every score is 0
+                continue;
             }
         }
+        // Pass 0 (threshold path) adds only the essential words' impacts; a doc's score is at
+        // most that plus cg of its granule, so granules where no doc reaches the threshold are
+        // dead. Pass 1 scores the live granules exactly (the plain path makes only pass 1).
+        uint32_t segLo = lo, segLen = len;
+        int nseg = n;
         BM25MI_PHASE(1);
         bool dead = false;
         for (int pass = pruned ? 0 : 1; pass < 2; ++pass) {
@@ -628,7 +662,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             scanned += streamAdd(segLo, segLen, nseg);
             if (pass == 1)
                 break;
-            const float thrLive = thr * 0.99998474F; // thr * (1 - 2^-16)
             unsigned long long live = 0;
 #pragma unroll
             for (int r = 0; r < kVec; ++r) {
@@ -643,9 +676,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 dead = true;
                 break;
             }
-            // Pass 1 segments: each long word's runs of live granules and each short word's
-            // window, in query order (lane s: segment s); more than 64 fall back to one span
-            // per word over [first live, last live] granules.
+            // Windows of the non-essential short words, then the pass 1 segments: each long
+            // word's runs of live granules and each short word's window, in query order (lane
+            // s: segment s); more than 64 fall back to one span per word over [first live,
+            // last live] granules.
+            unsigned long long nonEssShort = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0 && !ess);
+            while (nonEssShort) {
+                const int j = __ffsll(static_cast<long long>(nonEssShort)) - 1;
+                nonEssShort &= nonEssShort - 1;
+                shortWindow(j, base, end, lo, len);
+            }
             const int ga = __ffsll(static_cast<long long>(live)) - 1;
             const int gz = 64 - __clzll(static_cast<long long>(live));
             uint32_t sLo = 0, sLen = 0;

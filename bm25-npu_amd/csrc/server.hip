@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -34,6 +35,9 @@ using Clock = std::chrono::steady_clock;
 struct Request {
     std::vector<int32_t> terms;
     int32_t k = 0;
+    float thr = 0.0F;       // pruning threshold (0 = plain); a pruned run that keeps fewer than K
+                            // docs is requeued at the head as a plain request
+    bool fellBack = false;
     Clock::time_point submitted, finished;
     std::vector<float> scores;
     std::vector<int32_t> ids;
@@ -63,8 +67,9 @@ public:
         session_.serving = false;
     }
 
-    uint64_t Submit(const int32_t *terms, int32_t n, int32_t k)
+    uint64_t Submit(const int32_t *terms, int32_t n, int32_t k, float thr = 0.0F)
     {
+        Require(std::isfinite(thr) && thr >= 0.0F, "Threshold must be finite and >= 0");
         Require(n >= 0 && n <= kMaxTerms && n <= session_.cfg.max_words, "Query exceeds word capacity or 64 terms");
         Require(n == 0 || terms, "Null query terms");
         Require(k >= 1 && k <= session_.cfg.max_k && k <= session_.ndocs, "K out of range");
@@ -74,6 +79,7 @@ public:
         auto req = std::make_unique<Request>();
         req->terms.assign(terms, terms + n);
         req->k = k;
+        req->thr = thr;
         uint64_t ticket;
         {
             std::lock_guard<std::mutex> g(mutex_);
@@ -111,7 +117,7 @@ public:
             *latencyUs = std::chrono::duration<double, std::micro>(req->finished - req->submitted).count();
     }
 
-    void Counters(int64_t *batches, int64_t *queries, int32_t *maxInflight)
+    void Counters(int64_t *batches, int64_t *queries, int32_t *maxInflight, int64_t *fallbacks)
     {
         std::lock_guard<std::mutex> g(mutex_);
         if (batches)
@@ -120,6 +126,8 @@ public:
             *queries = queries_;
         if (maxInflight)
             *maxInflight = maxInflight_;
+        if (fallbacks)
+            *fallbacks = fallbacks_;
     }
 
 private:
@@ -131,12 +139,13 @@ private:
 
     // Admits the oldest waiting requests into a batch for a free slot (called with the lock).
     bool TakeBatch(std::vector<uint64_t> &tickets, std::vector<int64_t> &offsets, std::vector<int32_t> &terms,
-                   std::vector<int32_t> &ks)
+                   std::vector<int32_t> &ks, std::vector<float> &thr)
     {
         tickets.clear();
         offsets.assign(1, 0);
         terms.clear();
         ks.clear();
+        thr.clear();
         int64_t pool = 0;
         while (!queue_.empty() && static_cast<int32_t>(tickets.size()) < maxBatch_) {
             const Request &r = *requests_.at(queue_.front());
@@ -149,25 +158,36 @@ private:
             terms.insert(terms.end(), r.terms.begin(), r.terms.end());
             offsets.push_back(static_cast<int64_t>(terms.size()));
             ks.push_back(r.k);
+            thr.push_back(r.thr);
             queue_.pop_front();
         }
         return !tickets.empty();
     }
 
-    void Finish(InFlight &f, const std::vector<float> &scores, const std::vector<int32_t> &ids, int status,
-                const std::string &error)
+    // Hands each request its rows; a pruned request with fewer than K survivors (status 1)
+    // goes back to the head of the queue as a plain request instead.
+    void Finish(InFlight &f, const std::vector<float> &scores, const std::vector<int32_t> &ids,
+                const std::vector<int32_t> *status, int code, const std::string &error)
     {
         const auto now = Clock::now();
         std::lock_guard<std::mutex> g(mutex_);
         int64_t row = 0;
         for (size_t i = 0; i < f.tickets.size(); ++i) {
             Request &r = *requests_.at(f.tickets[i]);
-            if (status == BM25MI_OK) {
+            if (code == BM25MI_OK && status && (*status)[i]) {
+                r.thr = 0.0F;
+                r.fellBack = true;
+                ++fallbacks_;
+                queue_.push_front(f.tickets[i]);
+                row += f.ks[i];
+                continue;
+            }
+            if (code == BM25MI_OK) {
                 r.scores.assign(scores.begin() + row, scores.begin() + row + f.ks[i]);
                 r.ids.assign(ids.begin() + row, ids.begin() + row + f.ks[i]);
             }
             row += f.ks[i];
-            r.status = status;
+            r.status = code;
             r.error = error;
             r.finished = now;
             r.done = true;
@@ -183,7 +203,7 @@ private:
         std::vector<uint64_t> tickets;
         std::vector<int64_t> offsets;
         std::vector<int32_t> terms, ks, status;
-        std::vector<float> scores;
+        std::vector<float> scores, thr;
         std::vector<int32_t> ids;
         for (;;) {
             bool launched = false;
@@ -207,11 +227,13 @@ private:
                     return;
                 if (queue_.empty() && inflight.empty())
                     continue;
-                if (!free.empty() && !failed_ && TakeBatch(tickets, offsets, terms, ks)) {
+                if (!free.empty() && !failed_ && TakeBatch(tickets, offsets, terms, ks, thr)) {
                     lock.unlock();
                     Slot *slot = free.back();
                     free.pop_back();
-                    Query q{offsets.data(), terms.data(), static_cast<int32_t>(ks.size()), ks.data(), nullptr};
+                    const bool anyThr = std::any_of(thr.begin(), thr.end(), [](float t) { return t > 0.0F; });
+                    Query q{offsets.data(), terms.data(), static_cast<int32_t>(ks.size()), ks.data(),
+                            anyThr ? thr.data() : nullptr};
                     InFlight f{slot, tickets, ks};
                     try {
                         session_.Pack(q, *slot);
@@ -253,7 +275,8 @@ private:
                 status.resize(f.ks.size());
                 try {
                     session_.Complete(*f.slot, scores.data(), ids.data(), status.data());
-                    Finish(f, scores, ids, BM25MI_OK, std::string());
+                    Finish(f, scores, ids, &status, BM25MI_OK, std::string());
+                    work_.notify_one();
                 } catch (const std::exception &e) {
                     Fail(f, e.what());
                 }
@@ -271,7 +294,7 @@ private:
             failed_ = true; // a device failure ends the session (large_search.cpp:208-210)
         }
         session_.failed = true;
-        Finish(f, {}, {}, BM25MI_ERR_INNER, what);
+        Finish(f, {}, {}, nullptr, BM25MI_ERR_INNER, what);
         done_.notify_all();
     }
 
@@ -283,7 +306,7 @@ private:
     std::unordered_map<uint64_t, std::unique_ptr<Request>> requests_;
     uint64_t next_ = 1;
     bool stop_ = false, failed_ = false;
-    int64_t batches_ = 0, queries_ = 0;
+    int64_t batches_ = 0, queries_ = 0, fallbacks_ = 0;
     int32_t maxInflight_ = 0;
     std::thread worker_;
 };
@@ -320,6 +343,15 @@ extern "C" int bm25mi_server_submit(bm25mi_server *srv, const int32_t *terms, in
     });
 }
 
+extern "C" int bm25mi_server_submit_pruned(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k,
+                                           float threshold, uint64_t *ticket)
+{
+    return Invoke([&] {
+        Require(srv && srv->impl && ticket, "Null server or ticket");
+        *ticket = srv->impl->Submit(terms, n, k, threshold);
+    });
+}
+
 extern "C" int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *scores, int32_t *ids,
                                   double *latency_us)
 {
@@ -334,8 +366,9 @@ extern "C" int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *sc
 // second one. latency_us[i] = submit to completion; scores/ids (optional) receive the rows
 // of request i at offset sum_{j<i} ks[j].
 extern "C" int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offsets, const int32_t *terms,
-                                         int32_t n, const int32_t *ks, double qps, uint64_t seed,
-                                         double *latency_us, float *scores, int32_t *ids, double *elapsed_s)
+                                         int32_t n, const int32_t *ks, const float *thresholds, double qps,
+                                         uint64_t seed, double *latency_us, float *scores, int32_t *ids,
+                                         double *elapsed_s)
 {
     return Invoke([&] {
         Require(srv && srv->impl && offsets && ks && latency_us && n >= 1 && qps > 0, "Invalid Poisson run");
@@ -373,7 +406,7 @@ extern "C" int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offs
                 std::this_thread::yield();
             try {
                 tickets[i] = srv->impl->Submit(terms + offsets[i], static_cast<int32_t>(offsets[i + 1] - offsets[i]),
-                                               ks[i]);
+                                               ks[i], thresholds ? thresholds[i] : 0.0F);
             } catch (const std::exception &e) {
                 std::lock_guard<std::mutex> g(failMutex);
                 if (failure.empty())
@@ -389,11 +422,12 @@ extern "C" int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offs
     });
 }
 
-extern "C" int bm25mi_server_counters(bm25mi_server *srv, int64_t *batches, int64_t *queries, int32_t *max_inflight)
+extern "C" int bm25mi_server_counters(bm25mi_server *srv, int64_t *batches, int64_t *queries, int32_t *max_inflight,
+                                      int64_t *fallbacks)
 {
     return Invoke([&] {
         Require(srv && srv->impl, "Null server");
-        srv->impl->Counters(batches, queries, max_inflight);
+        srv->impl->Counters(batches, queries, max_inflight, fallbacks);
     });
 }
 

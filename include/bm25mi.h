@@ -177,17 +177,25 @@ typedef struct bm25mi_server bm25mi_server;
 int bm25mi_server_start(bm25mi_session *s, int32_t max_batch, bm25mi_server **out);
 /* Queues one query (n term IDs, top-K); *ticket identifies it for bm25mi_server_wait. */
 int bm25mi_server_submit(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, uint64_t *ticket);
+/* Same with a pruning threshold (a predicted K-th score, as bm25mi_search_pruned): a query
+ * that keeps fewer than K docs at or above it is rerun as a plain query, so the result is
+ * the same. */
+int bm25mi_server_submit_pruned(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, float threshold,
+                                uint64_t *ticket);
 /* Blocks until the query is done; writes its K scores and doc IDs (either may be NULL) and
  * the submit-to-completion latency in microseconds (may be NULL). */
 int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *scores, int32_t *ids, double *latency_us);
-/* Open-loop load: query i (terms offsets[i]..offsets[i+1], K ks[i]) is submitted at the i-th
- * arrival of a seeded Poisson process of rate qps; latency_us[i] gets its latency, scores /
- * ids (optional) its rows at offset sum_{j<i} ks[j], *elapsed_s the wall time of the run. */
+/* Open-loop load: query i (terms offsets[i]..offsets[i+1], K ks[i], threshold thresholds[i]
+ * when thresholds is not NULL) is submitted at the i-th arrival of a seeded Poisson process
+ * of rate qps; latency_us[i] gets its latency, scores / ids (optional) its rows at offset
+ * sum_{j<i} ks[j], *elapsed_s the wall time of the run. */
 int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offsets, const int32_t *terms, int32_t n,
-                              const int32_t *ks, double qps, uint64_t seed, double *latency_us, float *scores,
-                              int32_t *ids, double *elapsed_s);
-/* Batches launched, queries served and the largest number of batches in flight so far. */
-int bm25mi_server_counters(bm25mi_server *srv, int64_t *batches, int64_t *queries, int32_t *max_inflight);
+                              const int32_t *ks, const float *thresholds, double qps, uint64_t seed,
+                              double *latency_us, float *scores, int32_t *ids, double *elapsed_s);
+/* Batches launched, queries served (reruns included), the largest number of batches in flight
+ * and the pruned queries rerun as plain ones, so far. Any output may be NULL. */
+int bm25mi_server_counters(bm25mi_server *srv, int64_t *batches, int64_t *queries, int32_t *max_inflight,
+                           int64_t *fallbacks);
 /* Finishes the submitted queries and stops; NULL is a no-op. */
 int bm25mi_server_stop(bm25mi_server *srv);
 

@@ -103,3 +103,22 @@ def test_pool_admission_splits_batches_and_fences_session(corpus, need_gpu):
         got_s, got_i, _ = small.search([queries[0]], 1000)  # usable again after the server stops
         assert np.array_equal(got_i.ravel(), want_i[:1000])
         small.close()
+
+
+def test_pruned_requests_fall_back_and_match_plain(corpus, need_gpu):
+    _, index = corpus
+    queries, ks = workload(corpus, 96, seed=31)
+    with bm25mi.Session(index, max_queries=32, max_words=32 * 32, max_k=1000, slots=3) as s:
+        want_s, want_i = alone(s, queries, ks)
+        ends = np.cumsum(ks) - 1
+        kth = want_s[ends]
+        # Every third threshold sits above the true K-th score (forcing a plain rerun), the
+        # others below it.
+        factor = np.where(np.arange(len(ks)) % 3 == 0, 1.05, 0.6).astype(np.float32)
+        thr = (kth * factor).astype(np.float32)
+        with bm25mi.Server(s, max_batch=32) as srv:
+            r = srv.run_poisson(queries, ks, qps=1e6, seed=9, thresholds=thr)
+            assert np.array_equal(r['ids'], want_i)
+            assert np.array_equal(r['scores'].view(np.uint32), want_s.view(np.uint32))
+            c = srv.counters()
+            assert c['fallbacks'] >= int(((factor > 1) & (kth > 0)).sum())
