@@ -1,14 +1,15 @@
 """BM25 full-Search benchmark on one MI355X.
 
-Workload (default): the 10,000,000-doc synthetic corpus (vocabulary 1,000,000,
-Zipf 1.07, lognormal lengths of mean 56, k1 1.5, b 0.75, seed 0), Q=8 queries of
-3 distinct df-weighted terms, K=10. One step is one full Search of one Q8 batch:
-host term IDs in, host top-K (doc IDs + FP32 scores) out, index resident in HBM.
-Steps alternate between two fixed batches A and B, like the reference's A/B
-protocol (FR/test/ascendhost/bm25/benchmark_common.h:51-107).
+Headline workload: the 10,000,000-doc synthetic corpus (vocabulary 1,000,000, Zipf 1.07,
+lognormal lengths of mean 56, k1 1.5, b 0.75, seed 0), Q=8 queries of 3 distinct df-weighted
+terms, K=10. One step is one full Search of one Q8 batch: host term IDs in, host top-K (doc IDs +
+FP32 scores) out, index resident in HBM. Steps alternate between two fixed batches A and B, like
+the reference's A/B protocol (FR/test/ascendhost/bm25/benchmark_common.h:51-107).
 
-Prints one JSON line: QPS (value), ms per Search, the scoring kernel's HBM
-roofline fraction, the CPU oracle timed on this host's cores, and parity.
+Prints one JSON line: QPS (value), ms per Search, the scoring kernel's HBM roofline fraction,
+the reference's own CPU Search timed on this host's cores, parity against the CPU oracle, and
+(unless disabled) the learned-threshold path against the plain path and the serving loop under
+Poisson load. --batched adds the uneven-batch runs on the 8.84M-passage corpus.
 """
 
 import argparse
@@ -25,6 +26,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT), str(ROOT / 'bm25-npu_amd')]
 
 import bm25mi  # noqa: E402
+from bm25mi import learned  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -36,6 +38,12 @@ def log(*a):
 def nearest_rank(values, p):
     v = sorted(values)
     return v[max(0, int(np.ceil(p * len(v))) - 1)]
+
+
+def summarize(ms, queries):
+    """median / nearest-rank P95 / QPS = Q*1000/mean (standalone/final_support.py:78-90)."""
+    return {'ms_median': statistics.median(ms), 'ms_p95': nearest_rank(ms, 0.95), 'ms_mean': statistics.mean(ms),
+            'qps': queries * 1e3 / statistics.mean(ms), 'samples': len(ms)}
 
 
 def host_cores():
@@ -75,6 +83,213 @@ def measured_traffic(kernel, args):
     return rec['traffic_bytes_per_launch'], rec['source']
 
 
+def ab_protocol(call, batches, warmup=5, cycles=30):
+    """The reference's measurement protocol: warm-up A/B cycles, then timed A/B cycles
+    (each cycle calls A then B, the next B then A). Returns per-call ms and last results."""
+    out = {}
+    for c in range(warmup):
+        for i in ((0, 1) if c % 2 == 0 else (1, 0)):
+            call(batches[i])
+    ms = []
+    for c in range(cycles):
+        for i in ((0, 1) if c % 2 == 0 else (1, 0)):
+            t = time.perf_counter()
+            out[i] = call(batches[i])
+            ms.append((time.perf_counter() - t) * 1e3)
+    return ms, out
+
+
+def same(a, b):
+    """Mismatching entries between two (scores, ids) results: doc IDs or FP32 score bits."""
+    return int((np.asarray(a[1]) != np.asarray(b[1])).sum()) + int(
+        (np.asarray(a[0], np.float32).view(np.uint32) != np.asarray(b[0], np.float32).view(np.uint32)).sum())
+
+
+def cpu_legs(args, index, packed, gpu_results):
+    from oracle import oracle
+    cores = host_cores()
+    threads = min(32, cores)
+    out = {'host': {'cores': cores, 'model': cpu_model()}}
+    # The oracle (restated reference CPU Search at this index's k1): parity and its timing.
+    port_ms, port_res = ab_protocol(
+        lambda b: oracle.search(index, b[0], b[1], args.k, shards=threads, threads=threads, chunk=args.queries),
+        packed, warmup=2, cycles=args.cpu_cycles)
+    out['port'] = summarize(port_ms, args.queries)
+    mismatches = sum(same(gpu_results[i], port_res[i]) for i in (0, 1))
+    out['parity'] = {'mismatches_vs_oracle': mismatches, 'checked_batches': 2,
+                     'rule': 'doc IDs and FP32 score bits equal'}
+    # The reference's own CPU Search (compiled from its sources into oracle/_ref): both variants.
+    if oracle.ref_available() and args.k <= 32:
+        for variant in ('plain', 'directory'):
+            ref = oracle.RefSession(index, args.k, threads, variant=variant, max_queries=args.queries,
+                                    max_words=args.queries * args.terms)
+            ms, _ = ab_protocol(lambda b: ref.search(b[0], b[1]), packed, warmup=5, cycles=args.cpu_cycles)
+            ref.close()
+            out[f'reference_{variant}'] = summarize(ms, args.queries)
+    # The bm25s-numba retrieve algorithm restated (dense row per query + heap top-k).
+    impacts = index.impacts()
+    best = None
+    for workers in sorted({w for w in (1, 8, 32) if w <= cores} | {min(8, cores)}):
+        ms, _ = ab_protocol(lambda b: oracle.bm25s_search(index, impacts, b[0], b[1], args.k, threads=workers),
+                            packed, warmup=1, cycles=max(2, args.cpu_cycles // 6))
+        r = summarize(ms, args.queries)
+        r['workers'] = workers
+        out.setdefault('bm25s_restated', []).append(r)
+        best = r if best is None or r['ms_mean'] < best['ms_mean'] else best
+    del impacts
+    ref = min((out[k] for k in ('reference_plain', 'reference_directory') if k in out),
+              key=lambda r: r['ms_mean'], default=None)
+    if ref is not None:
+        kind, base = 'reference', ref
+        sample = (f'{args.cpu_cycles} A/B cycles ({2 * args.cpu_cycles} Q{args.queries} Searches) after 5 warm-up '
+                  f'cycles of the same workload; the reference CpuSearchSession / DirectoryCpuSearchSession '
+                  f'(faster one) compiled from its sources (oracle/_ref), {threads} OpenMP threads = shards, '
+                  f'{cpu_model()}; its arithmetic uses the reference constant k1=1.2, so only its timing is '
+                  f'compared; parity is checked against oracle/bm25_oracle.c')
+    else:
+        kind, base = 'port', out['port']
+        sample = (f'{args.cpu_cycles} A/B cycles of Q{args.queries} Searches of the same workload; '
+                  f'oracle/bm25_oracle.c (restated reference CPU Search), {threads} threads, {cpu_model()}')
+    cpu_baseline = {'value': base['qps'], 'unit': 'queries/s', 'cores': threads, 'kind': kind,
+                    'ms_median': base['ms_median'], 'sample': sample}
+    return cpu_baseline, out
+
+
+def learned_leg(args, index, offsets):
+    """Trains the K-th-score MLP on the GPU from exact plain-path outputs and compares the
+    pruned path with the plain path on held-out batches."""
+    t0 = time.time()
+    s = bm25mi.Session(index, max_queries=64, max_words=64 * 12, max_k=100, block_docs=args.block_docs)
+    qs, ks = learned.make_queries(offsets, args.train_queries, seed=100)
+    data = learned.collect(s, qs, ks, batch=64)
+    t_collect = time.time() - t0
+    t1 = time.time()
+    model = learned.ThresholdModel(seed=0).fit(data, epochs=args.train_epochs, batch=256, lr=2e-3, seed=0)
+    t_fit = time.time() - t1
+    vq, vk = learned.make_queries(offsets, 2048, seed=200)
+    vdata = learned.collect(s, vq, vk, batch=64)
+    model.calibrate(vdata, fallback_rate=args.fallback_target)
+    out = {'train_pairs': int(len(data.kth)), 'train_terms': '2-12 df-weighted', 'train_k': [10, 100],
+           'collect_s': round(t_collect, 2), 'fit_s': round(t_fit, 2), 'epochs': args.train_epochs,
+           'loss_first_last': [float(model.losses[0]), float(model.losses[-1])], 'safety': model.safety,
+           'cases': []}
+
+    def run_case(name, batches, k):
+        thr = []
+        for b in batches:
+            feats = s.features(b, k)
+            lower, upper = s.bounds(b, k)
+            thr.append(model.thresholds(feats, lower, upper))
+        plain_ms, plain_res = ab_protocol(lambda i: s.search(batches[i], k), [0, 1], warmup=3, cycles=args.cycles)
+        prune_ms, prune_res = ab_protocol(lambda i: s.search(batches[i], k, thresholds=thr[i]), [0, 1], warmup=3,
+                                          cycles=args.cycles)
+        mism = sum(same(plain_res[i][:2], prune_res[i][:2]) for i in (0, 1))
+        fb = sum(int(prune_res[i][2].sum()) for i in (0, 1))
+        scan = {'plain': 0, 'learned': 0}
+        dev = {'plain': [], 'learned': []}
+        for i in (0, 1):
+            s.search(batches[i], k, timed=True)
+            st = s.stats()
+            scan['plain'] += st['postings_scanned']
+            dev['plain'].append(st['device_ms'])
+            s.search(batches[i], k, thresholds=thr[i], timed=True)
+            st = s.stats()
+            scan['learned'] += st['postings_scanned']
+            dev['learned'].append(st['device_ms'])
+        nq = len(batches[0])
+        p, lr = summarize(plain_ms, nq), summarize(prune_ms, nq)
+        out['cases'].append({
+            'case': name, 'queries': nq, 'k': k, 'plain_ms_median': p['ms_median'],
+            'learned_ms_median': lr['ms_median'], 'speedup': p['ms_median'] / lr['ms_median'],
+            'plain_qps': p['qps'], 'learned_qps': lr['qps'],
+            'plain_device_ms': statistics.mean(dev['plain']), 'learned_device_ms': statistics.mean(dev['learned']),
+            'mismatches': mism, 'fallback_rate': fb / (2 * nq),
+            'postings_plain': scan['plain'], 'postings_learned': scan['learned'],
+            'bytes_skipped_frac': 1 - scan['learned'] / max(1, scan['plain'])})
+
+    run_case('Q8x3 K10 (headline batches)', [bm25mi.draw_queries(offsets, 8, 3, seed=1),
+                                            bm25mi.draw_queries(offsets, 8, 3, seed=2)], 10)
+    for nq in (8, 64):
+        for k in (10, 100):
+            batches = [learned.make_queries(offsets, nq, seed=900 + 10 * nq + k + j, k_choices=(k,))[0]
+                       for j in (0, 1)]
+            run_case(f'Q{nq} x 2-12 terms K{k} (held out)', batches, k)
+    s.close()
+    return out, model
+
+
+def serving_leg(args, index, offsets, model):
+    """Poisson arrivals through the continuous-batching server, plain and learned side by side."""
+    s = bm25mi.Session(index, max_queries=256, max_words=256 * 32, max_k=1000, slots=4,
+                       pool_entries=args.pool_entries, block_docs=args.block_docs)
+    out = {'slots': 4, 'max_batch': 256, 'pool_entries_per_slot': s.config.pool_entries or 'full',
+           'terms': 'uniform 1-32, df-weighted', 'k_mix': {'10': 0.7, '100': 0.25, '1000': 0.05}, 'runs': []}
+    rng = np.random.default_rng(55)
+    for qps in args.serving_qps:
+        n = int(min(args.serving_requests, max(500, qps * args.serving_seconds)))
+        lengths = rng.integers(1, 33, size=n)
+        ks = rng.choice(np.array([10, 100, 1000], np.int32), size=n, p=[0.7, 0.25, 0.05]).astype(np.int32)
+        queries = bm25mi.draw_queries(offsets, n, lengths, seed=int(qps))
+        thr = None
+        if model is not None:
+            feats = np.concatenate([s.features(queries[i:i + 256], ks[i:i + 256]) for i in range(0, n, 256)])
+            bounds = [s.bounds(queries[i:i + 256], ks[i:i + 256]) for i in range(0, n, 256)]
+            lower = np.concatenate([b[0] for b in bounds])
+            upper = np.concatenate([b[1] for b in bounds])
+            thr = model.thresholds(feats, lower, upper)
+        for mode in (['plain', 'learned'] if thr is not None else ['plain']):
+            with bm25mi.Server(s, max_batch=256) as srv:
+                r = srv.run_poisson(queries, ks, qps=qps, seed=int(qps) + 1, keep_results=(mode == 'learned'),
+                                    thresholds=thr if mode == 'learned' else None)
+                c = srv.counters()
+            rec = {'mode': mode, 'offered_qps': qps, 'requests': n, 'sustained_qps': r['sustained_qps'],
+                   'p50_ms': r['p50_ms'], 'p95_ms': r['p95_ms'], 'p99_ms': r['p99_ms'], 'batches': c['batches'],
+                   'mean_batch': c['queries'] / max(1, c['batches']), 'max_inflight': c['max_inflight'],
+                   'fallbacks': c['fallbacks']}
+            if mode == 'learned':
+                # Spot-check results against the plain path on the first requests.
+                rows = np.concatenate([[0], np.cumsum(ks)])
+                mism = 0
+                for i in range(0, min(n, 64)):
+                    ws, wi, _ = s.search([queries[i]], int(ks[i]))
+                    mism += same((r['scores'][rows[i]:rows[i + 1]], r['ids'][rows[i]:rows[i + 1]]),
+                                 (ws.ravel(), wi.ravel()))
+                rec['mismatches_first_64'] = mism
+            out['runs'].append(rec)
+            log(f'serving {mode} {qps} QPS: sustained {r["sustained_qps"]:.0f}, p50 {r["p50_ms"]:.2f} ms, '
+                f'p99 {r["p99_ms"]:.2f} ms')
+    s.close()
+    return out
+
+
+def batched_leg(args):
+    """Uneven batches on the 8.84M-passage corpus: Q in {1, 8, 64, 256}, geometric 1-32 terms
+    (mean 6), K drawn from {10, 100, 1000}; parity against the oracle for one batch per Q."""
+    from oracle import oracle
+    corpus = bm25mi.generate('passages8p8m')
+    index = corpus.index()
+    offsets = corpus.offsets
+    del corpus
+    s = bm25mi.Session(index, max_queries=256, max_words=256 * 32, max_k=1000, block_docs=args.block_docs)
+    out = {'corpus': 'passages8p8m', 'docs': index.num_docs, 'postings': index.postings, 'runs': []}
+    rng = np.random.default_rng(77)
+    for nq in (1, 8, 64, 256):
+        batches = []
+        for j in (0, 1):
+            lengths = bm25mi.corpus.geometric_lengths(nq, 6, 1, 32, seed=1000 * nq + j)
+            ks = rng.choice(np.array([10, 100, 1000], np.int32), size=nq).astype(np.int32)
+            batches.append((bm25mi.draw_queries(offsets, nq, lengths, seed=2000 * nq + j), ks))
+        ms, res = ab_protocol(lambda i: s.search(batches[i][0], batches[i][1]), [0, 1], warmup=3,
+                              cycles=args.cycles)
+        want = oracle.search(index, *bm25mi.pack(batches[0][0]), batches[0][1], threads=host_cores())
+        r = summarize(ms, nq)
+        r.update({'queries': nq, 'mismatches_vs_oracle': same(res[0][:2], want)})
+        out['runs'].append(r)
+        log(f'batched Q{nq}: median {r["ms_median"]:.3f} ms, {r["qps"]:.0f} QPS')
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -85,10 +300,22 @@ def main():
     ap.add_argument('--terms', type=int, default=3)
     ap.add_argument('--k', type=int, default=10)
     ap.add_argument('--block-docs', type=int, default=0, help='docs per scoring task (0 = library default)')
-    ap.add_argument('--grid', type=int, default=0, help='persistent scoring workgroups (0 = fill the GPU)')
-    ap.add_argument('--cpu-calls', type=int, default=10, help='timed CPU oracle calls (A/B alternating)')
+    ap.add_argument('--grid', type=int, default=0, help='persistent scoring waves (0 = fill the GPU)')
+    ap.add_argument('--task-cost', type=int, default=0, help='per-task cost of the wave schedule (0 = default)')
+    ap.add_argument('--cpu-cycles', type=int, default=30, help='timed A/B cycles of each CPU baseline')
+    ap.add_argument('--cycles', type=int, default=30, help='timed A/B cycles of the learned and batched legs')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: no CPU leg')
+    ap.add_argument('--no-learned', action='store_true')
+    ap.add_argument('--no-serving', action='store_true')
+    ap.add_argument('--batched', action='store_true', help='add the 8.84M-passage uneven-batch runs')
+    ap.add_argument('--train-queries', type=int, default=65536, help='exact (query, K-th score) training pairs')
+    ap.add_argument('--train-epochs', type=int, default=40)
+    ap.add_argument('--fallback-target', type=float, default=0.02)
+    ap.add_argument('--serving-qps', type=float, nargs='*', default=[2000.0, 10000.0, 40000.0])
+    ap.add_argument('--serving-requests', type=int, default=6000)
+    ap.add_argument('--serving-seconds', type=float, default=1.0)
+    ap.add_argument('--pool-entries', type=int, default=1 << 26, help='candidate-pool entries per serving slot')
+    ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: headline leg only')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)',
@@ -98,15 +325,16 @@ def main():
     t0 = time.time()
     corpus = bm25mi.generate(args.corpus)
     index = corpus.index()
+    offsets = corpus.offsets
     log(f'corpus {args.corpus}: N={index.num_docs} V={index.vocab} M={index.postings} ({time.time() - t0:.1f}s)')
-    qa = bm25mi.draw_queries(corpus.offsets, args.queries, args.terms, seed=1)
-    qb = bm25mi.draw_queries(corpus.offsets, args.queries, args.terms, seed=2)
+    qa = bm25mi.draw_queries(offsets, args.queries, args.terms, seed=1)
+    qb = bm25mi.draw_queries(offsets, args.queries, args.terms, seed=2)
     packed = [bm25mi.pack(qa), bm25mi.pack(qb)]
     del corpus
 
     t0 = time.time()
     session = bm25mi.Session(index, max_queries=args.queries, max_words=args.queries * args.terms, max_k=args.k,
-                             block_docs=args.block_docs, grid_blocks=args.grid)
+                             block_docs=args.block_docs, grid_blocks=args.grid, task_cost=args.task_cost)
     log(f'session create {time.time() - t0:.1f}s')
 
     results = {}
@@ -163,7 +391,7 @@ def main():
         'config': {'workload': f'{args.corpus} full Search, Q{args.queries}x{args.terms} terms, K{args.k}',
                    'docs': index.num_docs, 'vocab': index.vocab, 'postings': index.postings,
                    'queries': args.queries, 'terms_per_query': args.terms, 'k': args.k,
-                   'block_docs': args.block_docs, 'k1': index.k1, 'b': index.b},
+                   'block_docs': st['block_docs'], 'k1': index.k1, 'b': index.b},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'traffic_source': traffic_note,
                      'kernel': f'KScore<{st["block_docs"]}>', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg,
@@ -182,33 +410,29 @@ def main():
         },
         'device': {'compute_units': cus.value, 'lds_per_block': lds_block.value, 'lds_per_cu': lds_cu.value},
     }
+    session.close()
 
     if not args.profile and not args.no_cpu:
-        from oracle import oracle
-        cores = host_cores()
-        threads = min(32, cores)
-        mismatches = 0
-        cpu_ms = []
-        for call in range(args.cpu_calls + 2):
-            offs, terms = packed[call % 2]
-            t = time.perf_counter()
-            want_s, want_i = oracle.search(index, offs, terms, args.k, shards=threads, threads=threads,
-                                           chunk=args.queries)
-            dt = (time.perf_counter() - t) * 1e3
-            if call >= 2:
-                cpu_ms.append(dt)
-            got_s, got_i = results[call % 2]
-            mismatches += int((got_i != want_i).sum()) + int((got_s.view(np.uint32) != want_s.view(np.uint32)).sum())
-        record['cpu_baseline'] = {
-            'value': args.queries * 1e3 / statistics.mean(cpu_ms), 'unit': 'queries/s', 'cores': threads,
-            'kind': 'port', 'ms_median': statistics.median(cpu_ms),
-            'sample': f'{args.cpu_calls} A/B-alternating Q{args.queries} Searches of the same workload after 2 '
-                      f'warm-up calls; oracle/bm25_oracle.c (restated reference CPU Search), '
-                      f'{threads} OpenMP threads = shards, {cpu_model()}'}
-        record['parity'] = {'mismatches_vs_oracle': mismatches, 'checked_batches': 2,
-                            'rule': 'doc IDs and FP32 score bits equal'}
+        t0 = time.time()
+        record['cpu_baseline'], cpu = cpu_legs(args, index, packed, results)
+        record['parity'] = cpu.pop('parity')
+        record['cpu_baselines'] = cpu
+        log(f'CPU legs {time.time() - t0:.1f}s')
+    model = None
+    if not args.profile and not args.no_learned:
+        t0 = time.time()
+        record['learned'], model = learned_leg(args, index, offsets)
+        log(f'learned leg {time.time() - t0:.1f}s')
+    if not args.profile and not args.no_serving:
+        t0 = time.time()
+        record['serving'] = serving_leg(args, index, offsets, model)
+        log(f'serving leg {time.time() - t0:.1f}s')
+    if args.batched and not args.profile:
+        del index
+        t0 = time.time()
+        record['batched'] = batched_leg(args)
+        log(f'batched leg {time.time() - t0:.1f}s')
     print(json.dumps(record), flush=True)
-    session.close()
 
 
 if __name__ == '__main__':

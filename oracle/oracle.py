@@ -60,6 +60,12 @@ def ref_lib():
         _ref = C.CDLL(str(REF_LIB))
         _ref.ref_cpu_search.restype = C.c_int
         _ref.ref_cpu_search.argtypes = [P, i64, P, P, i64, P, i64, P, P, P, i32, i32, i32, i32, P, P, P]
+        _ref.ref_cpu_create.restype = P
+        _ref.ref_cpu_create.argtypes = [P, i64, P, P, i64, P, i64, P, i32, i32, i32, i32, i32, i64]
+        _ref.ref_cpu_run.restype = C.c_int
+        _ref.ref_cpu_run.argtypes = [P, P, P, i32, P, P]
+        _ref.ref_cpu_destroy.restype = None
+        _ref.ref_cpu_destroy.argtypes = [P]
     return _ref
 
 
@@ -133,3 +139,35 @@ def ref_search(index, offsets, terms, k, threads=4, shards=None, full=False):
     if full:
         return scores, ids, fulls.reshape(nq, index.num_docs)
     return scores, ids
+
+
+class RefSession:
+    """The reference's own CPU Search session built once (variant 'plain' = CpuSearchSession,
+    'directory' = DirectoryCpuSearchSession), for timing. Its arithmetic uses the reference's
+    constant k1 = 1.2 (2.2F) whatever the index was built with."""
+
+    def __init__(self, index, k, threads, shards=None, variant='plain', max_queries=8, max_words=24):
+        shards = threads if shards is None else shards
+        self._h = ref_lib().ref_cpu_create(_p(index.offsets), index.vocab, _p(index.docs), _p(index.tf),
+                                           index.postings, _p(index.norm), index.num_docs, _p(index.idf),
+                                           0 if variant == 'plain' else 1, threads, shards, k, max_queries,
+                                           max_words)
+        if not self._h:
+            raise ValueError('reference CPU session rejected its configuration')
+        self.k = k
+
+    def search(self, offsets, terms):
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        terms = np.ascontiguousarray(terms, np.int32)
+        nq = len(offsets) - 1
+        scores = np.empty((nq, self.k), np.float32)
+        ids = np.empty((nq, self.k), np.int32)
+        if ref_lib().ref_cpu_run(self._h, _p(offsets), _p(terms if terms.size else np.zeros(1, np.int32)), nq,
+                                 _p(scores), _p(ids)) != 0:
+            raise ValueError('reference CPU Search rejected its input')
+        return scores, ids
+
+    def close(self):
+        if self._h:
+            ref_lib().ref_cpu_destroy(self._h)
+            self._h = None
