@@ -95,6 +95,7 @@ SIGNATURES = {
     'bm25mi_server_run_poisson': (C.c_int, [P, P, P, i32, P, P, f64, C.c_uint64, P, P, P, P]),
     'bm25mi_server_counters': (C.c_int, [P, P, P, P, P]),
     'bm25mi_server_stop': (C.c_int, [P]),
+    'bm25mi_set_launch': (C.c_int, [P, i32, i32]),
     'bm25mi_last_stats': (C.c_int, [P, P]),
     'bm25mi_destroy': (C.c_int, [P]),
     'bm25mi_last_error': (i64, [P, C.c_char_p, C.c_size_t]),

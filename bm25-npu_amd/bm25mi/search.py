@@ -135,6 +135,10 @@ class Session:
                                                  _lib.ptr(lower), _lib.ptr(upper)), self.handle)
         return lower, upper
 
+    def set_launch(self, grid_waves=0, task_cost=0):
+        """Re-plans the scoring launch (waves, schedule task cost); 0 keeps a value."""
+        _lib.check(self._lib.bm25mi_set_launch(self.handle, grid_waves, task_cost), self.handle)
+
     def stats(self):
         st = _lib.Stats()
         _lib.check(self._lib.bm25mi_last_stats(self.handle, C.byref(st)), self.handle)

@@ -69,6 +69,7 @@ public:
     bm25mi_config cfg{};
     int64_t ndocs = 0, vocab = 0, postings = 0;
     int32_t blockDocs = 2048, granule = kGranule, blocks = 0, grid = 0, cus = 0;
+    int32_t gridCapacity = 0; // the packet and counter rows hold this many scoring waves
     int64_t poolEntries = 0; // candidate-pool entries of each slot
     int64_t taskCost = kTaskCost;
     std::vector<int64_t> toffHost;
@@ -120,6 +121,8 @@ public:
     void Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *status);
     // Pack + Launch + Complete on slot 0; any device failure marks the session failed.
     void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
+    // Changes the scoring launch (waves, schedule task cost) and recaptures the slots' graphs.
+    void SetLaunch(int32_t waves, int64_t cost);
 
 private:
     void InitSlot(Slot &slot);

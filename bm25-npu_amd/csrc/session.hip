@@ -106,6 +106,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     const int perCu = ScoreBlocksPerCu(blockDocs);
     Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
+    Require(grid >= 1 && grid <= (1 << 20), "grid_blocks out of range");
+    gridCapacity = std::max(grid, 2 * cus * perCu);
     Require(static_cast<int64_t>(cfg.max_queries) * blocks < (int64_t(1) << 32), "Too many scoring tasks per Search");
 
     // Candidate pools: by default room for max_queries queries of K = max_k in every slot;
@@ -174,7 +176,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     offTerms = at;
     at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 16, 16);
     offWave = at;
-    at = Align(at + (static_cast<size_t>(grid) + 1) * 4, 256);
+    at = Align(at + (static_cast<size_t>(gridCapacity) + 1) * 4, 256);
     packetBytes = at;
     // Result block layout: per final-workgroup counter rows, status, scores, IDs.
     resStatus = q * kStats * 8;
@@ -204,7 +206,7 @@ void Session::InitSlot(Slot &slot)
     slot.dCount = Alloc<uint32_t>(q, "alloc pool counts");
     slot.dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
-    slot.dStats = Alloc<unsigned long long>(static_cast<size_t>(grid) * kStats, "alloc stats");
+    slot.dStats = Alloc<unsigned long long>(static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
     auto *pk = static_cast<unsigned char *>(slot.dPacket);
     slot.bt = BatchView{reinterpret_cast<const int32_t *>(pk),
                         reinterpret_cast<const int64_t *>(pk + offQoff),
@@ -440,6 +442,26 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
     } catch (...) {
         failed = true;
         throw;
+    }
+}
+
+void Session::SetLaunch(int32_t waves, int64_t cost)
+{
+    Require(waves >= 1 && waves <= gridCapacity, "waves out of range for this session");
+    Require(cost >= 1, "task cost must be >= 1");
+    for (auto &slot : slots)
+        BM25MI_HIP(hipStreamSynchronize(slot.stream), "drain before relaunch");
+    grid = waves;
+    taskCost = cost;
+    for (auto &slot : slots) {
+        if (slot.exec)
+            BM25MI_HIP(hipGraphExecDestroy(slot.exec), "destroy graph exec");
+        if (slot.graph)
+            BM25MI_HIP(hipGraphDestroy(slot.graph), "destroy graph");
+        slot.exec = nullptr;
+        slot.graph = nullptr;
+        if (cfg.use_graph)
+            Capture(slot);
     }
 }
 
@@ -741,6 +763,26 @@ extern "C" int bm25mi_query_bounds(bm25mi_session *s, const int64_t *query_offse
                             ub += ss.tmaxHost[t[w]];
                     out_upper[q] = ub;
                 }
+            }
+        },
+        &s->error);
+}
+
+extern "C" int bm25mi_set_launch(bm25mi_session *s, int32_t grid_waves, int32_t task_cost)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            Require(s->impl && !s->impl->closed && !s->impl->failed, "BM25 session closed/failed");
+            Require(!s->impl->serving, "Stop the session's server before changing its launch");
+            Session &ss = *s->impl;
+            try {
+                ss.SetLaunch(grid_waves > 0 ? grid_waves : ss.grid, task_cost > 0 ? task_cost : ss.taskCost);
+            } catch (const DeviceError &) {
+                ss.failed = true;
+                throw;
             }
         },
         &s->error);

@@ -151,6 +151,12 @@ int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const
 int bm25mi_query_bounds(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                         int32_t queries, const int32_t *k_per_query, float *out_lower, float *out_upper);
 
+/* Changes the scoring launch of a session without rebuilding it (for the autotuner):
+ * grid_waves persistent scoring waves (at most twice the waves that fit the GPU at once, or
+ * the configured grid_blocks) and the schedule's per-task cost; 0 keeps the current value.
+ * Results do not depend on either. */
+int bm25mi_set_launch(bm25mi_session *s, int32_t grid_waves, int32_t task_cost);
+
 int bm25mi_last_stats(const bm25mi_session *s, bm25mi_stats *out);
 
 /* Close: idempotent; frees device memory. The handle is freed on success. */
