@@ -14,7 +14,7 @@ constexpr int kLongMin = 4096;             // lists this long use the granule di
 constexpr int kSampleShift = 4;            // samples[i] = doc ID of posting 16 * i (global CSC index)
 constexpr int kFinalThreads = 1024;        // selection workgroup of the final kernel
 constexpr int kSortCap = 16384;            // pool entries the final kernel sorts in LDS (128 KB)
-constexpr int kTaskCost = 4096;            // estimated fixed cost of a scoring task, in postings
+constexpr int kTaskCost = 2048;            // estimated fixed cost of a scoring task, in postings
 #ifndef BM25MI_COLUMNS
 #define BM25MI_COLUMNS 12
 #endif

@@ -146,6 +146,7 @@ __device__ __forceinline__ uint32_t HistEdge(uint32_t bucket)
 }
 
 constexpr int kHistPerLane = (kHistBins + 63) / 64;
+constexpr int kShortSlots = 4; // short lists per query with cached doc chunks in the scoring kernel
 
 // Lower bound of a query's K-th score from its kept-candidate histogram, computed by one
 // wave from the bins it holds (lane l: bins l*kHistPerLane ...): the edge of the highest
@@ -221,170 +222,101 @@ __device__ __forceinline__ uint32_t WaveKthLargest(uint32_t v, uint32_t k)
 
 constexpr int Log2(int x) { return x <= 1 ? 0 : 1 + Log2(x / 2); }
 
-// The rare part of a task's selection, kept out of line so that its registers do not
-// constrain the streaming loop: cut the kept docs to BlockKeepCap(K) when there are more,
-// append them to the query's pool as composite keys, count them in the score histogram
-// and raise tau by the cut. kept/mine are the wave and lane counts of docs scoring at least
-// floor, mx the lane's largest score bits.
-struct SelectResult {
-    uint32_t kept, cut, overflow;
-};
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const U32x4 LdsUint4; // 16-byte LDS words
 
+// Keys (score bits << log2(B) | (B-1-e)) of accumulator entries e at least (xb, xe), counted
+// by each lane over its entries.
 template <int B>
-__device__ __attribute__((noinline)) SelectResult SelectAppend(const float *acc, uint32_t *sHist,
-                                                               unsigned long long *pool, int64_t capq,
-                                                               uint32_t *count, uint32_t *histw, uint32_t *tau,
-                                                               int32_t base, uint32_t floor, uint32_t k, uint32_t kept,
-                                                               uint32_t mine, uint32_t mx)
+__device__ __forceinline__ uint32_t CountKeys(LdsUint4 *bits4, uint32_t xb, uint32_t xe)
 {
     constexpr int kVec = B / (4 * kWave);
-    constexpr int kIdxBits = Log2(B);
     const uint32_t lane = __lane_id();
-    const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
-    SelectResult res{0, 0, 0};
-    // Keep elements whose key (bits << kIdxBits | (B-1-e)) is at least (tb, te).
-    uint32_t tb = floor, te = 0, tauNew = 0;
-    auto countKey = [&](uint32_t xb, uint32_t xe) {
-        uint32_t c = 0;
+    uint32_t c = 0;
 #pragma unroll 1
-        for (int r = 0; r < kVec; ++r) {
-            const uint4 v = bits4[r * kWave + lane];
-            const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
-            c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) +
-                 (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
-                 (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) +
-                 (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
-        }
-        return c;
-    };
-    const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
-    if (kept > cap) {
-        res.cut = 1;
-        if (k <= static_cast<uint32_t>(kWave)) {
-            const uint32_t f = max(WaveKthLargest(mx, k), floor);
-            mine = countKey(f, 0);
-            kept = WaveSum(mine);
-            tb = f;
-            if (kept <= cap)
-                tauNew = f; // at least k docs of the block score >= f
-        }
-        if (kept > cap) {
-            // Bisection over composite keys: count(>= lo) > cap, count(>= hi) < k.
-            unsigned long long lok = static_cast<unsigned long long>(tb) << kIdxBits;
-            unsigned long long hik = (static_cast<unsigned long long>(WaveMax(mx)) + 1) << kIdxBits;
-            for (int it = 0; it < 64 && hik - lok > 1; ++it) {
-                const unsigned long long mid = lok + (hik - lok) / 2;
-                const uint32_t mb = static_cast<uint32_t>(mid >> kIdxBits);
-                const uint32_t me = static_cast<uint32_t>(mid) & static_cast<uint32_t>(B - 1);
-                const uint32_t cm = WaveSum(countKey(mb, me));
-                if (cm > cap) {
-                    lok = mid;
-                } else if (cm < k) {
-                    hik = mid;
-                } else {
-                    lok = mid;
-                    break;
-                }
-            }
-            tb = static_cast<uint32_t>(lok >> kIdxBits);
-            te = static_cast<uint32_t>(lok) & static_cast<uint32_t>(B - 1);
-            mine = countKey(tb, te);
-            kept = WaveSum(mine);
-            if (kept >= k)
-                tauNew = tb;
-        }
+    for (int r = 0; r < kVec; ++r) {
+        const U32x4 v = bits4[r * kWave + lane];
+        const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
+        c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) + (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
+             (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) + (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
     }
+    return c;
+}
 
-    // Append the kept docs to the pool and count them in the histogram.
-    uint32_t before = mine;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
-        if (lane >= static_cast<uint32_t>(o))
-            before += y;
+struct CutResult {
+    uint32_t tb, te, kept, mine, tauNew;
+};
+
+// The rare part of a task's selection, kept out of line so that its registers do not
+// constrain the streaming loop (a call drains the wave's memory counters, so it touches
+// only LDS and runs only when needed): more than cap = BlockKeepCap(K) docs score at least
+// floor, so the kept set is cut to between K and cap docs, the keys at least (tb, te): at
+// the K-th largest per-lane maximum when K <= 64, else by a bisection over composite keys.
+// tauNew (when nonzero) is a lower bound of the query's K-th score: at least K docs of the
+// block score at least it. kept/mine are the wave and lane counts of docs scoring at least
+// floor, mx the lane's largest score bits.
+template <int B>
+__device__ __attribute__((noinline)) CutResult SelectCut(LdsUint4 *bits4, uint32_t floor, uint32_t k, uint32_t cap,
+                                                         uint32_t mx)
+{
+    constexpr int kIdxBits = Log2(B);
+    static_assert(kIdxBits > 0, "block size");
+    CutResult res{floor, 0, 0, 0, 0};
+    uint32_t kept = cap + 1;
+    if (k <= static_cast<uint32_t>(kWave)) {
+        const uint32_t f = max(WaveKthLargest(mx, k), floor);
+        res.mine = CountKeys<B>(bits4, f, 0);
+        kept = WaveSum(res.mine);
+        res.tb = f;
+        if (kept <= cap)
+            res.tauNew = f;
     }
-    before -= mine;
-    uint32_t slot0 = 0;
-    if (lane == 0)
-        slot0 = atomicAdd(count, kept);
-    slot0 = ReadLane(slot0, 0);
-#pragma unroll
-    for (int r = 0; r < kHistPerLane; ++r)
-        if (lane * kHistPerLane + r < kHistBins)
-            sHist[lane * kHistPerLane + r] = 0;
-    if (mine) {
-        uint32_t slot = slot0 + before;
-#pragma unroll 1
-        for (int r = 0; r < kVec; ++r) {
-            const uint4 v = bits4[r * kWave + lane];
-            const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
-            const uint32_t e0 = (r * kWave + lane) * 4;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const uint32_t x = vals[c], rk = static_cast<uint32_t>(B - 1) - (e0 + c);
-                if (x > tb || (x == tb && rk >= te)) {
-                    if (static_cast<int64_t>(slot) < capq)
-                        pool[slot] = (static_cast<unsigned long long>(x) << 32) |
-                                     static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e0 + c));
-                    else
-                        ++res.overflow;
-                    ++slot;
-                    atomicAdd(sHist + HistBucket(x), 1u);
-                }
+    if (kept > cap) {
+        // Bisection over composite keys: count(>= lo) > cap, count(>= hi) < k.
+        unsigned long long lok = static_cast<unsigned long long>(res.tb) << kIdxBits;
+        unsigned long long hik = (static_cast<unsigned long long>(WaveMax(mx)) + 1) << kIdxBits;
+        for (int it = 0; it < 64 && hik - lok > 1; ++it) {
+            const unsigned long long mid = lok + (hik - lok) / 2;
+            const uint32_t mb = static_cast<uint32_t>(mid >> kIdxBits);
+            const uint32_t me = static_cast<uint32_t>(mid) & static_cast<uint32_t>(B - 1);
+            const uint32_t cm = WaveSum(CountKeys<B>(bits4, mb, me));
+            if (cm > cap) {
+                lok = mid;
+            } else if (cm < k) {
+                hik = mid;
+            } else {
+                lok = mid;
+                break;
             }
         }
+        res.tb = static_cast<uint32_t>(lok >> kIdxBits);
+        res.te = static_cast<uint32_t>(lok) & static_cast<uint32_t>(B - 1);
+        res.mine = CountKeys<B>(bits4, res.tb, res.te);
+        kept = WaveSum(res.mine);
+        if (kept >= k)
+            res.tauNew = res.tb;
     }
-    // One global atomic per non-empty bucket of this append.
-#pragma unroll
-    for (int r = 0; r < kHistPerLane; ++r) {
-        const uint32_t bin = lane * kHistPerLane + r;
-        if (bin < kHistBins) {
-            const uint32_t c = sHist[bin];
-            if (c)
-                atomicAdd(histw + bin, c);
-        }
-    }
-    if (lane == 0 && tauNew)
-        atomicMax(tau, tauNew);
     res.kept = kept;
     return res;
 }
 
-// The scoring kernel. Every wave is its own workgroup and walks (query, block) tasks of B
-// docs independently, without barriers, so the waves of a CU overlap one another's memory
-// round trips. Tasks are ordered every kPilotStride-th block of every query first, so the
-// first tasks sample the whole corpus and the per-query lower bounds rise early.
-//
-// Per task:
-//  1. Locate each query word's postings in the block (lane j: word j). Long lists read two
-//     granule-directory entries; short lists (< kLongMin postings) count the posting samples
-//     below the block's ends and take the window of postings between the neighbouring
-//     samples (at most 15 postings outside the block on each side, masked when added).
-//     Under a threshold, lanes g < B/256 first sum the words' granule maxima in word order
-//     (an upper bound of every doc's FP32 score in granule g); granules below the threshold
-//     are cut from both ends of the block, and a block with none left is skipped.
-//  2. Stream the postings in rounds of kColumns 64-posting columns (one 8-byte load per lane
-//     per column, all issued before any add) and add each impact into the wave's LDS
-//     accumulator row. Columns run in query-word order and a doc occurs once per word, so
-//     each doc's score is the sequential FP32 sum in query order (cpu_index_search.h:176-189)
-//     in the wave's own program order.
-//  3. Keep the docs scoring at least the query's running lower bound of its K-th score
-//     (tau, or the score-histogram bound). More than BlockKeepCap(K) of them are cut to
-//     between K and that many: at the K-th largest per-lane maximum when K <= 64, else by a
-//     bisection over unique (score, doc) keys. The cut is a lower bound of the query's K-th
-//     score and raises tau. Kept docs go to the query's candidate pool as 64-bit keys
-//     (score bits << 32 | ~doc) and into its score histogram.
 template <int B>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
 {
     constexpr int kVec = B / (4 * kWave);   // float4 accumulators per lane
-    constexpr int kIdxBits = Log2(B);
     constexpr int kGpb = B / kGranule;      // granules per block
     constexpr int U = kColumns;
     constexpr int kSampleRounds = (kLongMin >> kSampleShift) / kWave;
     static_assert(kGpb <= kWave && kVec >= 1, "block size");
+    static_assert(kHistBins == kWave * kHistPerLane, "every lane loads its histogram bins");
     __shared__ __align__(16) float acc[B];
     __shared__ uint32_t sHist[kHistBins]; // this wave's appends per histogram bucket
+    // Doc IDs of 64 consecutive postings of each of the query's first kShortSlots short lists
+    // (slot c: the c-th short word in query order), entry i at posting cbase[c] + i, INT32_MAX
+    // past the list's end. A wave's tasks walk the blocks of a query in ascending order, so
+    // a short list's postings in a block are found from the cached docs, without memory
+    // traffic, until the cursor leaves the chunk.
+    __shared__ int32_t sChunk[kShortSlots * kWave];
 
     const uint32_t lane = __lane_id();
     unsigned long long scanned = 0, useful = 0, scored = 0, skipped = 0, emitted = 0, overflow = 0, emitting = 0,
@@ -416,11 +348,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t k = 1, tau0q = 0;
     uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
     // Long-list cursor (plain queries): a task's slice starts where the previous block's
-    // ended, and the next block's end is loaded one task ahead (lane j: word j).
-    uint32_t hiPrev = 0, hiNext = 0;
-    uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / hiNext were prepared for
-    int histQ = -1;
-    uint32_t histBound = 1;
+    // ended, and the next block's directory entry is loaded one task ahead (lane j: word j).
+    // The loaded entry is kept as loaded and only used by the next task, so that its load
+    // stays in flight through this task's streaming.
+    uint32_t hiPrev = 0, dirNext = 0;
+    uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / dirNext were prepared for
+    uint32_t cbaseV = 0;     // lane c: posting index of slot c's first cached entry
+    bool chunksValid = false; // the slots hold chunks of the current query's lists
     const uint32_t taskFirst = bt.waveStart[blockIdx.x];
     for (uint32_t task = taskFirst; task < tEnd; ++task) {
         BM25MI_PHASE(0);
@@ -438,17 +372,19 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
             if (static_cast<int>(lane) < n)
                 info = bt.words[w0 + lane];
+            chunksValid = false;
         }
 
-        // Lower bound of the query's K-th score: the histogram bound is refreshed when the
-        // query changes and every 8th task (these bounds only grow, so an older value is
-        // still a lower bound).
-        if (q != histQ || ((task - taskFirst) & 7) == 0) {
-            histQ = q;
-            histBound = WaveHistBound(wk.hist + static_cast<int64_t>(q) * kHistBins, k);
-        }
-        const uint32_t tauq =
-            max(max(__hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP), tau0q), histBound);
+        // Lower bound of the query's K-th score, needed only by the selection: tau and the
+        // score histogram are loaded here, every task, and used there, so the loads stay in
+        // flight through the streaming (unconditional loads keep the wait counts exact).
+        // These bounds only grow, so a value read early is still a lower bound.
+        uint32_t hregs[kHistPerLane];
+#pragma unroll
+        for (int i = 0; i < kHistPerLane; ++i)
+            hregs[i] = __hip_atomic_load(wk.hist + static_cast<int64_t>(q) * kHistBins + lane * kHistPerLane + i,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t tauLoaded = __hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 
         const int64_t g0 = static_cast<int64_t>(blk) * kGpb;
         const int gpb = static_cast<int>(min(static_cast<int64_t>(kGpb), ix.granules - g0));
@@ -463,15 +399,18 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         auto shortWindow = [&](int j, int32_t wlo, int32_t whi, uint32_t &oLo, uint32_t &oLen) {
             const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
             const uint32_t i0 = (first + 15u) >> kSampleShift, i1 = (first + cnt + 15u) >> kSampleShift;
+            // Every lane loads (indices clamped into the list's samples, i1 > i0 - 1 >= 0 since
+            // the list is not empty), so that all loads are in flight before the first use.
+            int32_t smp[kSampleRounds];
+#pragma unroll
+            for (int r = 0; r < kSampleRounds; ++r)
+                smp[r] = ix.samples[min(i0 + r * kWave + lane, i1 - 1u)];
             uint32_t ca = 0, cb = 0;
 #pragma unroll
             for (int r = 0; r < kSampleRounds; ++r) {
-                const uint32_t i = i0 + r * kWave + lane;
-                if (i < i1) {
-                    const int32_t smp = ix.samples[i];
-                    ca += smp < wlo ? 1u : 0u;
-                    cb += smp < whi ? 1u : 0u;
-                }
+                const bool in = i0 + r * kWave + lane < i1;
+                ca += in && smp[r] < wlo ? 1u : 0u;
+                cb += in && smp[r] < whi ? 1u : 0u;
             }
             ca = WaveSum(ca);
             cb = WaveSum(cb);
@@ -480,6 +419,79 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (static_cast<int>(lane) == j) {
                 oLo = ws;
                 oLen = we > ws ? we - ws : 0u;
+            }
+        };
+
+        // Short words in query order; word j's cache slot is its rank among them.
+        const unsigned long long shorts = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
+        // Loads every slot's chunk so that it holds the list's first posting with doc >= at
+        // (or the list's end): the samples locate the last sample below `at`, and the chunk
+        // starts there. All sample loads are issued before any is used, then all chunk loads:
+        // two memory round trips for up to kShortSlots lists. The loads are unconditional
+        // (slots without a word load any in-range entry and are never read), so that the
+        // compiler keeps them in flight together.
+        auto establish = [&](int32_t at) {
+            int32_t sv[kShortSlots][kSampleRounds];
+            unsigned long long m = shorts;
+#pragma unroll
+            for (int c = 0; c < kShortSlots; ++c) {
+                const int j = m ? __ffsll(static_cast<long long>(m)) - 1 : 0;
+                m &= m - 1;
+                const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
+                const uint32_t i0 = (first + 15u) >> kSampleShift, i1 = (first + cnt + 15u) >> kSampleShift;
+#pragma unroll
+                for (int r = 0; r < kSampleRounds; ++r) {
+                    const uint32_t i = i0 + r * kWave + lane;
+                    const int32_t v = ix.samples[min(i, max(i1, 1u) - 1u)];
+                    sv[c][r] = i < i1 ? v : INT32_MAX;
+                }
+            }
+            int32_t dv[kShortSlots];
+            m = shorts;
+#pragma unroll
+            for (int c = 0; c < kShortSlots; ++c) {
+                const int j = m ? __ffsll(static_cast<long long>(m)) - 1 : 0;
+                m &= m - 1;
+                const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
+                const uint32_t i0 = (first + 15u) >> kSampleShift;
+                uint32_t ca = 0;
+#pragma unroll
+                for (int r = 0; r < kSampleRounds; ++r)
+                    ca += sv[c][r] < at ? 1u : 0u;
+                ca = WaveSum(ca);
+                const uint32_t start = ca > 0 ? (i0 + ca - 1) << kSampleShift : first;
+                const uint32_t idx = start + lane;
+                const int32_t v = static_cast<int32_t>(ix.post[min(idx, max(first + cnt, 1u) - 1u)].x);
+                dv[c] = idx < first + cnt ? v : INT32_MAX;
+                cbaseV = static_cast<int>(lane) == c ? start : cbaseV;
+            }
+#pragma unroll
+            for (int c = 0; c < kShortSlots; ++c)
+                sChunk[c * kWave + lane] = dv[c];
+        };
+        // Posting range of short word j, cached in slot c, for docs [wlo, whi): exact from the
+        // chunk (the chunk is reloaded when it lies wholly below wlo); a block that reaches past
+        // the chunk's last entry takes the sample window instead. Lane j receives it.
+        auto shortRange = [&](int j, int32_t wlo, int32_t whi, uint32_t &oLo, uint32_t &oLen) {
+            const int c = __popcll(shorts & ((1ULL << j) - 1));
+            if (c >= kShortSlots) {
+                shortWindow(j, wlo, whi, oLo, oLen);
+                return;
+            }
+            const uint32_t listEnd = ReadLane(info.x, j) + ReadLane(info.y, j);
+            if (__popcll(__ballot(sChunk[c * kWave + lane] < wlo)) == kWave && ReadLane(cbaseV, c) + kWave < listEnd)
+                establish(wlo); // the chunk lies wholly below wlo
+            const int32_t d = sChunk[c * kWave + lane];
+            const uint32_t cb = ReadLane(cbaseV, c);
+            const uint32_t nl = static_cast<uint32_t>(__popcll(__ballot(d < wlo)));
+            const uint32_t nh = static_cast<uint32_t>(__popcll(__ballot(d < whi)));
+            if (nh == kWave && cb + kWave < listEnd) {
+                shortWindow(j, wlo, whi, oLo, oLen);
+                return;
+            }
+            if (static_cast<int>(lane) == j) {
+                oLo = cb + nl;
+                oLen = nh - nl;
             }
         };
 
@@ -576,12 +588,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         float cg = 0.0F; // lane g < gpb: FP32 sum of the non-essential words' maxima in granule g
         // 1. Posting range of each word in the block (lane j: word j). Short lists: sample
         // windows (on the threshold path only the essential ones, needed by the pre-check).
+        if (shorts && !chunksValid) {
+            establish(base); // the chunks of the query's cached short lists, at its first task here
+            chunksValid = true;
+        }
         unsigned long long shortMask =
             __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0 && (!pruned || ess));
         while (shortMask) {
             const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
             shortMask &= shortMask - 1;
-            shortWindow(j, base, end, lo, len);
+            shortRange(j, base, end, lo, len);
         }
         if (pruned) {
             // Pre-check: an FP32 upper bound of every doc's score in granule g, in query order:
@@ -632,7 +648,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (isLong) {
                 if (cursorTask == task) {
                     lo = hiPrev;
-                    len = hiNext - lo;
+                    len = info.x + dirNext - lo;
                 } else {
                     lo = info.x + dirRow[g0];
                     len = info.x + dirRow[g0 + gpb] - lo;
@@ -643,7 +659,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (thr <= 0.0F && task + 1 < tEnd && blk + 1 < blocks) {
                 cursorTask = task + 1;
                 if (isLong)
-                    hiNext = info.x + dirRow[min(g0 + 2 * kGpb, ix.granules)];
+                    dirNext = dirRow[min(g0 + 2 * kGpb, ix.granules)];
             }
             if (WaveSum(len) == 0) {
                 ++skipped; // no query word has postings here: every score is 0
@@ -684,7 +700,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             while (nonEssShort) {
                 const int j = __ffsll(static_cast<long long>(nonEssShort)) - 1;
                 nonEssShort &= nonEssShort - 1;
-                shortWindow(j, base, end, lo, len);
+                shortRange(j, base, end, lo, len);
             }
             const int ga = __ffsll(static_cast<long long>(live)) - 1;
             const int gz = 64 - __clzll(static_cast<long long>(live));
@@ -754,7 +770,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 
         // 3. Selection.
         BM25MI_PHASE(2);
-        uint32_t floor = max(tauq, 1u);
+        uint32_t floor = max(max(max(tauLoaded, tau0q), WaveHistBoundRegs(hregs, k)), 1u);
         if (thr > 0.0F)
             floor = max(floor, __float_as_uint(thr));
         const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
@@ -770,12 +786,71 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         if (kept == 0)
             continue;
         BM25MI_PHASE(3);
-        const SelectResult sel =
-            SelectAppend<B>(acc, sHist, wk.pool + bt.poolOff[q], bt.poolOff[q + 1] - bt.poolOff[q], wk.count + q,
-                            wk.hist + static_cast<int64_t>(q) * kHistBins, wk.tau + q, base, floor, k, kept, mine, mx);
-        cuts += sel.cut;
-        overflow += sel.overflow;
-        emitted += sel.kept;
+        // Cut (rare), then append the kept docs to the pool as composite keys (score bits << 32 |
+        // ~doc) and count them in the histogram: one global atomic reserves the pool slots,
+        // one per non-empty bucket counts them, and the cut raises tau.
+        LdsUint4 *bitsL = (LdsUint4 *)bits4; // acc is in LDS
+        uint32_t tb = floor, te = 0, tauNew = 0;
+        const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
+        if (kept > cap) {
+            const CutResult cr = SelectCut<B>(bitsL, floor, k, cap, mx);
+            tb = cr.tb;
+            te = cr.te;
+            kept = cr.kept;
+            mine = cr.mine;
+            tauNew = cr.tauNew;
+            ++cuts;
+        }
+        uint32_t slot0 = 0;
+        if (lane == 0)
+            slot0 = atomicAdd(wk.count + q, kept);
+        uint32_t before = mine;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
+            if (lane >= static_cast<uint32_t>(o))
+                before += y;
+        }
+        before -= mine;
+#pragma unroll
+        for (int r = 0; r < kHistPerLane; ++r)
+            sHist[lane * kHistPerLane + r] = 0;
+        slot0 = ReadLane(slot0, 0);
+        unsigned long long *pool = wk.pool + bt.poolOff[q];
+        const int64_t capq = bt.poolOff[q + 1] - bt.poolOff[q];
+        if (mine) {
+            uint32_t slot = slot0 + before;
+#pragma unroll 1
+            for (int r = 0; r < kVec; ++r) {
+                const U32x4 v = bitsL[r * kWave + lane];
+                const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
+                const uint32_t e0 = (r * kWave + lane) * 4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint32_t x = vals[c], rk = static_cast<uint32_t>(B - 1) - (e0 + c);
+                    if (x > tb || (x == tb && rk >= te)) {
+                        if (static_cast<int64_t>(slot) < capq)
+                            pool[slot] = (static_cast<unsigned long long>(x) << 32) |
+                                         static_cast<unsigned long long>(0xFFFFFFFFu - (static_cast<uint32_t>(base) + e0 + c));
+                        else
+                            ++overflow;
+                        ++slot;
+                        atomicAdd(sHist + HistBucket(x), 1u);
+                    }
+                }
+            }
+        }
+        uint32_t *histw = wk.hist + static_cast<int64_t>(q) * kHistBins;
+#pragma unroll
+        for (int r = 0; r < kHistPerLane; ++r) {
+            const uint32_t bin = lane * kHistPerLane + r;
+            const uint32_t c = sHist[bin];
+            if (c)
+                atomicAdd(histw + bin, c);
+        }
+        if (lane == 0 && tauNew)
+            atomicMax(wk.tau + q, tauNew);
+        emitted += kept;
         ++emitting;
     }
     BM25MI_PHASE(0);

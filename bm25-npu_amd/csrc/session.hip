@@ -788,6 +788,22 @@ extern "C" int bm25mi_set_launch(bm25mi_session *s, int32_t grid_waves, int32_t 
         &s->error);
 }
 
+#ifdef BM25MI_PHASES
+// Measurement builds only: the per-wave counter rows of slot 0's last Search ([grid][16]).
+extern "C" int bm25mi_debug_wave_rows(bm25mi_session *s, unsigned long long *out, int64_t rows)
+{
+    return Invoke([&] {
+        Require(s && s->impl && out, "Null session or output");
+        std::lock_guard<std::mutex> guard(s->mutex);
+        Session &ss = *s->impl;
+        Require(rows >= ss.grid, "Output holds fewer rows than the scoring grid");
+        BM25MI_HIP(hipMemcpy(out, ss.slots[0].dStats, sizeof(unsigned long long) * kStats * ss.grid,
+                             hipMemcpyDeviceToHost),
+                   "copy wave rows");
+    });
+}
+#endif
+
 extern "C" int bm25mi_last_stats(const bm25mi_session *s, bm25mi_stats *out)
 {
     return Invoke([&] {
