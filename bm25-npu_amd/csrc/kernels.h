@@ -61,10 +61,11 @@ struct IndexView {
 
 // One Search batch, laid out in a single device packet uploaded per Search.
 struct BatchView {
-    const int32_t *header;  // [0] = queries
+    const int32_t *header;  // [0] = queries, [1] = queries with a threshold
     const int64_t *qoff;    // [maxQ+1]
     const int32_t *k;       // [maxQ]
-    const float *thr;       // [maxQ] pruning threshold (0 = plain)
+    const float *thr;       // [maxQ] pruning threshold (0 = plain): the live blocks of a threshold
+                            // query are scored after the wave ranges below, strided over the waves
     const uint32_t *tau0;   // [maxQ] lower bound of the K-th score known before the Search
                             // (score bits): the largest K-th term impact over the query's terms
     const int64_t *poolOff; // [maxQ+1]
@@ -84,6 +85,11 @@ struct WorkView {
     uint32_t *hist;           // [maxQ * kHistBins] histogram of kept candidates' scores
     uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
     unsigned long long *stats;// [grid][kStats]: one counter row per scoring wave
+    // Written by the planning kernel for batches with a threshold query: per task
+    // (q * blocks + blk) the mask of the block's granules whose upper bound reaches the query's
+    // threshold (0 for dead blocks and for plain queries), and per wave the dead blocks it found.
+    uint32_t *liveMask;       // [maxQ * blocks]
+    uint32_t *dead;           // [grid]
     int32_t *status;          // [maxQ] 0 ok, 1 fewer than K survivors under a threshold
     float *scores;            // [maxQ*maxK]
     int32_t *ids;             // [maxQ*maxK]

@@ -355,11 +355,36 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / dirNext were prepared for
     uint32_t cbaseV = 0;     // lane c: posting index of slot c's first cached entry
     bool chunksValid = false; // the slots hold chunks of the current query's lists
-    const uint32_t taskFirst = bt.waveStart[blockIdx.x];
-    for (uint32_t task = taskFirst; task < tEnd; ++task) {
+    uint32_t task = bt.waveStart[blockIdx.x];
+    // After its range, the wave scores the live blocks of threshold queries: tasks
+    // blockIdx.x + i * gridDim.x of the whole task order, whose masks the planning kernel wrote,
+    // 64 masks per load. Striding spreads every query's live blocks evenly over the waves.
+    const uint64_t listEnd = bt.header[1] > 0 ? static_cast<uint64_t>(bt.header[0]) * static_cast<uint32_t>(blocks) : 0;
+    const uint64_t listStep = static_cast<uint64_t>(gridDim.x) * kWave;
+    uint64_t listAt = blockIdx.x; // first task of the next 64 masks to load
+    unsigned long long cand = 0;  // lanes of the loaded masks still to score
+    uint32_t maskV = 0;
+    for (;;) {
         BM25MI_PHASE(0);
-        const int q = static_cast<int>(task / static_cast<uint32_t>(blocks));
-        const int blk = static_cast<int>(task - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks));
+        uint32_t t, liveMask = 0;
+        if (task < tEnd) {
+            t = task++;
+        } else {
+            while (cand == 0 && listAt < listEnd) {
+                const uint64_t tl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
+                maskV = tl < listEnd ? wk.liveMask[tl] : 0u;
+                cand = __ballot(maskV != 0);
+                listAt += listStep;
+            }
+            if (cand == 0)
+                break;
+            const int i = __ffsll(static_cast<long long>(cand)) - 1;
+            cand &= cand - 1;
+            t = static_cast<uint32_t>(listAt - listStep + static_cast<uint64_t>(i) * gridDim.x);
+            liveMask = ReadLane(maskV, i);
+        }
+        const int q = static_cast<int>(t / static_cast<uint32_t>(blocks));
+        const int blk = static_cast<int>(t - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks));
         const int32_t base = blk * B;
         const int32_t end = min(base + B, ix.ndocs);
         if (q != qCur) {
@@ -373,6 +398,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (static_cast<int>(lane) < n)
                 info = bt.words[w0 + lane];
             chunksValid = false;
+        }
+        if (liveMask == 0 && thr > 0.0F) {
+            // A threshold query's live blocks are scored after the range.
+            task = max(task, min(tEnd, static_cast<uint32_t>(q + 1) * static_cast<uint32_t>(blocks)));
+            continue;
         }
 
         // Lower bound of the query's K-th score, needed only by the selection: tau and the
@@ -576,77 +606,25 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             return streamed;
         };
 
-        // Threshold path (MaxScore on granules). The host marked each word essential (bit 31
-        // of its max-impact word) unless the sum of the non-essential words' maxima stays
-        // below the threshold: a doc with no essential posting cannot reach it. With no
-        // non-essential word, the block is scored directly (docs below the threshold are
-        // dropped by the selection floor).
-        const bool ess = info.y > 0 && (info.w >> 31);
-        const bool pruned = thr > 0.0F && __ballot(static_cast<int>(lane) < n && info.y > 0 && !ess) != 0;
-        const float thrLive = thr * 0.99998474F; // thr * (1 - 2^-16): covers any summation order
         uint32_t lo = 0, len = 0;
-        float cg = 0.0F; // lane g < gpb: FP32 sum of the non-essential words' maxima in granule g
-        // 1. Posting range of each word in the block (lane j: word j). Short lists: sample
-        // windows (on the threshold path only the essential ones, needed by the pre-check).
-        if (shorts && !chunksValid) {
-            establish(base); // the chunks of the query's cached short lists, at its first task here
-            chunksValid = true;
-        }
-        unsigned long long shortMask =
-            __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0 && (!pruned || ess));
-        while (shortMask) {
-            const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
-            shortMask &= shortMask - 1;
-            shortRange(j, base, end, lo, len);
-        }
-        if (pruned) {
-            // Pre-check: an FP32 upper bound of every doc's score in granule g, in query order:
-            // long words add their granule maximum, short words their term maximum (an
-            // essential one only when its window reaches into the block). A block where no
-            // granule reaches the threshold is skipped before any posting is read. The
-            // non-essential part is kept as cg for the pass below.
-            float ub = 0.0F;
-            constexpr int kWordsPerLoad = kWave / kGpb;
-            for (int j0 = 0; j0 < n; j0 += kWordsPerLoad) {
-                // Granule maxima of up to kWordsPerLoad words in one load (lane (j-j0)*kGpb + g).
-                const int jr = j0 + static_cast<int>(lane) / kGpb, gl = static_cast<int>(lane) % kGpb;
-                const int jl = min(jr, kWave - 1);
-                const uint32_t ly = static_cast<uint32_t>(__shfl(static_cast<int>(info.y), jl));
-                const int32_t lsl = __shfl(static_cast<int>(info.z), jl);
-                float bv = 0.0F;
-                if (jr < n && ly > 0 && lsl >= 0 && gl < gpb)
-                    bv = ix.bmax[static_cast<int64_t>(lsl) * ix.granules + g0 + gl];
-                for (int j = j0; j < min(n, j0 + kWordsPerLoad); ++j) {
-                    if (ReadLane(info.y, j) == 0)
-                        continue;
-                    const uint32_t wj = ReadLane(info.w, j);
-                    const bool essj = (wj >> 31) != 0;
-                    float c;
-                    if (static_cast<int32_t>(ReadLane(info.z, j)) >= 0)
-                        c = __shfl(bv, (j - j0) * kGpb + gl);
-                    else
-                        c = (!essj || ReadLane(len, j) > 0) ? __uint_as_float(wj & 0x7FFFFFFFu) : 0.0F;
-                    if (static_cast<int>(lane) < gpb) {
-                        ub += c;
-                        if (!essj)
-                            cg += c;
-                    }
-                }
+        uint32_t segLo, segLen;
+        int nseg;
+        if (liveMask == 0) {
+            // 1. Plain task: the posting range of each word in the block (lane j: word j). Short
+            // lists from the cached chunks, long lists from the cursor (this block starts where the
+            // previous one ended; the next block's end is loaded one task ahead) or the directory.
+            if (shorts && !chunksValid) {
+                establish(base); // the chunks of the query's cached short lists, at its first task here
+                chunksValid = true;
             }
-            if (!__ballot(static_cast<int>(lane) < gpb && ub >= thrLive)) {
-                ++skipped; // no doc of the block can reach the threshold
-                continue;
+            unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
+            while (shortMask) {
+                const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
+                shortMask &= shortMask - 1;
+                shortRange(j, base, end, lo, len);
             }
-            if (ess && isLong) {
-                lo = info.x + dirRow[g0];
-                len = info.x + dirRow[g0 + gpb] - lo;
-            }
-            cursorTask = 0xFFFFFFFFu;
-        } else {
-            // Long lists: the directory, or on the plain path the cursor (this block starts where
-            // the previous one ended; the next block's end is loaded one task ahead).
             if (isLong) {
-                if (cursorTask == task) {
+                if (cursorTask == t) {
                     lo = hiPrev;
                     len = info.x + dirNext - lo;
                 } else {
@@ -656,8 +634,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             }
             hiPrev = lo + len;
             cursorTask = 0xFFFFFFFFu;
-            if (thr <= 0.0F && task + 1 < tEnd && blk + 1 < blocks) {
-                cursorTask = task + 1;
+            if (t + 1 < tEnd && blk + 1 < blocks) {
+                cursorTask = t + 1;
                 if (isLong)
                     dirNext = dirRow[min(g0 + 2 * kGpb, ix.granules)];
             }
@@ -665,51 +643,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 ++skipped; // no query word has postings here: every score is 0
                 continue;
             }
-        }
-        // Pass 0 (threshold path) adds only the essential words' impacts; a doc's score is at
-        // most that plus cg of its granule, so granules where no doc reaches the threshold are
-        // dead. Pass 1 scores the live granules exactly (the plain path makes only pass 1).
-        uint32_t segLo = lo, segLen = len;
-        int nseg = n;
-        BM25MI_PHASE(1);
-        bool dead = false;
-        for (int pass = pruned ? 0 : 1; pass < 2; ++pass) {
-            // 2. Stream and add, word by word in query order.
-            scanned += streamAdd(segLo, segLen, nseg);
-            if (pass == 1)
-                break;
-            unsigned long long live = 0;
-#pragma unroll
-            for (int r = 0; r < kVec; ++r) {
-                // Row r of the accumulators (docs r*256 ...) is granule r of the block.
-                const float4 v = acc4[r * kWave + lane];
-                const float m = fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w));
-                const float c = __uint_as_float(ReadLane(__float_as_uint(cg), r));
-                if (__ballot(r < gpb && m + c >= thrLive))
-                    live |= 1ULL << r;
-            }
-            if (!live) {
-                dead = true;
-                break;
-            }
-            // Windows of the non-essential short words, then the pass 1 segments: each long
-            // word's runs of live granules and each short word's window, in query order (lane
-            // s: segment s); more than 64 fall back to one span per word over [first live,
-            // last live] granules.
-            unsigned long long nonEssShort = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0 && !ess);
-            while (nonEssShort) {
-                const int j = __ffsll(static_cast<long long>(nonEssShort)) - 1;
-                nonEssShort &= nonEssShort - 1;
-                shortRange(j, base, end, lo, len);
-            }
-            const int ga = __ffsll(static_cast<long long>(live)) - 1;
-            const int gz = 64 - __clzll(static_cast<long long>(live));
-            uint32_t sLo = 0, sLen = 0;
-            int ns = 0;
+            segLo = lo;
+            segLen = len;
+            nseg = n;
+        } else {
+            // 1. Live item of a threshold query: only the granules in liveMask can hold a doc
+            // scoring at least the threshold (the planning kernel summed the words' granule
+            // maxima). Segments, in query order (lane s: segment s): each long word's runs of
+            // live granules, from the directory, and each short word's sample window (docs
+            // outside the block are masked when added; docs in dead granules score below the
+            // threshold and the selection floor drops them). More than 64 segments fall back to
+            // one span per word over [first live, last live] granules.
+            cursorTask = 0xFFFFFFFFu;
+            const unsigned long long live = liveMask;
             constexpr int kDirPerWord = kGpb + 1, kWordsPerLoad = kWave / kDirPerWord;
-            for (int j0 = 0; j0 < n && ns <= kWave; j0 += kWordsPerLoad) {
-                // Directory entries of up to kWordsPerLoad words in one load: lane
-                // (j - j0) * kDirPerWord + g holds word j's entry for granule g0 + g.
+            // Directory entries of up to kWordsPerLoad words in one load: lane
+            // (j - j0) * kDirPerWord + g holds word j's entry for granule g0 + g.
+            auto dirLoad = [&](int j0) {
                 const int jl = min(j0 + static_cast<int>(lane) / kDirPerWord, kWave - 1);
                 const int gl = static_cast<int>(lane) % kDirPerWord;
                 const uint32_t ly = static_cast<uint32_t>(__shfl(static_cast<int>(info.y), jl));
@@ -717,6 +667,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 uint32_t dv = 0;
                 if (static_cast<int>(lane) < kWordsPerLoad * kDirPerWord && jl < n && ly > 0 && lsl >= 0 && gl <= gpb)
                     dv = ix.dir[static_cast<int64_t>(lsl) * (ix.granules + 1) + g0 + gl];
+                return dv;
+            };
+            const uint32_t dvFirst = dirLoad(0); // in flight through the short windows
+            unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
+            while (shortMask) {
+                const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
+                shortMask &= shortMask - 1;
+                shortWindow(j, base, end, lo, len);
+            }
+            const int ga = __ffsll(static_cast<long long>(live)) - 1;
+            const int gz = 64 - __clzll(static_cast<long long>(live));
+            uint32_t sLo = 0, sLen = 0;
+            int ns = 0;
+            for (int j0 = 0; j0 < n && ns <= kWave; j0 += kWordsPerLoad) {
+                const uint32_t dv = j0 == 0 ? dvFirst : dirLoad(j0);
                 for (int j = j0; j < min(n, j0 + kWordsPerLoad); ++j) {
                     if (ReadLane(info.y, j) == 0)
                         continue;
@@ -761,10 +726,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             segLen = sLen;
             nseg = ns;
         }
-        if (dead) {
-            ++skipped; // no doc of the block can reach the threshold
-            continue;
-        }
+        // 2. Stream and add, word by word in query order.
+        BM25MI_PHASE(1);
+        scanned += streamAdd(segLo, segLen, nseg);
         ++scored;
         useful += inRange;
 
@@ -858,6 +822,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // Counters go to this wave's own row (plain stores); the final kernel sums the rows.
     useful = WaveSum64(useful);
     overflow = WaveSum64(overflow);
+    if (listEnd) // blocks of threshold queries the planning kernel found dead
+        skipped += wk.dead[blockIdx.x];
     if (lane == 0) {
         unsigned long long *row = wk.stats + static_cast<int64_t>(blockIdx.x) * kStats;
         row[kStatPostings] = scanned;
@@ -882,6 +848,140 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+__device__ __forceinline__ unsigned long long WaveOr64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v |= __shfl_xor(v, o);
+    return v;
+}
+
+constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
+
+// Planning kernel of threshold queries (the learned path; no reference counterpart). Every
+// wave takes a contiguous range of (query, 64-granule chunk) items in query-major order; lane
+// g sums, over the query's words, an upper bound of each word's impact in granule g: the
+// granule maximum of a long list, the term maximum of a short list when one of its postings
+// lies in the granule (found through the posting samples, then the postings themselves). FP32
+// addition of non-negative values is monotone, so the FP32 score of every doc in the granule,
+// summed in query order, is at most the same sum of these bounds; the sum here may be taken
+// in another order, which the factor (1 - 2^-16) on the threshold covers for up to 64 words.
+// A block with no granule reaching the threshold cannot hold a doc scoring at least it. The
+// kernel writes every block's mask of live granules (0 for dead blocks and for the blocks of
+// plain queries) and, per wave, the number of dead blocks; it runs with the scoring grid.
+template <int B>
+__global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
+{
+    constexpr int kGpb = B / kGranule;
+    constexpr int kBpc = kPruneSpan / kGpb; // blocks per item
+    constexpr int kSampleRounds = (kLongMin >> kSampleShift) / kWave;
+    static_assert(kPruneSpan % kGpb == 0 && kGpb <= 32, "block size");
+    const uint32_t lane = __lane_id();
+    const int32_t nq = bt.header[0];
+    const int64_t granules = ix.granules;
+    const uint32_t cpq = static_cast<uint32_t>((granules + kPruneSpan - 1) / kPruneSpan);
+    const uint32_t items = static_cast<uint32_t>(nq) * cpq;
+    const uint32_t per = (items + gridDim.x - 1) / gridDim.x;
+    const uint32_t i0 = min(items, blockIdx.x * per), i1 = min(items, i0 + per);
+    int qCur = -1, n = 0;
+    float thrLive = 0.0F;
+    uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+    uint32_t dead = 0;
+    for (uint32_t it = i0; it < i1; ++it) {
+        const int q = static_cast<int>(it / cpq);
+        const int64_t gBase = static_cast<int64_t>(it - static_cast<uint32_t>(q) * cpq) * kPruneSpan;
+        if (q != qCur) {
+            qCur = q;
+            const float thr = bt.thr[q];
+            thrLive = thr * 0.99998474F; // thr * (1 - 2^-16)
+            n = 0;
+            info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+            if (thr > 0.0F) {
+                const int64_t w0 = bt.qoff[q];
+                n = static_cast<int>(bt.qoff[q + 1] - w0);
+                if (static_cast<int>(lane) < n)
+                    info = bt.words[w0 + lane];
+            }
+        }
+        const int64_t blk = gBase / kGpb + lane; // lane b < kBpc: block gBase / kGpb + b of this chunk
+        const bool inRange = static_cast<int>(lane) < kBpc && blk < blocks;
+        uint32_t *maskOut = wk.liveMask + static_cast<int64_t>(q) * blocks;
+        if (!(thrLive > 0.0F)) {
+            if (inRange)
+                maskOut[blk] = 0u; // a plain query
+            continue;
+        }
+        const int64_t g = gBase + lane;
+        const bool valid = g < granules;
+        float ub = 0.0F;
+        // Long words: eight granule-maximum rows in flight at a time.
+        unsigned long long longs = __ballot(static_cast<int>(lane) < n && info.y > 0 && static_cast<int32_t>(info.z) >= 0);
+        while (longs) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                v[u] = 0.0F;
+                if (longs) {
+                    const int j = __ffsll(static_cast<long long>(longs)) - 1;
+                    longs &= longs - 1;
+                    const int64_t slotRow = static_cast<int64_t>(static_cast<int32_t>(ReadLane(info.z, j))) * granules;
+                    if (valid)
+                        v[u] = ix.bmax[slotRow + g];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                ub += v[u];
+        }
+        // Short words: the postings between the samples that bracket the chunk's docs.
+        unsigned long long shorts = __ballot(static_cast<int>(lane) < n && info.y > 0 && static_cast<int32_t>(info.z) < 0);
+        if (shorts) {
+            const int32_t dlo = static_cast<int32_t>(gBase << kGranuleShift);
+            const int32_t dhi = static_cast<int32_t>(min((gBase + kPruneSpan) << kGranuleShift, static_cast<int64_t>(ix.ndocs)));
+            while (shorts) {
+                const int j = __ffsll(static_cast<long long>(shorts)) - 1;
+                shorts &= shorts - 1;
+                const uint32_t first = ReadLane(info.x, j), cnt = ReadLane(info.y, j);
+                const uint32_t s0 = (first + 15u) >> kSampleShift, s1 = (first + cnt + 15u) >> kSampleShift;
+                int32_t smp[kSampleRounds];
+#pragma unroll
+                for (int r = 0; r < kSampleRounds; ++r)
+                    smp[r] = ix.samples[min(s0 + r * kWave + lane, s1 - 1u)];
+                uint32_t ca = 0, cb = 0;
+#pragma unroll
+                for (int r = 0; r < kSampleRounds; ++r) {
+                    const bool in = s0 + r * kWave + lane < s1;
+                    ca += in && smp[r] < dlo ? 1u : 0u;
+                    cb += in && smp[r] < dhi ? 1u : 0u;
+                }
+                ca = WaveSum(ca);
+                cb = WaveSum(cb);
+                const uint32_t ws = ca > 0 ? ((s0 + ca - 1) << kSampleShift) + 1 : first;
+                const uint32_t we = cb < s1 - s0 ? (s0 + cb) << kSampleShift : first + cnt;
+                unsigned long long m = 0;
+                for (uint32_t p0 = ws; p0 < we; p0 += kWave)
+                    if (p0 + lane < we) {
+                        const uint32_t rel = (ix.post[p0 + lane].x >> kGranuleShift) - static_cast<uint32_t>(gBase);
+                        if (rel < static_cast<uint32_t>(kPruneSpan))
+                            m |= 1ULL << rel;
+                    }
+                m = WaveOr64(m);
+                if ((m >> lane) & 1ULL)
+                    ub += __uint_as_float(ReadLane(info.w, j));
+            }
+        }
+        const unsigned long long liveG = __ballot(valid && ub >= thrLive);
+        uint32_t bm = 0;
+        if (static_cast<int>(lane) < kBpc)
+            bm = static_cast<uint32_t>((liveG >> (lane * kGpb)) & ((1ULL << kGpb) - 1ULL));
+        if (inRange)
+            maskOut[blk] = bm;
+        dead += static_cast<uint32_t>(__popcll(__ballot(inRange && bm == 0)));
+    }
+    if (lane == 0)
+        wk.dead[blockIdx.x] = dead;
+}
+
 // Initial state of the per-query counters; run once at session creation. After that
 // the scoring and final kernels restore it for the next Search.
 __global__ void KReset(WorkView wk, int32_t maxQ)
@@ -891,6 +991,7 @@ __global__ void KReset(WorkView wk, int32_t maxQ)
         wk.count[i] = 0;
         wk.tau[i] = 1;
     }
+
     if (i < static_cast<int64_t>(maxQ) * kHistBins)
         wk.hist[i] = 0;
 }
@@ -972,6 +1073,7 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
         wk.count[q] = 0;
         wk.tau[q] = 1;
     }
+
     for (int i = tid; i < kHistBins; i += kFinalThreads)
         hist[i] = 0;
     // Keep entries with score bits >= tau.
@@ -1075,6 +1177,24 @@ bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const 
         return true;
     case 8192:
         hipLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        return true;
+    default:
+        return false;
+    }
+}
+
+bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
+                 int32_t grid, hipStream_t stream)
+{
+    switch (blockDocs) {
+    case 2048:
+        hipLaunchKernelGGL(KPrune<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        return true;
+    case 4096:
+        hipLaunchKernelGGL(KPrune<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        return true;
+    case 8192:
+        hipLaunchKernelGGL(KPrune<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
         return true;
     default:
         return false;

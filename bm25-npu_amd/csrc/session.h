@@ -48,16 +48,19 @@ struct Query {
 // counters, result block, stream and captured graph.
 struct Slot {
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr;
+    void *dHist = nullptr, *dLive = nullptr, *dDead = nullptr;
     void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     BatchView bt{};
     WorkView wk{};
     hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // Captured Searches: [0] plain batches, [1] batches with a threshold query (the planning
+    // kernel runs first).
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // The launched batch: query count, output row offsets, plain-path postings, timing mode.
     int32_t nq = 0;
+    bool pruned = false; // the packed batch has a threshold query
     std::vector<int64_t> out;
     int64_t total = 0;
     bool timed = false;
@@ -105,8 +108,9 @@ public:
     // least each of its impacts, so the query's K-th score is at least this (0 if unknown).
     uint32_t StartBound(const int32_t *terms, int64_t n, int32_t k) const;
     // Cuts the query-major task order into `grid` contiguous ranges of equal estimated work,
-    // a task of query q costing taskCost + ceil(postings of q / blocks).
-    void PlanWaves(const int64_t *postingsPerQuery, int32_t nq, uint32_t *waveStart) const;
+    // a task of query q costing taskCost + ceil(postings of q / blocks), or nothing when the
+    // query has a threshold (its blocks are scored from the live list).
+    void PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, uint32_t *waveStart) const;
     // Validates a whole request (throws InvalidArgument, touching nothing, when it is not
     // acceptable), then writes it into the slot's packet.
     void Pack(const Query &query, Slot &slot);
@@ -127,7 +131,7 @@ public:
 private:
     void InitSlot(Slot &slot);
     void Capture(Slot &slot);
-    void Enqueue(Slot &slot, bool timed);
+    void Enqueue(Slot &slot, bool timed, bool pruned);
     void Bootstrap();
     template <class T> T *Alloc(size_t count, const char *what)
     {
@@ -139,6 +143,8 @@ private:
 
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
+                 int32_t grid, hipStream_t stream);
+bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream);
 int ScoreBlocksPerCu(int blockDocs);
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,

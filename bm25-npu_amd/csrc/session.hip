@@ -38,13 +38,18 @@ uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
     return bits;
 }
 
-void Session::PlanWaves(const int64_t *postingsPerQuery, int32_t nq, uint32_t *waveStart) const
+void Session::PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, uint32_t *waveStart) const
 {
     std::vector<int64_t> cost(static_cast<size_t>(nq));
     int64_t all = 0;
     for (int32_t q = 0; q < nq; ++q) {
-        cost[q] = taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
+        cost[q] = listed[q] ? 0 : taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
         all += cost[q] * blocks;
+    }
+    if (all == 0) { // every query takes the live list: empty ranges
+        for (int32_t w = 0; w <= grid; ++w)
+            waveStart[w] = 0;
+        return;
     }
     int32_t q = 0;
     int64_t before = 0; // cost of queries < q
@@ -120,7 +125,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     BM25MI_HIP(hipMemGetInfo(&freeBytes, &totalBytes), "memory info");
     const uint64_t known = static_cast<uint64_t>(m) * 8 + static_cast<uint64_t>(m) / 4 + 4 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
-                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(poolEntries) * 8;
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(poolEntries) * 8 +
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks * 4;
     // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
     Require(known <= static_cast<uint64_t>(freeBytes) / 5 * 4, "Index, pools and arenas exceed 80% of free HBM");
 
@@ -207,6 +213,8 @@ void Session::InitSlot(Slot &slot)
     slot.dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
     slot.dStats = Alloc<unsigned long long>(static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
+    slot.dLive = Alloc<uint32_t>(q * static_cast<size_t>(blocks), "alloc live masks");
+    slot.dDead = Alloc<uint32_t>(static_cast<size_t>(gridCapacity), "alloc dead counts");
     auto *pk = static_cast<unsigned char *>(slot.dPacket);
     slot.bt = BatchView{reinterpret_cast<const int32_t *>(pk),
                         reinterpret_cast<const int64_t *>(pk + offQoff),
@@ -223,6 +231,8 @@ void Session::InitSlot(Slot &slot)
                        static_cast<uint32_t *>(slot.dHist),
                        static_cast<uint32_t *>(slot.dTau),
                        static_cast<unsigned long long *>(slot.dStats),
+                       static_cast<uint32_t *>(slot.dLive),
+                       static_cast<uint32_t *>(slot.dDead),
                        reinterpret_cast<int32_t *>(rs + resStatus),
                        reinterpret_cast<float *>(rs + resScores),
                        reinterpret_cast<int32_t *>(rs + resIds)};
@@ -232,12 +242,14 @@ void Session::InitSlot(Slot &slot)
         Capture(slot);
 }
 
-void Session::Enqueue(Slot &slot, bool timed)
+void Session::Enqueue(Slot &slot, bool timed, bool pruned)
 {
     BM25MI_HIP(hipMemcpyAsync(slot.dPacket, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
                "packet H2D");
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
+    if (pruned)
+        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
     Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
@@ -247,18 +259,20 @@ void Session::Enqueue(Slot &slot, bool timed)
 
 void Session::Capture(Slot &slot)
 {
-    BM25MI_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
-    try {
-        Enqueue(slot, false);
-    } catch (...) {
-        hipGraph_t discarded = nullptr;
-        (void)hipStreamEndCapture(slot.stream, &discarded);
-        if (discarded)
-            (void)hipGraphDestroy(discarded);
-        throw;
+    for (int g = 0; g < 2; ++g) {
+        BM25MI_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
+        try {
+            Enqueue(slot, false, g == 1);
+        } catch (...) {
+            hipGraph_t discarded = nullptr;
+            (void)hipStreamEndCapture(slot.stream, &discarded);
+            if (discarded)
+                (void)hipGraphDestroy(discarded);
+            throw;
+        }
+        BM25MI_HIP(hipStreamEndCapture(slot.stream, &slot.graph[g]), "end graph capture");
+        BM25MI_HIP(hipGraphInstantiate(&slot.exec[g], slot.graph[g], nullptr, nullptr, 0), "instantiate graph");
     }
-    BM25MI_HIP(hipStreamEndCapture(slot.stream, &slot.graph), "end graph capture");
-    BM25MI_HIP(hipGraphInstantiate(&slot.exec, slot.graph, nullptr, nullptr, 0), "instantiate graph");
 }
 
 void Session::Pack(const Query &query, Slot &slot)
@@ -321,30 +335,16 @@ void Session::Pack(const Query &query, Slot &slot)
                 info[w] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
             }
         }
-    // Threshold queries: mark the essential words (bit 31 of the max-impact word). Words are
-    // non-essential, smallest maximum first, while the sum of their maxima stays below the
-    // threshold with a 2^-16 relative margin (any FP32 summation order of up to 64 terms
-    // stays within it): a doc with no essential posting then scores below the threshold.
-    std::vector<std::pair<float, int64_t>> order;
+    // Threshold queries are planned by the planning kernel and scored from its live list.
+    std::unique_ptr<bool[]> listed(new bool[static_cast<size_t>(nq)]);
+    int32_t withThr = 0;
     for (int32_t q = 0; q < nq; ++q) {
-        if (!(thr[q] > 0.0F))
-            continue;
-        order.clear();
-        for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w)
-            if (info[w].y > 0)
-                order.emplace_back(tmaxHost[query.terms[w]], w);
-        std::sort(order.begin(), order.end());
-        double sum = 0;
-        size_t cut = 0;
-        while (cut < order.size() && (sum + order[cut].first) * (1.0 + 1.0 / 65536) < static_cast<double>(thr[q]) *
-                                                                                       (1.0 - 1.0 / 65536)) {
-            sum += order[cut].first;
-            ++cut;
-        }
-        for (size_t i = cut; i < order.size(); ++i)
-            info[order[i].second].w |= 0x80000000u;
+        listed[q] = thr[q] > 0.0F;
+        withThr += listed[q] ? 1 : 0;
     }
-    PlanWaves(perQuery.data(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
+    reinterpret_cast<int32_t *>(pk)[1] = withThr;
+    slot.pruned = withThr > 0;
+    PlanWaves(perQuery.data(), listed.get(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
     slot.nq = nq;
     slot.total = total;
 }
@@ -353,10 +353,10 @@ void Session::Launch(Slot &slot, bool timed)
 {
     slot.timed = timed;
     BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
-    if (slot.exec && !timed)
-        BM25MI_HIP(hipGraphLaunch(slot.exec, slot.stream), "replay Search graph");
+    if (slot.exec[slot.pruned ? 1 : 0] && !timed)
+        BM25MI_HIP(hipGraphLaunch(slot.exec[slot.pruned ? 1 : 0], slot.stream), "replay Search graph");
     else
-        Enqueue(slot, timed);
+        Enqueue(slot, timed, slot.pruned);
     BM25MI_HIP(hipEventRecord(slot.ev[3], slot.stream), "record end");
 }
 
@@ -454,12 +454,14 @@ void Session::SetLaunch(int32_t waves, int64_t cost)
     grid = waves;
     taskCost = cost;
     for (auto &slot : slots) {
-        if (slot.exec)
-            BM25MI_HIP(hipGraphExecDestroy(slot.exec), "destroy graph exec");
-        if (slot.graph)
-            BM25MI_HIP(hipGraphDestroy(slot.graph), "destroy graph");
-        slot.exec = nullptr;
-        slot.graph = nullptr;
+        for (int g = 0; g < 2; ++g) {
+            if (slot.exec[g])
+                BM25MI_HIP(hipGraphExecDestroy(slot.exec[g]), "destroy graph exec");
+            if (slot.graph[g])
+                BM25MI_HIP(hipGraphDestroy(slot.graph[g]), "destroy graph");
+            slot.exec[g] = nullptr;
+            slot.graph[g] = nullptr;
+        }
         if (cfg.use_graph)
             Capture(slot);
     }
@@ -491,12 +493,14 @@ void Session::Bootstrap()
 void Session::Release() noexcept
 {
     for (auto &slot : slots) {
-        if (slot.exec)
-            (void)hipGraphExecDestroy(slot.exec);
-        if (slot.graph)
-            (void)hipGraphDestroy(slot.graph);
-        slot.exec = nullptr;
-        slot.graph = nullptr;
+        for (int g = 0; g < 2; ++g) {
+            if (slot.exec[g])
+                (void)hipGraphExecDestroy(slot.exec[g]);
+            if (slot.graph[g])
+                (void)hipGraphDestroy(slot.graph[g]);
+            slot.exec[g] = nullptr;
+            slot.graph[g] = nullptr;
+        }
         for (auto &e : slot.ev)
             if (e) {
                 (void)hipEventDestroy(e);
@@ -507,7 +511,8 @@ void Session::Release() noexcept
             (void)hipStreamDestroy(slot.stream);
             slot.stream = nullptr;
         }
-        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist})
+        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dLive,
+                         &slot.dDead})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;

@@ -170,6 +170,37 @@ def test_million_docs_block_sizes(block, graph, need_gpu, million):
         assert_same(got_s, got_i, want_s, want_i, f'uneven block={block}')
 
 
+@pytest.mark.parametrize('block', [2048, 4096, 8192])
+def test_million_docs_pruned(block, need_gpu, million):
+    """Threshold path through the planning kernel and the live list: uniform random terms
+    (mostly short lists) mixed into df-weighted queries, plain queries in the same batch, graph
+    replay and direct launches, thresholds at and below the K-th score (no rerun) and above it
+    (every such query reruns on the plain path); results equal the oracle bit for bit."""
+    corpus, index, queries = million
+    rng = np.random.default_rng(block)
+    extra = [rng.integers(0, index.vocab, size=3).tolist() + q[:2] for q in queries[:8]]
+    qs = queries + extra
+    offsets, terms = bm25mi.pack(qs)
+    ks = np.array([10, 100, 1, 37] * 6, np.int32)
+    want_s, want_i = orc.search(index, offsets, terms, ks, threads=8)
+    kth = want_s[np.cumsum(ks) - 1].astype(np.float32)
+    with bm25mi.Session(index, max_queries=32, max_words=1024, max_k=100, block_docs=block) as s:
+        for factor in (0.5, 0.9, 1.0):
+            thr = (kth * np.float32(factor)).astype(np.float32)
+            thr[::3] = 0.0
+            for timed in (False, True):
+                got_s, got_i, fb = s.search((offsets, terms), ks, thresholds=thr, timed=timed)
+                assert_same(got_s, got_i, want_s, want_i, f'pruned block={block} f={factor} timed={timed}')
+                assert not fb.any()
+            if factor == 1.0:
+                st = s.stats()
+                assert st['blocks_skipped'] > 0 and st['postings_scanned'] < st['postings_total']
+        thr = (kth * np.float32(1.01)).astype(np.float32)
+        got_s, got_i, fb = s.search((offsets, terms), ks, thresholds=thr)
+        assert_same(got_s, got_i, want_s, want_i, f'rerun block={block}')
+        assert fb[kth > 0].all() and not fb[kth == 0].any()
+
+
 @pytest.fixture(scope='module')
 def million():
     config = bm25mi.CorpusConfig(1_000_000, 1_000_000, 1.07, 56.0, seed=3)
