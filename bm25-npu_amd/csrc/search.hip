@@ -355,34 +355,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / dirNext were prepared for
     uint32_t cbaseV = 0;     // lane c: posting index of slot c's first cached entry
     bool chunksValid = false; // the slots hold chunks of the current query's lists
-    uint32_t task = bt.waveStart[blockIdx.x];
-    // After its range, the wave scores the live blocks of threshold queries: tasks
-    // blockIdx.x + i * gridDim.x of the whole task order, whose masks the planning kernel wrote,
-    // 64 masks per load. Striding spreads every query's live blocks evenly over the waves.
-    const uint64_t listEnd = bt.header[1] > 0 ? static_cast<uint64_t>(bt.header[0]) * static_cast<uint32_t>(blocks) : 0;
-    const uint64_t listStep = static_cast<uint64_t>(gridDim.x) * kWave;
-    uint64_t listAt = blockIdx.x; // first task of the next 64 masks to load
-    unsigned long long cand = 0;  // lanes of the loaded masks still to score
-    uint32_t maskV = 0;
-    for (;;) {
+    for (uint32_t t = bt.waveStart[blockIdx.x]; t < tEnd; ++t) {
         BM25MI_PHASE(0);
-        uint32_t t, liveMask = 0;
-        if (task < tEnd) {
-            t = task++;
-        } else {
-            while (cand == 0 && listAt < listEnd) {
-                const uint64_t tl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
-                maskV = tl < listEnd ? wk.liveMask[tl] : 0u;
-                cand = __ballot(maskV != 0);
-                listAt += listStep;
-            }
-            if (cand == 0)
-                break;
-            const int i = __ffsll(static_cast<long long>(cand)) - 1;
-            cand &= cand - 1;
-            t = static_cast<uint32_t>(listAt - listStep + static_cast<uint64_t>(i) * gridDim.x);
-            liveMask = ReadLane(maskV, i);
-        }
         const int q = static_cast<int>(t / static_cast<uint32_t>(blocks));
         const int blk = static_cast<int>(t - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks));
         const int32_t base = blk * B;
@@ -399,9 +373,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 info = bt.words[w0 + lane];
             chunksValid = false;
         }
-        if (liveMask == 0 && thr > 0.0F) {
-            // A threshold query's live blocks are scored after the range.
-            task = max(task, min(tEnd, static_cast<uint32_t>(q + 1) * static_cast<uint32_t>(blocks)));
+        if (thr > 0.0F) {
+            // A threshold query's live blocks are scored by KLive.
+            t = max(t, min(tEnd, static_cast<uint32_t>(q + 1) * static_cast<uint32_t>(blocks)) - 1);
             continue;
         }
 
@@ -607,136 +581,48 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         };
 
         uint32_t lo = 0, len = 0;
-        uint32_t segLo, segLen;
-        int nseg;
-        if (liveMask == 0) {
-            // 1. Plain task: the posting range of each word in the block (lane j: word j). Short
-            // lists from the cached chunks, long lists from the cursor (this block starts where the
-            // previous one ended; the next block's end is loaded one task ahead) or the directory.
-            if (shorts && !chunksValid) {
-                establish(base); // the chunks of the query's cached short lists, at its first task here
-                chunksValid = true;
+        // 1. Plain task: the posting range of each word in the block (lane j: word j). Short
+        // lists from the cached chunks, long lists from the cursor (this block starts where the
+        // previous one ended; the next block's end is loaded one task ahead) or the directory.
+        if (shorts && !chunksValid) {
+            establish(base); // the chunks of the query's cached short lists, at its first task here
+            chunksValid = true;
+        }
+        unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
+        while (shortMask) {
+            const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
+            shortMask &= shortMask - 1;
+            shortRange(j, base, end, lo, len);
+        }
+        if (isLong) {
+            if (cursorTask == t) {
+                lo = hiPrev;
+                len = info.x + dirNext - lo;
+            } else {
+                lo = info.x + dirRow[g0];
+                len = info.x + dirRow[g0 + gpb] - lo;
             }
-            unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
-            while (shortMask) {
-                const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
-                shortMask &= shortMask - 1;
-                shortRange(j, base, end, lo, len);
-            }
-            if (isLong) {
-                if (cursorTask == t) {
-                    lo = hiPrev;
-                    len = info.x + dirNext - lo;
-                } else {
-                    lo = info.x + dirRow[g0];
-                    len = info.x + dirRow[g0 + gpb] - lo;
-                }
-            }
-            hiPrev = lo + len;
-            cursorTask = 0xFFFFFFFFu;
-            if (t + 1 < tEnd && blk + 1 < blocks) {
-                cursorTask = t + 1;
-                if (isLong)
-                    dirNext = dirRow[min(g0 + 2 * kGpb, ix.granules)];
-            }
-            if (WaveSum(len) == 0) {
-                ++skipped; // no query word has postings here: every score is 0
-                continue;
-            }
-            segLo = lo;
-            segLen = len;
-            nseg = n;
-        } else {
-            // 1. Live item of a threshold query: only the granules in liveMask can hold a doc
-            // scoring at least the threshold (the planning kernel summed the words' granule
-            // maxima). Segments, in query order (lane s: segment s): each long word's runs of
-            // live granules, from the directory, and each short word's sample window (docs
-            // outside the block are masked when added; docs in dead granules score below the
-            // threshold and the selection floor drops them). More than 64 segments fall back to
-            // one span per word over [first live, last live] granules.
-            cursorTask = 0xFFFFFFFFu;
-            const unsigned long long live = liveMask;
-            constexpr int kDirPerWord = kGpb + 1, kWordsPerLoad = kWave / kDirPerWord;
-            // Directory entries of up to kWordsPerLoad words in one load: lane
-            // (j - j0) * kDirPerWord + g holds word j's entry for granule g0 + g.
-            auto dirLoad = [&](int j0) {
-                const int jl = min(j0 + static_cast<int>(lane) / kDirPerWord, kWave - 1);
-                const int gl = static_cast<int>(lane) % kDirPerWord;
-                const uint32_t ly = static_cast<uint32_t>(__shfl(static_cast<int>(info.y), jl));
-                const int32_t lsl = __shfl(static_cast<int>(info.z), jl);
-                uint32_t dv = 0;
-                if (static_cast<int>(lane) < kWordsPerLoad * kDirPerWord && jl < n && ly > 0 && lsl >= 0 && gl <= gpb)
-                    dv = ix.dir[static_cast<int64_t>(lsl) * (ix.granules + 1) + g0 + gl];
-                return dv;
-            };
-            const uint32_t dvFirst = dirLoad(0); // in flight through the short windows
-            unsigned long long shortMask = __ballot(static_cast<int>(lane) < n && info.y > 0 && sl < 0);
-            while (shortMask) {
-                const int j = __ffsll(static_cast<long long>(shortMask)) - 1;
-                shortMask &= shortMask - 1;
-                shortWindow(j, base, end, lo, len);
-            }
-            const int ga = __ffsll(static_cast<long long>(live)) - 1;
-            const int gz = 64 - __clzll(static_cast<long long>(live));
-            uint32_t sLo = 0, sLen = 0;
-            int ns = 0;
-            for (int j0 = 0; j0 < n && ns <= kWave; j0 += kWordsPerLoad) {
-                const uint32_t dv = j0 == 0 ? dvFirst : dirLoad(j0);
-                for (int j = j0; j < min(n, j0 + kWordsPerLoad); ++j) {
-                    if (ReadLane(info.y, j) == 0)
-                        continue;
-                    if (static_cast<int32_t>(ReadLane(info.z, j)) < 0) {
-                        const uint32_t wl = ReadLane(lo, j), wn = ReadLane(len, j);
-                        if (wn && ns < kWave && static_cast<int>(lane) == ns) {
-                            sLo = wl;
-                            sLen = wn;
-                        }
-                        ns += wn ? 1 : 0;
-                        continue;
-                    }
-                    const uint32_t first = ReadLane(info.x, j);
-                    const int at = (j - j0) * kDirPerWord;
-                    unsigned long long runs = live & ~(live << 1);
-                    while (runs) {
-                        const int r0 = __ffsll(static_cast<long long>(runs)) - 1;
-                        runs &= runs - 1;
-                        const unsigned long long after = ~live & (~0ULL << r0);
-                        const int r1 = after ? min(__ffsll(static_cast<long long>(after)) - 1, gpb) : gpb;
-                        const uint32_t s0 = first + ReadLane(dv, at + r0), s1 = first + ReadLane(dv, at + r1);
-                        if (s1 > s0) {
-                            if (ns < kWave && static_cast<int>(lane) == ns) {
-                                sLo = s0;
-                                sLen = s1 - s0;
-                            }
-                            ++ns;
-                        }
-                    }
-                }
-            }
-            if (ns > kWave) {
-                sLo = lo;
-                sLen = len;
-                if (isLong) {
-                    sLo = info.x + dirRow[g0 + ga];
-                    sLen = info.x + dirRow[g0 + gz] - sLo;
-                }
-                ns = n;
-            }
-            segLo = sLo;
-            segLen = sLen;
-            nseg = ns;
+        }
+        hiPrev = lo + len;
+        cursorTask = 0xFFFFFFFFu;
+        if (t + 1 < tEnd && blk + 1 < blocks) {
+            cursorTask = t + 1;
+            if (isLong)
+                dirNext = dirRow[min(g0 + 2 * kGpb, ix.granules)];
+        }
+        if (WaveSum(len) == 0) {
+            ++skipped; // no query word has postings here: every score is 0
+            continue;
         }
         // 2. Stream and add, word by word in query order.
         BM25MI_PHASE(1);
-        scanned += streamAdd(segLo, segLen, nseg);
+        scanned += streamAdd(lo, len, n);
         ++scored;
         useful += inRange;
 
         // 3. Selection.
         BM25MI_PHASE(2);
-        uint32_t floor = max(max(max(tauLoaded, tau0q), WaveHistBoundRegs(hregs, k)), 1u);
-        if (thr > 0.0F)
-            floor = max(floor, __float_as_uint(thr));
+        const uint32_t floor = max(max(max(tauLoaded, tau0q), WaveHistBoundRegs(hregs, k)), 1u);
         const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
         uint32_t mine = 0, mx = 0;
 #pragma unroll 2
@@ -822,8 +708,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // Counters go to this wave's own row (plain stores); the final kernel sums the rows.
     useful = WaveSum64(useful);
     overflow = WaveSum64(overflow);
-    if (listEnd) // blocks of threshold queries the planning kernel found dead
-        skipped += wk.dead[blockIdx.x];
     if (lane == 0) {
         unsigned long long *row = wk.stats + static_cast<int64_t>(blockIdx.x) * kStats;
         row[kStatPostings] = scanned;
@@ -982,6 +866,385 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
         wk.dead[blockIdx.x] = dead;
 }
 
+constexpr int kLiveSlots = 8; // granules a live batch scores together (8 x 256 FP32 = 8 KB of LDS)
+
+// Live kernel of threshold queries (the learned path; no reference counterpart). Every wave
+// walks the tasks blockIdx.x + i * gridDim.x of the task order (q * blocks + blk), 64 live
+// masks per load, and takes their live granules kLiveSlots at a time, from any query. A batch
+// resolves all its (granule, word) pairs together: one gather of the query words, one load of
+// two directory entries per long word (or a two-level 16-ary search of a short list's
+// samples), then it streams the pairs' postings as segments in (granule, query word) order
+// into its granules' FP32 accumulators, so every doc's score is the same sequential FP32 sum
+// in query-word order as on the plain path. Docs scoring at least the query's threshold (and
+// its start bound) are appended to the query's pool; a pool that overflows makes the final
+// kernel rerun the query on the plain path. Batching turns the per-block chain of dependent
+// loads into one chain per kLiveSlots granules.
+template <int B>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KLive(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
+{
+    constexpr int kGpb = B / kGranule;
+    constexpr int U = kColumns;
+    constexpr int kLevel = 16; // samples per level of the short-list search (16 x 16 >= 4096 / 16)
+    static_assert(kLevel * kLevel >= (kLongMin >> kSampleShift), "short lists must fit two levels");
+    __shared__ __align__(16) float acc[kLiveSlots * kGranule];
+    float4 *acc4 = reinterpret_cast<float4 *>(acc);
+    const uint32_t lane = __lane_id();
+    const int32_t nq = bt.header[0];
+    const uint64_t listEnd = bt.header[1] > 0 ? static_cast<uint64_t>(nq) * static_cast<uint32_t>(blocks) : 0;
+    const uint64_t listStep = static_cast<uint64_t>(gridDim.x) * kWave;
+    uint64_t listAt = blockIdx.x;
+    unsigned long long cand = 0; // lanes of the loaded masks still to take
+    uint32_t maskV = 0, curMask = 0, curTask = 0;
+    unsigned long long scanned = 0, scored = 0, emitted = 0, emitting = 0, useful = 0;
+
+    // Per segment of a batch in position-major order (lane r: segment r): first posting, count,
+    // slot | word position << 8.
+    __shared__ uint32_t sSeg[3][kWave];
+    __shared__ uint32_t sHist[kHistBins]; // one slot's appends per histogram bucket
+    int ns = 0; // slots held (lane s < ns: slot s's query and granule), some carried over
+    uint32_t slotQ = 0, slotG = 0;
+    for (;;) {
+        // 1. Up to kLiveSlots live granules.
+        while (ns < kLiveSlots) {
+            if (curMask == 0) {
+                while (cand == 0 && listAt < listEnd) {
+                    const uint64_t tl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
+                    maskV = tl < listEnd ? wk.liveMask[tl] : 0u;
+                    cand = __ballot(maskV != 0);
+                    listAt += listStep;
+                }
+                if (cand == 0)
+                    break;
+                const int i = __ffsll(static_cast<long long>(cand)) - 1;
+                cand &= cand - 1;
+                curTask = static_cast<uint32_t>(listAt - listStep + static_cast<uint64_t>(i) * gridDim.x);
+                curMask = ReadLane(maskV, i);
+                ++scored;
+            }
+            const int b = __ffs(static_cast<int>(curMask)) - 1;
+            curMask &= curMask - 1;
+            const uint32_t q = curTask / static_cast<uint32_t>(blocks);
+            const uint32_t blk = curTask - q * static_cast<uint32_t>(blocks);
+            if (static_cast<int>(lane) == ns) {
+                slotQ = q;
+                slotG = blk * kGpb + static_cast<uint32_t>(b);
+            }
+            ++ns;
+        }
+        if (ns == 0)
+            break;
+
+        // 2. The slots' query words and floors (the threshold, at least the start bound and tau).
+        const bool isSlot = static_cast<int>(lane) < ns;
+        int64_t w0 = 0;
+        uint32_t nw = 0, floorS = 1, kS = 1;
+        if (isSlot) {
+            w0 = bt.qoff[slotQ];
+            nw = static_cast<uint32_t>(bt.qoff[slotQ + 1] - w0);
+            kS = static_cast<uint32_t>(bt.k[slotQ]);
+            const uint32_t tauS = __hip_atomic_load(wk.tau + slotQ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            floorS = max(max(max(__float_as_uint(bt.thr[slotQ]), bt.tau0[slotQ]), tauS), 1u);
+        }
+        uint32_t pairEnd = nw; // inclusive prefix of the slots' word counts
+#pragma unroll
+        for (int o = 1; o < kLiveSlots; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(pairEnd), o));
+            if (lane >= static_cast<uint32_t>(o))
+                pairEnd += y;
+        }
+        // The leading m slots whose (granule, word) pairs fit the 64 lanes (m >= 1: a query has
+        // at most 64 words); the others carry over to the next batch.
+        const int m = __popcll(__ballot(isSlot && pairEnd <= static_cast<uint32_t>(kWave)));
+        const uint32_t pairs = ReadLane(pairEnd, m - 1);
+        const uint32_t pairStart = pairEnd - nw;
+#pragma unroll
+        for (int r = 0; r < kLiveSlots; ++r)
+            acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+
+        // 3. Lane p < pairs: pair p = (slot s, word j), slot-major. Its postings in the slot's
+        // granule: two directory entries of a long list; a short list's range from its samples.
+        {
+            const uint32_t idx = lane;
+            const bool has = idx < pairs;
+            int s = 0;
+#pragma unroll
+            for (int r = 1; r < kLiveSlots; ++r)
+                s += r < m && ReadLane(pairStart, r) <= idx ? 1 : 0;
+            const uint32_t j = idx - static_cast<uint32_t>(__shfl(static_cast<int>(pairStart), s));
+            const int64_t wBase = __shfl(w0, s);
+            const uint32_t g = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), s));
+            uint4 wd = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+            if (has)
+                wd = bt.words[wBase + j];
+            uint32_t sLo = 0, sLen = 0;
+            const bool isLongW = has && wd.y > 0 && static_cast<int32_t>(wd.z) >= 0;
+            const bool isShortW = has && wd.y > 0 && static_cast<int32_t>(wd.z) < 0;
+            if (isLongW) {
+                const uint32_t *row = ix.dir + static_cast<int64_t>(static_cast<int32_t>(wd.z)) * (ix.granules + 1) + g;
+                const uint32_t d0 = row[0], d1 = row[1];
+                sLo = wd.x + d0;
+                sLen = d1 - d0;
+            }
+            if (__ballot(isShortW)) {
+                // The short list's postings of docs [dlo, dhi): count its samples (doc of every
+                // 16th posting, global index) below dlo and dhi with a two-level 16-ary search,
+                // then the postings below each bound among the at most 15 between its bracketing
+                // samples.
+                const int32_t dlo = static_cast<int32_t>(g << kGranuleShift);
+                const int32_t dhi = min(dlo + kGranule, ix.ndocs);
+                const uint32_t i0 = (wd.x + 15u) >> kSampleShift, i1 = (wd.x + wd.y + 15u) >> kSampleShift;
+                const uint32_t S = isShortW ? i1 - i0 : 0u;
+                uint32_t ca = 0, cb = 0;
+#pragma unroll
+                for (int mm = 0; mm < kLevel; ++mm) {
+                    const uint32_t at = static_cast<uint32_t>(mm) * kLevel;
+                    const int32_t v = at < S ? ix.samples[i0 + at] : INT32_MAX;
+                    ca += v < dlo ? 1u : 0u;
+                    cb += v < dhi ? 1u : 0u;
+                }
+                const uint32_t ba = ca ? (ca - 1) * kLevel : 0u, bb = cb ? (cb - 1) * kLevel : 0u;
+                uint32_t ca2 = 0, cb2 = 0;
+#pragma unroll
+                for (int mm = 0; mm < kLevel; ++mm) {
+                    const uint32_t ia = ba + mm, ib = bb + mm;
+                    const int32_t va = ca && ia < S ? ix.samples[i0 + ia] : INT32_MAX;
+                    const int32_t vb = cb && ib < S ? ix.samples[i0 + ib] : INT32_MAX;
+                    ca2 += va < dlo ? 1u : 0u;
+                    cb2 += vb < dhi ? 1u : 0u;
+                }
+                ca = ca ? ba + ca2 : 0u; // samples of the list with doc < dlo
+                cb = cb ? bb + cb2 : 0u; // ... < dhi
+                // Postings before as are below dlo, postings from ae on are not (likewise bs, be
+                // for dhi); at most 15 lie between.
+                const uint32_t listEnd = wd.x + wd.y;
+                const uint32_t as = ca > 0 ? ((i0 + ca - 1) << kSampleShift) + 1 : wd.x;
+                const uint32_t ae = min(ca < S ? (i0 + ca) << kSampleShift : listEnd, listEnd);
+                const uint32_t bs = cb > 0 ? ((i0 + cb - 1) << kSampleShift) + 1 : wd.x;
+                const uint32_t be = min(cb < S ? (i0 + cb) << kSampleShift : listEnd, listEnd);
+                uint32_t na = 0, nb = 0;
+#pragma unroll
+                for (int mm = 0; mm < (1 << kSampleShift) - 1; ++mm) {
+                    const uint32_t ia = as + mm, ib = bs + mm;
+                    const int32_t da = isShortW && ia < ae ? static_cast<int32_t>(ix.post[ia].x) : INT32_MAX;
+                    const int32_t db = isShortW && ib < be ? static_cast<int32_t>(ix.post[ib].x) : INT32_MAX;
+                    na += da < dlo ? 1u : 0u;
+                    nb += db < dhi ? 1u : 0u;
+                }
+                if (isShortW) {
+                    sLo = as + na;
+                    const uint32_t hiIdx = bs + nb;
+                    sLen = hiIdx > sLo ? hiIdx - sLo : 0u;
+                }
+            }
+            // Position-major rank: pairs ordered by (word position j, slot s).
+            uint32_t below = 0, before = 0;
+#pragma unroll
+            for (int r = 0; r < kLiveSlots; ++r) {
+                const uint32_t nr = r < m ? ReadLane(nw, r) : 0u;
+                below += min(nr, j);
+                before += r < s && nr > j ? 1u : 0u;
+            }
+            if (has) {
+                sSeg[0][below + before] = sLo;
+                sSeg[1][below + before] = sLen;
+                sSeg[2][below + before] = static_cast<uint32_t>(s) | (j << 8);
+            }
+        }
+        // Lane r < pairs: segment r in position-major order. Lane p < groups: word position p's
+        // first segment, segments, postings; its columns start at gColStart.
+        const bool hasSeg = lane < pairs;
+        const uint32_t rLo = hasSeg ? sSeg[0][lane] : 0u, rLen = hasSeg ? sSeg[1][lane] : 0u;
+        const uint32_t rTag = hasSeg ? sSeg[2][lane] : 0u;
+        const uint32_t groups = WaveMax(static_cast<int>(lane) < m ? nw : 0u);
+        uint32_t gFirst = 0, gCount = 0;
+#pragma unroll
+        for (int r = 0; r < kLiveSlots; ++r) {
+            const uint32_t nr = r < m ? ReadLane(nw, r) : 0u;
+            gFirst += min(nr, lane);
+            gCount += nr > lane ? 1u : 0u;
+        }
+        uint32_t gTotal = 0;
+#pragma unroll
+        for (int t = 0; t < kLiveSlots; ++t) {
+            const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(rLen), static_cast<int>(min(gFirst + t, 63u))));
+            gTotal += static_cast<uint32_t>(t) < gCount ? v : 0u;
+        }
+        const uint32_t gCols = lane < groups ? (gTotal + kWave - 1) / kWave : 0u;
+        uint32_t gColEnd = gCols;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(gColEnd), o));
+            if (lane >= static_cast<uint32_t>(o))
+                gColEnd += y;
+        }
+        const uint32_t columns = ReadLane(gColEnd, kWave - 1);
+        const uint32_t gColStart = gColEnd - gCols;
+        if (lane >= groups)
+            gColEnd = 0xFFFFFFFFu;
+        // Segment r's first posting position within its word position's group.
+        const uint32_t rFirst = static_cast<uint32_t>(__shfl(static_cast<int>(gFirst), static_cast<int>(rTag >> 8)));
+        uint32_t rFlat = 0;
+#pragma unroll
+        for (int t = 0; t < kLiveSlots - 1; ++t) {
+            const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(rLen), static_cast<int>(min(rFirst + t, 63u))));
+            rFlat += rFirst + t < lane ? v : 0u;
+        }
+        uint32_t totalLen = WaveSum(rLen);
+        scanned += totalLen;
+
+        // 4. Stream the columns, U per round. A column lies within one word position, where the
+        // segments are of different granules and hold distinct docs; positions are added in
+        // order, so every doc's adds follow its query's word order.
+        uint32_t in = 0;
+        for (uint32_t r0 = 0; r0 < columns; r0 += U) {
+            uint2 e[U];
+            uint32_t cb[U], co[U];
+            int grp[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                e[u] = make_uint2(0xFFFFFFFFu, 0u);
+                cb[u] = 0;
+                co[u] = 0;
+                grp[u] = -1;
+                const uint32_t c = r0 + u;
+                if (c < columns) {
+                    const int jg = __popcll(__ballot(gColEnd <= c));
+                    grp[u] = jg;
+                    const uint32_t f = ((c - ReadLane(gColStart, jg)) << 6) + lane;
+                    const uint32_t first = ReadLane(gFirst, jg), cnt = ReadLane(gCount, jg);
+                    uint32_t r = first;
+#pragma unroll
+                    for (int t = 1; t < kLiveSlots; ++t)
+                        if (static_cast<uint32_t>(t) < cnt && ReadLane(rFlat, static_cast<int>(min(first + t, 63u))) <= f)
+                            r = first + t;
+                    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(rLo), static_cast<int>(r)));
+                    const uint32_t fl = static_cast<uint32_t>(__shfl(static_cast<int>(rFlat), static_cast<int>(r)));
+                    const uint32_t sl = static_cast<uint32_t>(__shfl(static_cast<int>(rTag), static_cast<int>(r))) & 0xFFu;
+                    cb[u] = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), static_cast<int>(sl))) << kGranuleShift;
+                    co[u] = sl << kGranuleShift;
+                    if (f < ReadLane(gTotal, jg))
+                        e[u] = ix.post[lo + f - fl];
+                }
+            }
+            const uint32_t cLast = min(r0 + U, columns) - 1 - r0;
+            if (grp[0] == __builtin_amdgcn_readfirstlane(grp[cLast < U ? cLast : 0])) {
+                // One word position: distinct docs, all reads first.
+                float v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = e[u].x - cb[u];
+                    v[u] = rel < static_cast<uint32_t>(kGranule) ? acc[co[u] + rel] : 0.0F;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = e[u].x - cb[u];
+                    if (rel < static_cast<uint32_t>(kGranule)) {
+                        acc[co[u] + rel] = v[u] + __uint_as_float(e[u].y);
+                        ++in;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = e[u].x - cb[u];
+                    if (rel < static_cast<uint32_t>(kGranule)) {
+                        acc[co[u] + rel] += __uint_as_float(e[u].y);
+                        ++in;
+                    }
+                }
+            }
+        }
+        useful += in;
+
+        // 5. Per slot: append the docs scoring at least its floor to its query's pool, at most
+        // min(K, 256) of them: past the granule's K-th key a doc has K docs of its query above
+        // it, so the cut keeps exactly the granule's top K and raises the query's bound tau
+        // (a threshold query's pool holds min(K, 256) entries per granule).
+        const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
+        LdsUint4 *bitsL = (LdsUint4 *)bits4; // acc is in LDS
+        for (int r = 0; r < m; ++r) {
+            const uint32_t fl = ReadLane(floorS, r);
+            const uint4 v = bits4[r * kWave + lane];
+            uint32_t mine = (v.x >= fl ? 1u : 0u) + (v.y >= fl ? 1u : 0u) + (v.z >= fl ? 1u : 0u) +
+                            (v.w >= fl ? 1u : 0u);
+            uint32_t kept = WaveSum(mine);
+            if (kept == 0)
+                continue;
+            const uint32_t q = ReadLane(slotQ, r);
+            const uint32_t kq = ReadLane(kS, r);
+            const uint32_t capG = min(kq, static_cast<uint32_t>(kGranule));
+            uint32_t tb = fl, te = 0;
+            if (kept > capG) {
+                const uint32_t mx = max(max(v.x, v.y), max(v.z, v.w));
+                const CutResult cr = SelectCut<kGranule>(bitsL + r * kWave, fl, kq, capG, mx);
+                tb = cr.tb;
+                te = cr.te;
+                kept = cr.kept;
+                mine = cr.mine;
+                if (lane == 0 && cr.tauNew)
+                    atomicMax(wk.tau + q, cr.tauNew);
+            }
+            const uint32_t doc0 = (ReadLane(slotG, r) << kGranuleShift) + lane * 4;
+            uint32_t slot0 = 0;
+            if (lane == 0)
+                slot0 = atomicAdd(wk.count + q, kept);
+            uint32_t before = mine;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
+                if (lane >= static_cast<uint32_t>(o))
+                    before += y;
+            }
+            uint32_t slot = ReadLane(slot0, 0) + before - mine;
+            unsigned long long *pool = wk.pool + bt.poolOff[q];
+            const int64_t capq = bt.poolOff[q + 1] - bt.poolOff[q];
+            const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int i = 0; i < kHistPerLane; ++i)
+                sHist[lane * kHistPerLane + i] = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (vals[c] > tb || (vals[c] == tb && static_cast<uint32_t>(kGranule - 1) - (lane * 4 + c) >= te)) {
+                    if (static_cast<int64_t>(slot) < capq)
+                        pool[slot] = (static_cast<unsigned long long>(vals[c]) << 32) |
+                                     static_cast<unsigned long long>(0xFFFFFFFFu - (doc0 + c));
+                    ++slot;
+                    atomicAdd(sHist + HistBucket(vals[c]), 1u);
+                }
+            // The query's score histogram, from which the final kernel bounds its K-th score.
+            uint32_t *histw = wk.hist + static_cast<int64_t>(q) * kHistBins;
+#pragma unroll
+            for (int i = 0; i < kHistPerLane; ++i) {
+                const uint32_t bin = lane * kHistPerLane + i;
+                const uint32_t cnt = sHist[bin];
+                if (cnt)
+                    atomicAdd(histw + bin, cnt);
+            }
+            emitted += kept;
+            ++emitting;
+        }
+        // Slots [m, ns) carry over to the next batch.
+        slotQ = static_cast<uint32_t>(__shfl(static_cast<int>(slotQ), static_cast<int>(min(lane + m, 63u))));
+        slotG = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), static_cast<int>(min(lane + m, 63u))));
+        ns -= m;
+    }
+    // Add to the counter row the plain kernel wrote for this wave (same grid, stream order), or
+    // write the row when no plain query ran.
+    useful = WaveSum64(useful);
+    if (lane == 0) {
+        unsigned long long *row = wk.stats + static_cast<int64_t>(blockIdx.x) * kStats;
+        if (bt.header[2] == 0)
+            for (int i = 0; i < kStats; ++i)
+                row[i] = 0;
+        row[kStatPostings] += scanned;
+        row[kStatScored] += scored;
+        row[kStatSkipped] += listEnd ? wk.dead[blockIdx.x] : 0u;
+        row[kStatCandidates] += emitted;
+        row[kStatEmitting] += emitting;
+        row[kStatUseful] += useful;
+    }
+}
+
 // Initial state of the per-query counters; run once at session creation. After that
 // the scoring and final kernels restore it for the next Search.
 __global__ void KReset(WorkView wk, int32_t maxQ)
@@ -1052,6 +1315,7 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     const uint32_t k = static_cast<uint32_t>(bt.k[q]);
     const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
     const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(wk.count[q]), cap));
+    const bool overflowed = static_cast<int64_t>(wk.count[q]) > cap; // only threshold queries (rerun)
     uint32_t *hist = wk.hist + static_cast<int64_t>(q) * kHistBins;
     if (tid < 64)
         sRed[0] = 0; // placeholder write order: the bound below is computed by wave 0
@@ -1086,10 +1350,10 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     }
     __syncthreads();
     const uint32_t c = sN;
-    if (pruned && c < k) {
+    if (pruned && (c < k || overflowed)) {
         if (tid == 0)
-            wk.status[q] = 1; // fewer than K survivors: the host reruns this query on the plain path
-        return;
+            wk.status[q] = 1; // fewer than K survivors, or a full pool: the host reruns this query
+        return;                // on the plain path
     }
     const uint32_t m = c < k ? c : k;
     if (c <= static_cast<uint32_t>(kSortCap)) {
@@ -1195,6 +1459,24 @@ bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const 
         return true;
     case 8192:
         hipLaunchKernelGGL(KPrune<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        return true;
+    default:
+        return false;
+    }
+}
+
+bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
+                int32_t grid, hipStream_t stream)
+{
+    switch (blockDocs) {
+    case 2048:
+        hipLaunchKernelGGL(KLive<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        return true;
+    case 4096:
+        hipLaunchKernelGGL(KLive<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        return true;
+    case 8192:
+        hipLaunchKernelGGL(KLive<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
         return true;
     default:
         return false;

@@ -74,6 +74,8 @@ public:
         Require(n == 0 || terms, "Null query terms");
         Require(k >= 1 && k <= session_.cfg.max_k && k <= session_.ndocs, "K out of range");
         Require(session_.PoolCapacity(k) <= session_.poolEntries, "K needs more pool memory than a slot holds");
+        if (session_.PoolCapacity(k, true) > session_.poolEntries)
+            thr = 0.0F; // a threshold pool would not fit a slot: the request runs on the plain path
         for (int32_t i = 0; i < n; ++i)
             Require(terms[i] >= 0 && terms[i] < INT32_MAX, "Invalid public word ID");
         auto req = std::make_unique<Request>();
@@ -149,7 +151,7 @@ private:
         int64_t pool = 0;
         while (!queue_.empty() && static_cast<int32_t>(tickets.size()) < maxBatch_) {
             const Request &r = *requests_.at(queue_.front());
-            const int64_t need = session_.PoolCapacity(r.k);
+            const int64_t need = session_.PoolCapacity(r.k, r.thr > 0.0F);
             if (static_cast<int64_t>(terms.size() + r.terms.size()) > session_.cfg.max_words ||
                 pool + need > session_.poolEntries)
                 break; // memory admission: the rest waits for the next free slot

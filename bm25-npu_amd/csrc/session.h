@@ -53,14 +53,14 @@ struct Slot {
     BatchView bt{};
     WorkView wk{};
     hipStream_t stream = nullptr;
-    // Captured Searches: [0] plain batches, [1] batches with a threshold query (the planning
-    // kernel runs first).
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // Captured Searches by batch mode: [0] plain queries only; [1] plain and threshold queries
+    // (planning, plain scoring, live scoring); [2] threshold queries only (no plain scoring).
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // The launched batch: query count, output row offsets, plain-path postings, timing mode.
     int32_t nq = 0;
-    bool pruned = false; // the packed batch has a threshold query
+    int mode = 0; // graph of the packed batch (index into exec)
     std::vector<int64_t> out;
     int64_t total = 0;
     bool timed = false;
@@ -101,9 +101,10 @@ public:
     Session &operator=(const Session &) = delete;
 
     void Release() noexcept;
-    // Candidate pool entries one query may need: every block appends at most
-    // BlockKeepCap(K) docs, and no doc is appended twice.
-    int64_t PoolCapacity(int32_t k) const;
+    // Candidate pool entries one query may need: every block of a plain query appends at most
+    // BlockKeepCap(K) docs, every live granule of a threshold query at most min(K, 256), and no
+    // doc is appended twice. A threshold query's pool also fits the plain path.
+    int64_t PoolCapacity(int32_t k, bool threshold = false) const;
     // Score bits of the largest K-th term impact among the query's terms: every doc scores at
     // least each of its impacts, so the query's K-th score is at least this (0 if unknown).
     uint32_t StartBound(const int32_t *terms, int64_t n, int32_t k) const;
@@ -131,7 +132,7 @@ public:
 private:
     void InitSlot(Slot &slot);
     void Capture(Slot &slot);
-    void Enqueue(Slot &slot, bool timed, bool pruned);
+    void Enqueue(Slot &slot, bool timed, int mode);
     void Bootstrap();
     template <class T> T *Alloc(size_t count, const char *what)
     {
@@ -144,6 +145,9 @@ private:
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream);
+// Live granules of threshold queries, after LaunchPrune and LaunchScore with the same grid.
+bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
+                int32_t grid, hipStream_t stream);
 bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream);
 int ScoreBlocksPerCu(int blockDocs);

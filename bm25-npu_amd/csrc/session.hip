@@ -19,9 +19,11 @@ namespace bm25mi {
 
 static size_t Align(size_t n, size_t a) { return (n + a - 1) / a * a; }
 
-int64_t Session::PoolCapacity(int32_t k) const
+int64_t Session::PoolCapacity(int32_t k, bool threshold) const
 {
-    return std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs), ndocs);
+    const int64_t plain = std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs), ndocs);
+    // A threshold query may be scored on the plain path (Pack), so its pool covers both.
+    return threshold ? std::max(plain, std::min<int64_t>(granules * std::min<int64_t>(k, kGranule), ndocs)) : plain;
 }
 
 uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
@@ -117,7 +119,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
 
     // Candidate pools: by default room for max_queries queries of K = max_k in every slot;
     // a smaller pool_entries budget makes callers admit fewer large-K queries per batch.
-    const int64_t fullPool = static_cast<int64_t>(cfg.max_queries) * PoolCapacity(cfg.max_k);
+    const int64_t fullPool =
+        static_cast<int64_t>(cfg.max_queries) * std::max(PoolCapacity(cfg.max_k), PoolCapacity(cfg.max_k, true));
     poolEntries = cfg.pool_entries > 0 ? std::min(cfg.pool_entries, fullPool) : fullPool;
     Require(poolEntries >= PoolCapacity(cfg.max_k), "pool_entries cannot hold one query of K = max_k");
     const int32_t nslots = cfg.slots > 0 ? cfg.slots : 1;
@@ -242,15 +245,18 @@ void Session::InitSlot(Slot &slot)
         Capture(slot);
 }
 
-void Session::Enqueue(Slot &slot, bool timed, bool pruned)
+void Session::Enqueue(Slot &slot, bool timed, int mode)
 {
     BM25MI_HIP(hipMemcpyAsync(slot.dPacket, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
                "packet H2D");
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
-    if (pruned)
+    if (mode != 0)
         Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
-    Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+    if (mode != 2)
+        Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+    if (mode != 0)
+        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
     LaunchFinal(slot.bt, slot.wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev), grid,
@@ -259,10 +265,10 @@ void Session::Enqueue(Slot &slot, bool timed, bool pruned)
 
 void Session::Capture(Slot &slot)
 {
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < 3; ++g) {
         BM25MI_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
         try {
-            Enqueue(slot, false, g == 1);
+            Enqueue(slot, false, g);
         } catch (...) {
             hipGraph_t discarded = nullptr;
             (void)hipStreamEndCapture(slot.stream, &discarded);
@@ -282,7 +288,7 @@ void Session::Pack(const Query &query, Slot &slot)
     Require(nq >= 1 && nq <= cfg.max_queries, "Query batch exceeds declared capacity or is empty");
     Require(query.offsets && query.k, "Null query or K array");
     Require(query.offsets[0] == 0, "query_offsets[0] must be 0");
-    int64_t words = 0, poolNeed = 0;
+    int64_t words = 0;
     for (int32_t q = 0; q < nq; ++q) {
         const int64_t len = query.offsets[q + 1] - query.offsets[q];
         Require(len >= 0 && len <= kMaxTerms && len <= cfg.max_words - words,
@@ -291,12 +297,27 @@ void Session::Pack(const Query &query, Slot &slot)
         Require(query.k[q] >= 1 && query.k[q] <= cfg.max_k && query.k[q] <= ndocs, "K out of range");
         if (query.thr)
             Require(std::isfinite(query.thr[q]) && query.thr[q] >= 0, "Threshold must be finite and >= 0");
-        poolNeed += PoolCapacity(query.k[q]);
     }
-    Require(poolNeed <= poolEntries, "Batch needs more candidate-pool memory than a slot holds");
     Require(words == 0 || query.terms, "Null query terms");
     for (int64_t w = 0; w < words; ++w)
         Require(query.terms[w] >= 0 && query.terms[w] < INT32_MAX, "Invalid public word ID");
+    // A threshold below kLiveMinRatio of the query's score upper bound (the FP32 sum of its
+    // terms' largest impacts) leaves most granules live: such a query is scored on the plain
+    // path, with the same results.
+    std::vector<float> thrUse(static_cast<size_t>(nq), 0.0F);
+    int64_t poolNeed = 0;
+    for (int32_t q = 0; q < nq; ++q) {
+        if (query.thr && query.thr[q] > 0.0F) {
+            float ub = 0.0F;
+            for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w)
+                if (query.terms[w] < vocab)
+                    ub += tmaxHost[query.terms[w]];
+            if (query.thr[q] >= kLiveMinRatio * ub)
+                thrUse[q] = query.thr[q];
+        }
+        poolNeed += PoolCapacity(query.k[q], thrUse[q] > 0.0F);
+    }
+    Require(poolNeed <= poolEntries, "Batch needs more candidate-pool memory than a slot holds");
 
     auto *pk = static_cast<unsigned char *>(slot.hPacket);
     reinterpret_cast<int32_t *>(pk)[0] = nq;
@@ -311,8 +332,8 @@ void Session::Pack(const Query &query, Slot &slot)
     for (int32_t q = 0; q < nq; ++q) {
         qoff[q + 1] = query.offsets[q + 1];
         kk[q] = query.k[q];
-        thr[q] = query.thr ? query.thr[q] : 0.0F;
-        pool[q + 1] = pool[q] + PoolCapacity(query.k[q]);
+        thr[q] = thrUse[q];
+        pool[q + 1] = pool[q] + PoolCapacity(query.k[q], thr[q] > 0.0F);
         out[q + 1] = out[q] + query.k[q];
         slot.out[q + 1] = out[q + 1];
         tau0[q] = StartBound(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q]);
@@ -335,7 +356,7 @@ void Session::Pack(const Query &query, Slot &slot)
                 info[w] = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
             }
         }
-    // Threshold queries are planned by the planning kernel and scored from its live list.
+    // Threshold queries are planned by the planning kernel and scored from its live masks.
     std::unique_ptr<bool[]> listed(new bool[static_cast<size_t>(nq)]);
     int32_t withThr = 0;
     for (int32_t q = 0; q < nq; ++q) {
@@ -343,7 +364,8 @@ void Session::Pack(const Query &query, Slot &slot)
         withThr += listed[q] ? 1 : 0;
     }
     reinterpret_cast<int32_t *>(pk)[1] = withThr;
-    slot.pruned = withThr > 0;
+    reinterpret_cast<int32_t *>(pk)[2] = nq - withThr;
+    slot.mode = withThr == 0 ? 0 : withThr < nq ? 1 : 2;
     PlanWaves(perQuery.data(), listed.get(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
     slot.nq = nq;
     slot.total = total;
@@ -353,10 +375,10 @@ void Session::Launch(Slot &slot, bool timed)
 {
     slot.timed = timed;
     BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
-    if (slot.exec[slot.pruned ? 1 : 0] && !timed)
-        BM25MI_HIP(hipGraphLaunch(slot.exec[slot.pruned ? 1 : 0], slot.stream), "replay Search graph");
+    if (slot.exec[slot.mode] && !timed)
+        BM25MI_HIP(hipGraphLaunch(slot.exec[slot.mode], slot.stream), "replay Search graph");
     else
-        Enqueue(slot, timed, slot.pruned);
+        Enqueue(slot, timed, slot.mode);
     BM25MI_HIP(hipEventRecord(slot.ev[3], slot.stream), "record end");
 }
 
@@ -454,7 +476,7 @@ void Session::SetLaunch(int32_t waves, int64_t cost)
     grid = waves;
     taskCost = cost;
     for (auto &slot : slots) {
-        for (int g = 0; g < 2; ++g) {
+        for (int g = 0; g < 3; ++g) {
             if (slot.exec[g])
                 BM25MI_HIP(hipGraphExecDestroy(slot.exec[g]), "destroy graph exec");
             if (slot.graph[g])
@@ -493,7 +515,7 @@ void Session::Bootstrap()
 void Session::Release() noexcept
 {
     for (auto &slot : slots) {
-        for (int g = 0; g < 2; ++g) {
+        for (int g = 0; g < 3; ++g) {
             if (slot.exec[g])
                 (void)hipGraphExecDestroy(slot.exec[g]);
             if (slot.graph[g])
