@@ -1228,20 +1228,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         slotG = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), static_cast<int>(min(lane + m, 63u))));
         ns -= m;
     }
-    // Add to the counter row the plain kernel wrote for this wave (same grid, stream order), or
-    // write the row when no plain query ran.
+    // This wave's counter row: row gridDim.x + blockIdx.x when the plain kernel runs beside this
+    // one (it writes rows [0, gridDim.x)), else row blockIdx.x.
     useful = WaveSum64(useful);
     if (lane == 0) {
-        unsigned long long *row = wk.stats + static_cast<int64_t>(blockIdx.x) * kStats;
-        if (bt.header[2] == 0)
-            for (int i = 0; i < kStats; ++i)
-                row[i] = 0;
-        row[kStatPostings] += scanned;
-        row[kStatScored] += scored;
-        row[kStatSkipped] += listEnd ? wk.dead[blockIdx.x] : 0u;
-        row[kStatCandidates] += emitted;
-        row[kStatEmitting] += emitting;
-        row[kStatUseful] += useful;
+        const int64_t at = (bt.header[2] > 0 ? gridDim.x : 0u) + blockIdx.x;
+        unsigned long long *row = wk.stats + at * kStats;
+        for (int i = 0; i < kStats; ++i)
+            row[i] = 0;
+        row[kStatPostings] = scanned;
+        row[kStatScored] = scored;
+        row[kStatSkipped] = listEnd ? wk.dead[blockIdx.x] : 0u;
+        row[kStatCandidates] = emitted;
+        row[kStatEmitting] = emitting;
+        row[kStatUseful] = useful;
     }
 }
 

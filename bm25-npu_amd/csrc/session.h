@@ -53,8 +53,11 @@ struct Slot {
     BatchView bt{};
     WorkView wk{};
     hipStream_t stream = nullptr;
-    // Captured Searches by batch mode: [0] plain queries only; [1] plain and threshold queries
-    // (planning, plain scoring, live scoring); [2] threshold queries only (no plain scoring).
+    hipStream_t side = nullptr;          // live scoring beside plain scoring (mixed batches)
+    hipEvent_t fork[2] = {nullptr, nullptr};
+    // Captured Searches by batch mode: [0] plain queries only; [2] threshold queries only
+    // (planning, live scoring). Mode 1 (both kinds: planning, then plain scoring beside live
+    // scoring on the side stream) is launched directly, [1] stays empty.
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -114,7 +114,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
     Require(grid >= 1 && grid <= (1 << 20), "grid_blocks out of range");
-    gridCapacity = std::max(grid, 2 * cus * perCu);
+    gridCapacity = std::max(grid, 2 * cus * perCu); // counter rows: 2 x grid (plain and live kernels)
     Require(static_cast<int64_t>(cfg.max_queries) * blocks < (int64_t(1) << 32), "Too many scoring tasks per Search");
 
     // Candidate pools: by default room for max_queries queries of K = max_k in every slot;
@@ -203,6 +203,9 @@ void Session::InitSlot(Slot &slot)
 {
     const size_t q = static_cast<size_t>(cfg.max_queries);
     BM25MI_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking), "create stream");
+    BM25MI_HIP(hipStreamCreateWithFlags(&slot.side, hipStreamNonBlocking), "create side stream");
+    for (auto &e : slot.fork)
+        BM25MI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create fork event");
     for (auto &e : slot.ev)
         BM25MI_HIP(hipEventCreate(&e), "create event");
     BM25MI_HIP(hipHostMalloc(&slot.hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
@@ -215,7 +218,7 @@ void Session::InitSlot(Slot &slot)
     slot.dCount = Alloc<uint32_t>(q, "alloc pool counts");
     slot.dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
-    slot.dStats = Alloc<unsigned long long>(static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
+    slot.dStats = Alloc<unsigned long long>(2 * static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
     slot.dLive = Alloc<uint32_t>(q * static_cast<size_t>(blocks), "alloc live masks");
     slot.dDead = Alloc<uint32_t>(static_cast<size_t>(gridCapacity), "alloc dead counts");
     auto *pk = static_cast<unsigned char *>(slot.dPacket);
@@ -253,19 +256,34 @@ void Session::Enqueue(Slot &slot, bool timed, int mode)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
     if (mode != 0)
         Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
-    if (mode != 2)
+    if (mode == 1) {
+        // Plain and threshold queries: the live kernel runs beside the plain one on the side
+        // stream (they touch different queries' pools and counters and separate counter rows).
+        BM25MI_HIP(hipEventRecord(slot.fork[0], slot.stream), "fork live scoring");
+        BM25MI_HIP(hipStreamWaitEvent(slot.side, slot.fork[0], 0), "fork live scoring");
+        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.side), "Unsupported block size");
+        BM25MI_HIP(hipEventRecord(slot.fork[1], slot.side), "join live scoring");
         Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
-    if (mode != 0)
+        BM25MI_HIP(hipStreamWaitEvent(slot.stream, slot.fork[1], 0), "join live scoring");
+    } else if (mode == 0) {
+        Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+    } else {
         Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+    }
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
-    LaunchFinal(slot.bt, slot.wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev), grid,
+    // Counter rows: [0, grid) from the plain or the only scoring kernel, [grid, 2 grid) from the
+    // live kernel when both ran.
+    LaunchFinal(slot.bt, slot.wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev),
+                mode == 1 ? 2 * grid : grid,
                 slot.stream);
 }
 
 void Session::Capture(Slot &slot)
 {
     for (int g = 0; g < 3; ++g) {
+        if (g == 1)
+            continue; // mixed batches launch directly (Launch)
         BM25MI_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
         try {
             Enqueue(slot, false, g);
@@ -375,7 +393,9 @@ void Session::Launch(Slot &slot, bool timed)
 {
     slot.timed = timed;
     BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
-    if (slot.exec[slot.mode] && !timed)
+    // Mixed batches launch directly: replaying the captured fork/join graph measured slower than
+    // the same launches issued on the two streams (Q64 x 2-12 terms: +0.35 ms end to end).
+    if (slot.exec[slot.mode] && !timed && slot.mode != 1)
         BM25MI_HIP(hipGraphLaunch(slot.exec[slot.mode], slot.stream), "replay Search graph");
     else
         Enqueue(slot, timed, slot.mode);
@@ -533,6 +553,16 @@ void Session::Release() noexcept
             (void)hipStreamDestroy(slot.stream);
             slot.stream = nullptr;
         }
+        if (slot.side) {
+            (void)hipStreamSynchronize(slot.side);
+            (void)hipStreamDestroy(slot.side);
+            slot.side = nullptr;
+        }
+        for (auto &e : slot.fork)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
         for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dLive,
                          &slot.dDead})
             if (*p) {
