@@ -14,7 +14,7 @@ constexpr int kLongMin = 4096;             // lists this long use the granule di
 constexpr int kSampleShift = 4;            // samples[i] = doc ID of posting 16 * i (global CSC index)
 constexpr int kFinalThreads = 1024;        // selection workgroup of the final kernel
 constexpr int kSortCap = 16384;            // pool entries the final kernel sorts in LDS (128 KB)
-constexpr int kTaskCost = 2048;            // estimated fixed cost of a scoring task, in postings
+constexpr int kTaskCost = 1024;            // estimated fixed cost of a scoring task, in postings
 // A threshold query takes the live path only when its threshold is at least this fraction of the
 // FP32 sum of its terms' largest impacts (below it, most granules stay live; measured on the
 // 10M-doc corpus: 0.64 of the postings streamed under 0.3, 0.13-0.30 above it).

@@ -503,6 +503,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // (lane s: segment s, s < nseg, in query-word order) for docs in [base, end), in
         // rounds of U 64-posting columns. Returns the postings streamed.
         uint32_t inRange = 0;
+        // Largest value this lane wrote to the accumulator row: impacts are non-negative, so every
+        // partial sum is at most the doc's final score, and the wave maximum is the block's top score.
+        float runMax = 0.0F;
         auto streamAdd = [&](uint32_t segLo, uint32_t segLen, int nseg) {
             const uint32_t myCols = static_cast<int>(lane) < nseg ? (segLen + kWave - 1) / kWave : 0u;
             uint32_t colEnd = myCols;
@@ -558,7 +561,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                     for (int u = 0; u < U; ++u) {
                         const uint32_t rel = e[u].x - static_cast<uint32_t>(base);
                         if (rel < span) {
-                            acc[rel] = v[u] + __uint_as_float(e[u].y);
+                            const float nv = v[u] + __uint_as_float(e[u].y);
+                            acc[rel] = nv;
+                            runMax = fmaxf(runMax, nv);
                             ++inRange;
                         }
                     }
@@ -567,7 +572,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                     for (int u = 0; u < U; ++u) {
                         const uint32_t rel = e[u].x - static_cast<uint32_t>(base);
                         if (rel < span) {
-                            acc[rel] += __uint_as_float(e[u].y);
+                            const float nv = acc[rel] + __uint_as_float(e[u].y);
+                            acc[rel] = nv;
+                            runMax = fmaxf(runMax, nv);
                             ++inRange;
                         }
                     }
@@ -620,9 +627,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         ++scored;
         useful += inRange;
 
-        // 3. Selection.
+        // 3. Selection. A block whose top score is below the floor appends nothing: the floor's
+        // cheap part (tau, the start bound) is checked first, then the histogram bound, before
+        // the accumulator row is scanned.
         BM25MI_PHASE(2);
-        const uint32_t floor = max(max(max(tauLoaded, tau0q), WaveHistBoundRegs(hregs, k)), 1u);
+        const uint32_t top = WaveMax(__float_as_uint(runMax));
+        uint32_t floor = max(max(tauLoaded, tau0q), 1u);
+        if (top < floor)
+            continue;
+        floor = max(floor, WaveHistBoundRegs(hregs, k));
+        if (top < floor)
+            continue;
         const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
         uint32_t mine = 0, mx = 0;
 #pragma unroll 2
