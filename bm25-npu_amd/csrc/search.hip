@@ -22,6 +22,7 @@
 //             unique per doc, so ties need no extra pass), bitonic-sorts it (score desc, doc
 //             asc: topk_heap.h:11-16) and writes it to pinned host memory. Fewer than K
 //             positive docs are padded with the lowest-ID zero-score docs, as the reference does.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -1445,17 +1446,19 @@ void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 }
 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream)
+                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
 {
+    // With events, hipExtLaunchKernelGGL stamps them at the kernel's own start and end (the
+    // interval the profiler reports), not around the launch.
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipExtLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipExtLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipExtLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks);
         return true;
     default:
         return false;

@@ -146,8 +146,9 @@ private:
 };
 
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
+// start/stop (optional): events stamped at the kernel's own start and end.
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream);
+                 int32_t grid, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // Live granules of threshold queries, after LaunchPrune and LaunchScore with the same grid.
 bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                 int32_t grid, hipStream_t stream);

@@ -252,7 +252,9 @@ void Session::Enqueue(Slot &slot, bool timed, int mode)
 {
     BM25MI_HIP(hipMemcpyAsync(slot.dPacket, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
                "packet H2D");
-    if (timed)
+    // Timed plain batches: events stamped by the scoring kernel itself; otherwise recorded
+    // around the scoring kernels (planning, plain and live scoring).
+    if (timed && mode != 0)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
     if (mode != 0)
         Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
@@ -266,11 +268,13 @@ void Session::Enqueue(Slot &slot, bool timed, int mode)
         Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
         BM25MI_HIP(hipStreamWaitEvent(slot.stream, slot.fork[1], 0), "join live scoring");
     } else if (mode == 0) {
-        Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+        Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, timed ? slot.ev[1] : nullptr,
+                            timed ? slot.ev[2] : nullptr),
+                "Unsupported block size");
     } else {
         Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
     }
-    if (timed)
+    if (timed && mode != 0)
         BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
     // Counter rows: [0, grid) from the plain or the only scoring kernel, [grid, 2 grid) from the
     // live kernel when both ran.
