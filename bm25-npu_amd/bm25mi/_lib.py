@@ -158,8 +158,9 @@ def check(rc, handle=None):
 
 
 def ptr(array):
-    """Raw data pointer of a C-contiguous numpy array (or None)."""
+    """Raw data address of a C-contiguous numpy array (or None), passed as a void* argument."""
     if array is None:
         return None
-    assert array.flags['C_CONTIGUOUS'], 'array must be C-contiguous'
-    return array.ctypes.data_as(C.c_void_p)
+    if not array.flags['C_CONTIGUOUS']:
+        raise ValueError('array must be C-contiguous')
+    return array.ctypes.data

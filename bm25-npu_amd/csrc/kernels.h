@@ -77,9 +77,10 @@ struct BatchView {
     // Per query word, resolved on the host from the term ID: {first posting, posting
     // count (0 for IDs >= vocab), long-term slot or -1, largest impact bits}.
     const uint4 *words;     // [maxWords]
-    // Scoring wave w takes tasks [waveStart[w], waveStart[w+1]) of the query-major task
-    // order (task = q * blocks + blk).
-    const uint32_t *waveStart; // [grid+1]
+    // Wave plan: costPrefix[q] = sum over queries p < q of (task cost of p) * blocks, in posting
+    // equivalents (0 for threshold queries). Scoring wave w takes the tasks of the query-major
+    // task order (task = q * blocks + blk) from cost costPrefix[nq] * w / grid on.
+    const int64_t *costPrefix; // [maxQ+1]
 };
 
 // Per-Search device scratch and results.

@@ -301,6 +301,38 @@ __device__ __attribute__((noinline)) CutResult SelectCut(LdsUint4 *bits4, uint32
     return res;
 }
 
+// First task of the query-major order at plan cost `target` (costPrefix: see BatchView); c =
+// queries whose cost prefix ends at or below target.
+__device__ __forceinline__ uint32_t PlanTask(const int64_t *prefix, int32_t nq, int32_t blocks, uint64_t target,
+                                             uint32_t c)
+{
+    if (c >= static_cast<uint32_t>(nq))
+        return static_cast<uint32_t>(nq) * static_cast<uint32_t>(blocks);
+    const uint64_t p0 = static_cast<uint64_t>(prefix[c]), cost = (static_cast<uint64_t>(prefix[c + 1]) - p0) / blocks;
+    const uint64_t in = (target - p0 + cost - 1) / cost;
+    return c * static_cast<uint32_t>(blocks) + static_cast<uint32_t>(min(in, static_cast<uint64_t>(blocks)));
+}
+
+// This wave's contiguous range [t0, t1) of the task order: an equal share of the plan's total
+// cost, found by comparing the cost prefix with the share's ends, 64 queries per load.
+__device__ __forceinline__ void WaveRange(const BatchView &bt, int32_t blocks, uint32_t &t0, uint32_t &t1)
+{
+    const int32_t nq = bt.header[0];
+    const uint64_t total = static_cast<uint64_t>(bt.costPrefix[nq]);
+    const uint64_t lo = total * blockIdx.x / gridDim.x, hi = total * (blockIdx.x + 1) / gridDim.x;
+    uint32_t c0 = 0, c1 = 0;
+    for (int32_t base = 0; base < nq; base += kWave) {
+        const int32_t i = base + static_cast<int32_t>(__lane_id()) + 1;
+        const uint64_t v = i <= nq ? static_cast<uint64_t>(bt.costPrefix[i]) : ~0ULL;
+        c0 += static_cast<uint32_t>(__popcll(__ballot(v <= lo)));
+        c1 += static_cast<uint32_t>(__popcll(__ballot(v <= hi)));
+        if (__ballot(v > hi))
+            break;
+    }
+    t0 = PlanTask(bt.costPrefix, nq, blocks, lo, c0);
+    t1 = PlanTask(bt.costPrefix, nq, blocks, hi, c1);
+}
+
 template <int B>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
 {
@@ -342,7 +374,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 
     // This wave's tasks: a contiguous range of the query-major task order (task = q * blocks
     // + blk), cut by the host so that every wave gets the same estimated work.
-    const uint32_t tEnd = bt.waveStart[blockIdx.x + 1];
+    uint32_t tBegin, tEnd;
+    WaveRange(bt, blocks, tBegin, tEnd);
     // Per-query state, reloaded only when the task's query changes.
     int qCur = -1, n = 0;
     float thr = 0.0F;
@@ -356,7 +389,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t cursorTask = 0xFFFFFFFFu; // the task hiPrev / dirNext were prepared for
     uint32_t cbaseV = 0;     // lane c: posting index of slot c's first cached entry
     bool chunksValid = false; // the slots hold chunks of the current query's lists
-    for (uint32_t t = bt.waveStart[blockIdx.x]; t < tEnd; ++t) {
+    for (uint32_t t = tBegin; t < tEnd; ++t) {
         BM25MI_PHASE(0);
         const int q = static_cast<int>(t / static_cast<uint32_t>(blocks));
         const int blk = static_cast<int>(t - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks));

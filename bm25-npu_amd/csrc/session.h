@@ -89,7 +89,7 @@ public:
     void *dPost = nullptr, *dSamples = nullptr, *dDir = nullptr, *dBmax = nullptr;
     IndexView ix{};
     size_t packetBytes = 0, resultBytes = 0;
-    size_t offQoff = 0, offK = 0, offThr = 0, offTau0 = 0, offPool = 0, offOut = 0, offTerms = 0, offWave = 0;
+    size_t offQoff = 0, offK = 0, offThr = 0, offTau0 = 0, offPool = 0, offOut = 0, offTerms = 0, offCost = 0;
     size_t resStatus = 0, resScores = 0, resIds = 0;
     std::vector<Slot> slots;
     int device = 0;
@@ -111,10 +111,10 @@ public:
     // Score bits of the largest K-th term impact among the query's terms: every doc scores at
     // least each of its impacts, so the query's K-th score is at least this (0 if unknown).
     uint32_t StartBound(const int32_t *terms, int64_t n, int32_t k) const;
-    // Cuts the query-major task order into `grid` contiguous ranges of equal estimated work,
-    // a task of query q costing taskCost + ceil(postings of q / blocks), or nothing when the
-    // query has a threshold (its blocks are scored from the live list).
-    void PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, uint32_t *waveStart) const;
+    // The wave plan: the prefix of per-query task costs (a task of query q costing taskCost +
+    // ceil(postings of q / blocks), threshold queries nothing); every scoring wave cuts its own
+    // contiguous range of equal estimated work out of the query-major task order (WaveRange).
+    void PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, int64_t *costPrefix) const;
     // Validates a whole request (throws InvalidArgument, touching nothing, when it is not
     // acceptable), then writes it into the slot's packet.
     void Pack(const Query &query, Slot &slot);

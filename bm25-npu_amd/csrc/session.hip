@@ -40,32 +40,15 @@ uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
     return bits;
 }
 
-void Session::PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, uint32_t *waveStart) const
+void Session::PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, int64_t *costPrefix) const
 {
-    std::vector<int64_t> cost(static_cast<size_t>(nq));
-    int64_t all = 0;
+    costPrefix[0] = 0;
     for (int32_t q = 0; q < nq; ++q) {
-        cost[q] = listed[q] ? 0 : taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
-        all += cost[q] * blocks;
+        const int64_t cost = listed[q] ? 0 : taskCost + (postingsPerQuery[q] + blocks - 1) / blocks;
+        costPrefix[q + 1] = costPrefix[q] + cost * blocks;
     }
-    if (all == 0) { // every query takes the live list: empty ranges
-        for (int32_t w = 0; w <= grid; ++w)
-            waveStart[w] = 0;
-        return;
-    }
-    int32_t q = 0;
-    int64_t before = 0; // cost of queries < q
-    for (int32_t w = 0; w <= grid; ++w) {
-        const int64_t target = static_cast<int64_t>((static_cast<__int128>(all) * w) / grid);
-        while (q < nq && before + cost[q] * blocks <= target) {
-            before += cost[q] * blocks;
-            ++q;
-        }
-        int64_t task = static_cast<int64_t>(q) * blocks;
-        if (q < nq)
-            task += std::min<int64_t>(blocks, (target - before + cost[q] - 1) / cost[q]);
-        waveStart[w] = static_cast<uint32_t>(task);
-    }
+    // The kernels multiply the total by wave indices below gridCapacity in 64 bits.
+    Require(costPrefix[nq] < (int64_t(1) << 62) / gridCapacity, "Batch too large for the wave plan");
 }
 
 Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const float *tf, int64_t m,
@@ -107,6 +90,10 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     BM25MI_HIP(hipGetDeviceCount(&count), "count devices");
     Require(device >= 0 && device < count, "device_id out of range");
     BM25MI_HIP(hipSetDevice(device), "set device");
+    // Spin while waiting for a Search (lowest wake-up latency). The runtime refuses the flag once
+    // the device is already active in this process; the Search is then only waited on differently.
+    (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+    (void)hipGetLastError();
     hipDeviceProp_t prop;
     BM25MI_HIP(hipGetDeviceProperties(&prop, device), "device properties");
     cus = prop.multiProcessorCount;
@@ -184,8 +171,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     at = Align(at + (q + 1) * 8, 8);
     offTerms = at;
     at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 16, 16);
-    offWave = at;
-    at = Align(at + (static_cast<size_t>(gridCapacity) + 1) * 4, 256);
+    offCost = at;
+    at = Align(at + (q + 1) * 8, 256);
     packetBytes = at;
     // Result block layout: per final-workgroup counter rows, status, scores, IDs.
     resStatus = q * kStats * 8;
@@ -230,7 +217,7 @@ void Session::InitSlot(Slot &slot)
                         reinterpret_cast<const int64_t *>(pk + offPool),
                         reinterpret_cast<const int64_t *>(pk + offOut),
                         reinterpret_cast<const uint4 *>(pk + offTerms),
-                        reinterpret_cast<const uint32_t *>(pk + offWave)};
+                        reinterpret_cast<const int64_t *>(pk + offCost)};
     auto *rs = static_cast<unsigned char *>(slot.hResultDev);
     slot.wk = WorkView{static_cast<unsigned long long *>(slot.dPool),
                        static_cast<uint32_t *>(slot.dCount),
@@ -388,7 +375,7 @@ void Session::Pack(const Query &query, Slot &slot)
     reinterpret_cast<int32_t *>(pk)[1] = withThr;
     reinterpret_cast<int32_t *>(pk)[2] = nq - withThr;
     slot.mode = withThr == 0 ? 0 : withThr < nq ? 1 : 2;
-    PlanWaves(perQuery.data(), listed.get(), nq, reinterpret_cast<uint32_t *>(pk + offWave));
+    PlanWaves(perQuery.data(), listed.get(), nq, reinterpret_cast<int64_t *>(pk + offCost));
     slot.nq = nq;
     slot.total = total;
 }
