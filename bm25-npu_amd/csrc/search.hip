@@ -560,6 +560,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 for (int r = 0; r < kVec; ++r)
                     acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
             }
+            const uint32_t span = static_cast<uint32_t>(end - base);
             for (uint32_t r0 = 0; r0 < columns; r0 += U) {
                 // Lanes past a column's end keep doc 0xFFFFFFFF, which the range test drops.
                 uint2 e[U];
@@ -583,7 +584,6 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 // segment, its docs are distinct, so all reads may go first (no column can
                 // see another's write).
                 const uint32_t cLast = min(r0 + U, columns) - 1;
-                const uint32_t span = static_cast<uint32_t>(end - base);
                 if (__popcll(__ballot(colEnd <= r0)) == __popcll(__ballot(colEnd <= cLast))) {
                     float v[U];
 #pragma unroll
@@ -1347,9 +1347,17 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
         const int per = (scoreGrid + gridDim.x - 1) / gridDim.x;
         const int r0 = q * per, r1 = min(scoreGrid, r0 + per);
         unsigned long long acc = 0;
-        for (int r = r0 + (tid & 63); r < r1; r += 64) {
-            const unsigned long long v = wk.stats[static_cast<int64_t>(r) * kStats + stat];
-            acc = stat >= kStatMax ? max(acc, v) : acc + v;
+        // Eight rows per lane in flight at a time.
+        for (int r8 = r0 + (tid & 63); r8 < r1; r8 += 64 * 8) {
+            unsigned long long v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int r = r8 + 64 * u;
+                v[u] = r < r1 ? wk.stats[static_cast<int64_t>(r) * kStats + stat] : 0ULL;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                acc = stat >= kStatMax ? max(acc, v[u]) : acc + v[u];
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
