@@ -950,9 +950,28 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // slot | word position << 8.
     __shared__ uint32_t sSeg[3][kWave];
     __shared__ uint32_t sHist[kHistBins]; // one slot's appends per histogram bucket
+#ifdef BM25MI_PHASES
+    // Shader-clock ticks per phase (live masks and slots, words and ranges, streaming, append),
+    // measurement builds only.
+    unsigned long long phase[4] = {0, 0, 0, 0}, stamp = wall_clock64();
+    const unsigned long long waveStart = stamp;
+    int cur = 0;
+#define BM25MI_LPHASE(next)                                                                                            \
+    do {                                                                                                               \
+        const unsigned long long now_ = wall_clock64();                                                                \
+        phase[cur] += now_ - stamp;                                                                                    \
+        stamp = now_;                                                                                                  \
+        cur = (next);                                                                                                  \
+    } while (0)
+#else
+#define BM25MI_LPHASE(next)                                                                                            \
+    do {                                                                                                               \
+    } while (0)
+#endif
     int ns = 0; // slots held (lane s < ns: slot s's query and granule), some carried over
     uint32_t slotQ = 0, slotG = 0;
     for (;;) {
+        BM25MI_LPHASE(0);
         // 1. Up to kLiveSlots live granules.
         while (ns < kLiveSlots) {
             if (curMask == 0) {
@@ -983,6 +1002,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         if (ns == 0)
             break;
 
+        BM25MI_LPHASE(1);
         // 2. The slots' query words and floors (the threshold, at least the start bound and tau).
         const bool isSlot = static_cast<int>(lane) < ns;
         int64_t w0 = 0;
@@ -991,7 +1011,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             w0 = bt.qoff[slotQ];
             nw = static_cast<uint32_t>(bt.qoff[slotQ + 1] - w0);
             kS = static_cast<uint32_t>(bt.k[slotQ]);
-            const uint32_t tauS = __hip_atomic_load(wk.tau + slotQ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // tau only grows, so a stale (workgroup-scope) read is still a lower bound.
+            const uint32_t tauS = __hip_atomic_load(wk.tau + slotQ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             floorS = max(max(max(__float_as_uint(bt.thr[slotQ]), bt.tau0[slotQ]), tauS), 1u);
         }
         uint32_t pairEnd = nw; // inclusive prefix of the slots' word counts
@@ -1141,6 +1162,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         uint32_t totalLen = WaveSum(rLen);
         scanned += totalLen;
 
+        BM25MI_LPHASE(2);
         // 4. Stream the columns, U per round. A column lies within one word position, where the
         // segments are of different granules and hold distinct docs; positions are added in
         // order, so every doc's adds follow its query's word order.
@@ -1205,6 +1227,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         }
         useful += in;
 
+        BM25MI_LPHASE(3);
         // 5. Per slot: append the docs scoring at least its floor to its query's pool, at most
         // min(K, 256) of them: past the granule's K-th key a doc has K docs of its query above
         // it, so the cut keeps exactly the granule's top K and raises the query's bound tau
@@ -1291,7 +1314,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         row[kStatCandidates] = emitted;
         row[kStatEmitting] = emitting;
         row[kStatUseful] = useful;
+#ifdef BM25MI_PHASES
+        BM25MI_LPHASE(0);
+        for (int i = 0; i < 4; ++i)
+            row[kStatPhase + i] = phase[i];
+        row[kStatMax] = phase[0] + phase[1] + phase[2] + phase[3];
+        row[kStatMax + 1] = stamp;
+        row[kStatMax + 2] = ~waveStart;
+        row[kStatMax + 3] = waveStart;
+#endif
     }
+#undef BM25MI_LPHASE
 }
 
 // Initial state of the per-query counters; run once at session creation. After that
