@@ -222,7 +222,7 @@ def test_pruned_path_equals_plain(synth, synth_session):
     corpus, index = synth
     queries = bm25mi.draw_queries(corpus.offsets, 16, 3, seed=77)
     offsets, terms = bm25mi.pack(queries)
-    for k in (10, 100):
+    for k in (10, 100, 1000):
         want_s, want_i = orc.search(index, offsets, terms, k)
         kth = want_s[:, -1].astype(np.float32)
         for factor in (0.5, 1.0, 1.01):
