@@ -122,11 +122,14 @@ int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offsets, const i
                         int32_t queries, const int32_t *k_per_query, float *out_scores, int32_t *out_ids);
 
 /* Threshold-pruned Search (no reference counterpart: the learned path).
- * thresholds[q] is a predicted K-th score. Blocks whose block-max upper bound
- * is below it are skipped; only docs scoring at least the threshold are kept.
- * A query with fewer than K survivors is rerun on the plain path, so results
- * always equal bm25mi_search_batch. fallback_out[q] (optional) is set to 1
- * for each rerun query. */
+ * thresholds[q] is a predicted K-th score (0 = plain). 256-doc granules whose
+ * upper bound (the FP32 sum of the query words' granule maxima) is below it are
+ * skipped; only docs scoring at least the threshold are kept. A query with fewer
+ * than K survivors, or whose candidate pool fills, is rerun on the plain path,
+ * so results always equal bm25mi_search_batch; fallback_out[q] (optional) is
+ * set to 1 for each rerun query. A threshold below 0.3 of the query's score
+ * upper bound (the FP32 sum of its terms' largest impacts) would leave most
+ * granules live: such a query is scored on the plain path directly. */
 int bm25mi_search_pruned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                          int32_t queries, const int32_t *k_per_query, const float *thresholds,
                          float *out_scores, int32_t *out_ids, int32_t *fallback_out);
