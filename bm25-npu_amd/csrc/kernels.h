@@ -28,6 +28,8 @@ constexpr int kStats = 16;
 // i.e. 16 buckets per octave over [2^-4, 2^8) (lower scores fall in bucket 0, higher ones
 // in the last bucket, which only makes the bound it yields smaller).
 constexpr int kHistBins = 192;
+// The scoring kernel reads a query's score histogram every kHistEvery-th task of the query.
+constexpr uint32_t kHistEvery = 16;
 constexpr uint32_t kHistBase = (127u - 4u) << 4;
 
 // Candidates a block may append: all of them when at most this many score at or above the
