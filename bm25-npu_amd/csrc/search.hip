@@ -937,8 +937,12 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
 constexpr int kLiveSlots = 8; // granules a live batch scores together (8 x 256 FP32 = 8 KB of LDS)
 
 // Live kernel of threshold queries (the learned path; no reference counterpart). Every wave
-// walks the tasks blockIdx.x + i * gridDim.x of the task order (q * blocks + blk), 64 live
-// masks per load, and takes their live granules kLiveSlots at a time, from any query. A batch
+// walks the granule positions blockIdx.x + i * gridDim.x of the live masks (position = task *
+// granules per block + granule, task = q * blocks + blk), 64 per round, and takes the live
+// ones kLiveSlots at a time, from any query. Live granules cluster in the blocks that hold a
+// query's top docs; striding over granules rather than blocks spreads a block's live granules
+// over consecutive waves (striding over blocks gave some waves six live blocks of eight live
+// granules while the median wave took about one batch). A batch
 // resolves all its (granule, word) pairs together: one gather of the query words, one load of
 // two directory entries per long word (or a two-level 16-ary search of a short list's
 // samples), then it streams the pairs' postings as segments in (granule, query word) order
@@ -958,11 +962,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     float4 *acc4 = reinterpret_cast<float4 *>(acc);
     const uint32_t lane = __lane_id();
     const int32_t nq = bt.header[0];
-    const uint64_t listEnd = bt.header[1] > 0 ? static_cast<uint64_t>(nq) * static_cast<uint32_t>(blocks) : 0;
+    // Granule positions task * kGpb + bit of the live masks (task = q * blocks + blk).
+    const uint64_t listEnd =
+        bt.header[1] > 0 ? static_cast<uint64_t>(nq) * static_cast<uint32_t>(blocks) * static_cast<uint32_t>(kGpb) : 0;
     const uint64_t listStep = static_cast<uint64_t>(gridDim.x) * kWave;
     uint64_t listAt = blockIdx.x;
-    unsigned long long cand = 0; // lanes of the loaded masks still to take
-    uint32_t maskV = 0, curMask = 0, curTask = 0;
+    unsigned long long cand = 0; // lanes whose loaded granule is live and still to take
+    uint32_t maskV = 0;
     unsigned long long scanned = 0, scored = 0, emitted = 0, emitting = 0, useful = 0;
 
     // Per segment of a batch in position-major order (lane r: segment r): first posting, count,
@@ -991,30 +997,27 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t slotQ = 0, slotG = 0;
     for (;;) {
         BM25MI_LPHASE(0);
-        // 1. Up to kLiveSlots live granules.
+        // 1. Up to kLiveSlots live granules: lane l tests granule position listAt + l * grid.
         while (ns < kLiveSlots) {
-            if (curMask == 0) {
-                while (cand == 0 && listAt < listEnd) {
-                    const uint64_t tl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
-                    maskV = tl < listEnd ? wk.liveMask[tl] : 0u;
-                    cand = __ballot(maskV != 0);
-                    listAt += listStep;
-                }
-                if (cand == 0)
-                    break;
-                const int i = __ffsll(static_cast<long long>(cand)) - 1;
-                cand &= cand - 1;
-                curTask = static_cast<uint32_t>(listAt - listStep + static_cast<uint64_t>(i) * gridDim.x);
-                curMask = ReadLane(maskV, i);
-                ++scored;
+            while (cand == 0 && listAt < listEnd) {
+                const uint64_t gl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
+                maskV = gl < listEnd ? wk.liveMask[gl / kGpb] : 0u;
+                cand = __ballot((maskV >> (gl % kGpb)) & 1u);
+                listAt += listStep;
             }
-            const int b = __ffs(static_cast<int>(curMask)) - 1;
-            curMask &= curMask - 1;
-            const uint32_t q = curTask / static_cast<uint32_t>(blocks);
-            const uint32_t blk = curTask - q * static_cast<uint32_t>(blocks);
+            if (cand == 0)
+                break;
+            const int i = __ffsll(static_cast<long long>(cand)) - 1;
+            cand &= cand - 1;
+            const uint64_t gp = listAt - listStep + static_cast<uint64_t>(i) * gridDim.x;
+            const uint32_t task = static_cast<uint32_t>(gp / kGpb), b = static_cast<uint32_t>(gp % kGpb);
+            // A live block counts once, for its first live granule.
+            scored += (ReadLane(maskV, i) & ((1u << b) - 1u)) == 0 ? 1u : 0u;
+            const uint32_t q = task / static_cast<uint32_t>(blocks);
+            const uint32_t blk = task - q * static_cast<uint32_t>(blocks);
             if (static_cast<int>(lane) == ns) {
                 slotQ = q;
-                slotG = blk * kGpb + static_cast<uint32_t>(b);
+                slotG = blk * kGpb + b;
             }
             ++ns;
         }
