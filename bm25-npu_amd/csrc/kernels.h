@@ -97,6 +97,9 @@ struct WorkView {
     // threshold (0 for dead blocks and for plain queries), and per wave the dead blocks it found.
     uint32_t *liveMask;       // [maxQ * blocks]
     uint32_t *dead;           // [grid]
+    // Also per (query word w of the batch, block) for the short words of threshold queries: the
+    // index of the word's first posting with doc at or after the block's first doc.
+    uint32_t *cursor;         // [maxWords * blocks]
     int32_t *status;          // [maxQ] 0 ok, 1 fewer than K survivors under a threshold
     float *scores;            // [maxQ*maxK]
     int32_t *ids;             // [maxQ*maxK]

@@ -836,6 +836,7 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
     const uint32_t per = (items + gridDim.x - 1) / gridDim.x;
     const uint32_t i0 = min(items, blockIdx.x * per), i1 = min(items, i0 + per);
     int qCur = -1, n = 0;
+    int64_t wq0 = 0; // the query's first word in the batch
     float thrLive = 0.0F;
     uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
     uint32_t dead = 0;
@@ -849,10 +850,10 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
             n = 0;
             info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
             if (thr > 0.0F) {
-                const int64_t w0 = bt.qoff[q];
-                n = static_cast<int>(bt.qoff[q + 1] - w0);
+                wq0 = bt.qoff[q];
+                n = static_cast<int>(bt.qoff[q + 1] - wq0);
                 if (static_cast<int>(lane) < n)
-                    info = bt.words[w0 + lane];
+                    info = bt.words[wq0 + lane];
             }
         }
         const int64_t blk = gBase / kGpb + lane; // lane b < kBpc: block gBase / kGpb + b of this chunk
@@ -911,15 +912,28 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
                 const uint32_t ws = ca > 0 ? ((s0 + ca - 1) << kSampleShift) + 1 : first;
                 const uint32_t we = cb < s1 - s0 ? (s0 + cb) << kSampleShift : first + cnt;
                 unsigned long long m = 0;
-                for (uint32_t p0 = ws; p0 < we; p0 += kWave)
-                    if (p0 + lane < we) {
-                        const uint32_t rel = (ix.post[p0 + lane].x >> kGranuleShift) - static_cast<uint32_t>(gBase);
-                        if (rel < static_cast<uint32_t>(kPruneSpan))
-                            m |= 1ULL << rel;
+                uint32_t below = 0; // lane b < kBpc: postings of [ws, we) before block b of the chunk
+                for (uint32_t p0 = ws; p0 < we; p0 += kWave) {
+                    const bool v = p0 + lane < we;
+                    const uint32_t d = v ? ix.post[p0 + lane].x : 0xFFFFFFFFu;
+                    const uint32_t rel = (d >> kGranuleShift) - static_cast<uint32_t>(gBase);
+                    if (v && rel < static_cast<uint32_t>(kPruneSpan))
+                        m |= 1ULL << rel;
+#pragma unroll
+                    for (int b = 0; b < kBpc; ++b) {
+                        const uint32_t c = static_cast<uint32_t>(
+                            __popcll(__ballot(d < static_cast<uint32_t>(dlo) + static_cast<uint32_t>(b * B))));
+                        below += static_cast<int>(lane) == b ? c : 0u;
                     }
+                }
                 m = WaveOr64(m);
                 if ((m >> lane) & 1ULL)
                     ub += __uint_as_float(ReadLane(info.w, j));
+                // Postings before ws lie below the chunk's first doc and those from we on at or
+                // above its end, so ws + below is the word's first posting in block b: the live
+                // kernel starts its search of the block's granules there.
+                if (static_cast<int>(lane) < kBpc && gBase / kGpb + lane < blocks)
+                    wk.cursor[(wq0 + j) * blocks + gBase / kGpb + lane] = ws + below;
             }
         }
         const unsigned long long liveG = __ballot(valid && ub >= thrLive);
@@ -1066,19 +1080,71 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             const int64_t wBase = __shfl(w0, s);
             const uint32_t g = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), s));
             uint4 wd = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-            if (has)
+            uint32_t cur = 0; // a short word's first posting in the granule's block (planning kernel)
+            if (has) {
                 wd = bt.words[wBase + j];
+                cur = wk.cursor[(wBase + j) * blocks + g / kGpb];
+            }
             uint32_t sLo = 0, sLen = 0;
             const bool isLongW = has && wd.y > 0 && static_cast<int32_t>(wd.z) >= 0;
-            const bool isShortW = has && wd.y > 0 && static_cast<int32_t>(wd.z) < 0;
+            bool isShortW = has && wd.y > 0 && static_cast<int32_t>(wd.z) < 0;
             if (isLongW) {
                 const uint32_t *row = ix.dir + static_cast<int64_t>(static_cast<int32_t>(wd.z)) * (ix.granules + 1) + g;
                 const uint32_t d0 = row[0], d1 = row[1];
                 sLo = wd.x + d0;
                 sLen = d1 - d0;
             }
+            const unsigned long long shortPairs = __ballot(isShortW);
+            if (shortPairs) {
+                // The kWin postings from the cursor, loaded kWin lanes per pair (coalesced), kPer
+                // pairs per round: round r, lane l holds posting l % kWin of pair r * kPer + l / kWin.
+                // They start at the block's first doc; when they reach the granule's end or the
+                // list's end, the granule's postings are exactly those among them in [dlo, dhi).
+                constexpr int kWin = 8, kPer = kWave / kWin;
+                static_assert(kWin * kPer == kWave, "kPer pairs of kWin lanes per round");
+                const uint32_t gDlo = g << kGranuleShift;
+                const uint32_t gDhi = min(gDlo + static_cast<uint32_t>(kGranule), static_cast<uint32_t>(ix.ndocs));
+                const uint32_t wEnd = wd.x + wd.y;
+                constexpr int kFly = 4; // rounds in flight (32 pairs per memory round trip)
+                const int rounds = static_cast<int>((pairs + kPer - 1) / kPer);
+                uint32_t na = 0, nb = 0;
+                bool full = false;
+                for (int r0 = 0; r0 < rounds; r0 += kFly) {
+                    uint32_t dw[kFly];
+#pragma unroll
+                    for (int u = 0; u < kFly; ++u) {
+                        const int pr = (r0 + u) * kPer + static_cast<int>(lane) / kWin;
+                        const uint32_t idx = static_cast<uint32_t>(__shfl(static_cast<int>(cur), pr & (kWave - 1))) + lane % kWin;
+                        const uint32_t pe = static_cast<uint32_t>(__shfl(static_cast<int>(wEnd), pr & (kWave - 1)));
+                        dw[u] = 0xFFFFFFFFu;
+                        if (pr < kWave && ((shortPairs >> pr) & 1ULL) && idx < pe)
+                            dw[u] = ix.post[idx].x;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kFly; ++u) {
+                        const int r = r0 + u;
+                        const int pr = r * kPer + static_cast<int>(lane) / kWin;
+                        const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(gDlo), pr & (kWave - 1)));
+                        const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(gDhi), pr & (kWave - 1)));
+                        const unsigned long long ba = __ballot(dw[u] < lo), bb = __ballot(dw[u] < hi);
+                        const unsigned long long bl = __ballot(lane % kWin == kWin - 1 && dw[u] < hi);
+                        if (static_cast<int>(lane) / kPer == r) {
+                            const int sh = static_cast<int>(lane % kPer) * kWin;
+                            na = static_cast<uint32_t>(__popcll((ba >> sh) & ((1ULL << kWin) - 1ULL)));
+                            nb = static_cast<uint32_t>(__popcll((bb >> sh) & ((1ULL << kWin) - 1ULL)));
+                            full = (bl >> (sh + kWin - 1)) & 1ULL;
+                        }
+                    }
+                }
+                if (isShortW && !full) {
+                    sLo = cur + na;
+                    sLen = nb - na;
+                    isShortW = false; // resolved
+                }
+            }
             if (__ballot(isShortW)) {
-                // The short list's postings of docs [dlo, dhi): count its samples (doc of every
+                // A window that ends below the granule's end (dense lists, small corpora): the
+                // short list's postings of docs [dlo, dhi): count its samples (doc of every
                 // 16th posting, global index) below dlo and dhi with a two-level 16-ary search,
                 // then the postings below each bound among the at most 15 between its bracketing
                 // samples.

@@ -48,7 +48,7 @@ struct Query {
 // counters, result block, stream and captured graph.
 struct Slot {
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr, *dLive = nullptr, *dDead = nullptr;
+    void *dHist = nullptr, *dLive = nullptr, *dDead = nullptr, *dCursor = nullptr;
     void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     BatchView bt{};
     WorkView wk{};

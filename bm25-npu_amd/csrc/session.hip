@@ -116,7 +116,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     const uint64_t known = static_cast<uint64_t>(m) * 8 + static_cast<uint64_t>(m) / 4 + 4 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
                            static_cast<uint64_t>(nslots) * static_cast<uint64_t>(poolEntries) * 8 +
-                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks * 4;
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks * 4 +
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(std::max<int64_t>(1, cfg.max_words)) * blocks * 4;
     // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
     Require(known <= static_cast<uint64_t>(freeBytes) / 5 * 4, "Index, pools and arenas exceed 80% of free HBM");
 
@@ -208,6 +209,8 @@ void Session::InitSlot(Slot &slot)
     slot.dStats = Alloc<unsigned long long>(2 * static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
     slot.dLive = Alloc<uint32_t>(q * static_cast<size_t>(blocks), "alloc live masks");
     slot.dDead = Alloc<uint32_t>(static_cast<size_t>(gridCapacity), "alloc dead counts");
+    slot.dCursor = Alloc<uint32_t>(std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * static_cast<size_t>(blocks),
+                                   "alloc short-list cursors");
     auto *pk = static_cast<unsigned char *>(slot.dPacket);
     slot.bt = BatchView{reinterpret_cast<const int32_t *>(pk),
                         reinterpret_cast<const int64_t *>(pk + offQoff),
@@ -226,6 +229,7 @@ void Session::InitSlot(Slot &slot)
                        static_cast<unsigned long long *>(slot.dStats),
                        static_cast<uint32_t *>(slot.dLive),
                        static_cast<uint32_t *>(slot.dDead),
+                       static_cast<uint32_t *>(slot.dCursor),
                        reinterpret_cast<int32_t *>(rs + resStatus),
                        reinterpret_cast<float *>(rs + resScores),
                        reinterpret_cast<int32_t *>(rs + resIds)};
@@ -555,7 +559,7 @@ void Session::Release() noexcept
                 e = nullptr;
             }
         for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dLive,
-                         &slot.dDead})
+                         &slot.dDead, &slot.dCursor})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
