@@ -3,9 +3,10 @@
 //
 // Mirrors bm25large::Session (FR/ascendhost/src/bm25/large_search.cpp:107-212, FR =
 // feature_retrieval/src/ascendfaiss): the index is validated and uploaded once; every
-// per-Search buffer is allocated up front; H2D packet copy -> kernels is captured into a
-// hipGraph; an empty bootstrap Search must return IDs 0..K-1 with score 0 (:184-187); a
-// request is validated whole before the reusable packet is touched
+// per-Search buffer is allocated up front; H2D packet copy -> kernels is launched directly or,
+// with use_graph, replayed from a captured hipGraph; an empty bootstrap Search must return IDs
+// 0..K-1 with score 0 (:184-187); a request is validated whole before the reusable packet is
+// touched
 // (FR/common/bm25/fixed_query_packet.h:26-49); results are checked the way Session::Receive
 // does (:64-85); a device error marks the session failed for good (:208-210).
 //

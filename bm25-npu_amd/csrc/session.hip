@@ -592,7 +592,7 @@ extern "C" void bm25mi_config_default(bm25mi_config *cfg)
     cfg->max_k = 10;
     cfg->block_docs = 2048;
     cfg->k1 = 1.2;
-    cfg->use_graph = 1;
+    cfg->use_graph = 0;
     cfg->grid_blocks = 0;
     cfg->task_cost = 0;
     cfg->slots = 1;
