@@ -161,6 +161,6 @@ def ptr(array):
     """Raw data address of a C-contiguous numpy array (or None), passed as a void* argument."""
     if array is None:
         return None
-    if not array.flags['C_CONTIGUOUS']:
+    if not array.flags.c_contiguous:
         raise ValueError('array must be C-contiguous')
     return array.ctypes.data

@@ -10,6 +10,7 @@ reference's BM25_QUERY_IDS_V1 text format (standalone/resident_index.py:97-106).
 
 import ctypes as C
 import hashlib
+import itertools
 import json
 from dataclasses import dataclass
 from pathlib import Path
@@ -152,10 +153,9 @@ def geometric_lengths(count, mean, low, high, seed):
 
 def pack(queries):
     """List-of-lists -> (offsets int64[Q+1], terms int32[W])."""
-    offsets = np.zeros(len(queries) + 1, np.int64)
-    offsets[1:] = np.cumsum([len(r) for r in queries])
-    terms = np.fromiter((t for r in queries for t in r), np.int32, count=int(offsets[-1]))
-    return offsets, terms
+    ends = [0]
+    ends.extend(itertools.accumulate(map(len, queries)))
+    return np.array(ends, np.int64), np.array(list(itertools.chain.from_iterable(queries)), np.int32)
 
 
 def write_queries(path, queries):

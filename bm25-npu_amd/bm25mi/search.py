@@ -46,6 +46,7 @@ class Session:
         cfg.slots = slots
         cfg.pool_entries = pool_entries
         self.config = cfg
+        self._kcache = {}
         self.num_docs, self.vocab = index.num_docs, index.vocab
         handle = C.c_void_p()
         _lib.check(self._lib.bm25mi_create(_lib.ptr(index.offsets), index.vocab, _lib.ptr(index.docs),
@@ -77,13 +78,15 @@ class Session:
             pass
 
     @staticmethod
-    def _prepare(queries, k):
+    def _pack(queries):
         if isinstance(queries, tuple):
             offsets, terms = queries
-            offsets = np.ascontiguousarray(offsets, np.int64)
-            terms = np.ascontiguousarray(terms, np.int32)
-        else:
-            offsets, terms = pack(queries)
+            return np.ascontiguousarray(offsets, np.int64), np.ascontiguousarray(terms, np.int32)
+        return pack(queries)
+
+    @classmethod
+    def _prepare(cls, queries, k):
+        offsets, terms = cls._pack(queries)
         nq = len(offsets) - 1
         ks = np.ascontiguousarray(np.full(nq, k, np.int32) if np.isscalar(k) else np.asarray(k, np.int32))
         if len(ks) != nq:
@@ -97,22 +100,38 @@ class Session:
         scores (learned path; results are still exact). Returns (scores, ids,
         fallback) with rows concatenated per query (shape (Q, K) when k is an int).
         """
-        offsets, terms, nq, ks = self._prepare(queries, k)
-        rows = int(ks.sum())
-        scores = np.empty(rows, np.float32)
-        ids = np.empty(rows, np.int32)
-        fallback = np.zeros(nq, np.int32)
-        thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
-        h = self.handle
-        if timed:
-            rc = self._lib.bm25mi_search_timed(h, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
-                                               _lib.ptr(thr), _lib.ptr(scores), _lib.ptr(ids), _lib.ptr(fallback))
-        elif thr is not None:
-            rc = self._lib.bm25mi_search_pruned(h, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
-                                                _lib.ptr(thr), _lib.ptr(scores), _lib.ptr(ids), _lib.ptr(fallback))
+        offsets, terms = self._pack(queries)
+        nq = len(offsets) - 1
+        if np.isscalar(k):
+            # K arrays of scalar-K calls are kept with their addresses (the library only reads them).
+            hit = self._kcache.get((nq, k))
+            if hit is None:
+                arr = np.full(nq, k, np.int32)
+                hit = (arr, _lib.ptr(arr))
+                if len(self._kcache) < 64:
+                    self._kcache[(nq, k)] = hit
+            ks, pk = hit
+            rows = nq * int(k)
         else:
-            rc = self._lib.bm25mi_search_batch(h, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
-                                               _lib.ptr(scores), _lib.ptr(ids))
+            ks = np.ascontiguousarray(np.asarray(k, np.int32))
+            if len(ks) != nq:
+                raise ValueError('one K per query required')
+            pk = _lib.ptr(ks)
+            rows = int(ks.sum())
+        # Scores, IDs and fallback flags share one buffer: one address lookup per call.
+        out = np.zeros(2 * rows + nq, np.int32)
+        base = _lib.ptr(out)
+        scores, ids, fallback = out[:rows].view(np.float32), out[rows:2 * rows], out[2 * rows:]
+        pScores, pIds, pFallback = base, base + 4 * rows, base + 8 * rows
+        pThr = None if thresholds is None else _lib.ptr(np.ascontiguousarray(thresholds, np.float32))
+        h = self.handle
+        po, pt = _lib.ptr(offsets), _lib.ptr(terms)
+        if timed:
+            rc = self._lib.bm25mi_search_timed(h, po, pt, nq, pk, pThr, pScores, pIds, pFallback)
+        elif pThr is not None:
+            rc = self._lib.bm25mi_search_pruned(h, po, pt, nq, pk, pThr, pScores, pIds, pFallback)
+        else:
+            rc = self._lib.bm25mi_search_batch(h, po, pt, nq, pk, pScores, pIds)
         _lib.check(rc, h)
         if np.isscalar(k):
             scores, ids = scores.reshape(nq, int(k)), ids.reshape(nq, int(k))

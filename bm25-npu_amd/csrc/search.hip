@@ -1459,6 +1459,30 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     const int q = blockIdx.x;
     const int32_t nq = bt.header[0];
     const int tid = threadIdx.x;
+    // The query's state, loaded before the counter rows so that all these loads share one
+    // memory round trip (wave 0 also loads the score histogram).
+    const bool active = q < nq;
+    uint32_t k = 1, countQ = 0, tauQ = 1, tau0Q = 1;
+    int64_t poolLo = 0, poolHi = 0;
+    float thrQ = 0.0F;
+    uint32_t hregs[kHistPerLane];
+#pragma unroll
+    for (int i = 0; i < kHistPerLane; ++i)
+        hregs[i] = 0;
+    if (active) {
+        k = static_cast<uint32_t>(bt.k[q]);
+        poolLo = bt.poolOff[q];
+        poolHi = bt.poolOff[q + 1];
+        countQ = wk.count[q];
+        tauQ = wk.tau[q];
+        tau0Q = bt.tau0[q];
+        thrQ = bt.thr[q];
+        if (tid < kWave) {
+#pragma unroll
+            for (int i = 0; i < kHistPerLane; ++i)
+                hregs[i] = wk.hist[static_cast<int64_t>(q) * kHistBins + tid * kHistPerLane + i];
+        }
+    }
     {
         // The scoring kernel has finished: this workgroup sums its share of the per-wave
         // counter rows (64 threads per counter; counters 12..15 take the maximum) into its
@@ -1488,25 +1512,21 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
         if ((tid & 63) == 0)
             hostStats[static_cast<int64_t>(q) * kStats + stat] = acc;
     }
-    if (q >= nq)
+    if (!active)
         return;
-    const uint32_t k = static_cast<uint32_t>(bt.k[q]);
-    const int64_t cap = bt.poolOff[q + 1] - bt.poolOff[q];
-    const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(wk.count[q]), cap));
-    const bool overflowed = static_cast<int64_t>(wk.count[q]) > cap; // only threshold queries (rerun)
+    const int64_t cap = poolHi - poolLo;
+    const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(countQ), cap));
+    const bool overflowed = static_cast<int64_t>(countQ) > cap; // only threshold queries (rerun)
     uint32_t *hist = wk.hist + static_cast<int64_t>(q) * kHistBins;
-    if (tid < 64)
-        sRed[0] = 0; // placeholder write order: the bound below is computed by wave 0
-    __syncthreads();
     if (tid < 64) {
-        const uint32_t hb = WaveHistBound(hist, k);
+        const uint32_t hb = WaveHistBoundRegs(hregs, k);
         if (tid == 0)
-            sRed[0] = max(max(max(wk.tau[q], hb), bt.tau0[q]), 1u);
+            sRed[0] = max(max(max(tauQ, hb), tau0Q), 1u);
     }
     __syncthreads();
     const uint32_t tau = sRed[0];
-    const unsigned long long *pool = wk.pool + bt.poolOff[q];
-    const bool pruned = bt.thr[q] > 0.0F;
+    const unsigned long long *pool = wk.pool + poolLo;
+    const bool pruned = thrQ > 0.0F;
     if (tid == 0)
         sN = 0;
     __syncthreads();
