@@ -845,15 +845,17 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
         const int64_t gBase = static_cast<int64_t>(it - static_cast<uint32_t>(q) * cpq) * kPruneSpan;
         if (q != qCur) {
             qCur = q;
+            // Independent loads, issued together.
             const float thr = bt.thr[q];
+            const int64_t w0 = bt.qoff[q], w1 = bt.qoff[q + 1];
             thrLive = thr * 0.99998474F; // thr * (1 - 2^-16)
             n = 0;
             info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
             if (thr > 0.0F) {
-                wq0 = bt.qoff[q];
-                n = static_cast<int>(bt.qoff[q + 1] - wq0);
+                wq0 = w0;
+                n = static_cast<int>(w1 - w0);
                 if (static_cast<int>(lane) < n)
-                    info = bt.words[wq0 + lane];
+                    info = bt.words[w0 + lane];
             }
         }
         const int64_t blk = gBase / kGpb + lane; // lane b < kBpc: block gBase / kGpb + b of this chunk
@@ -867,8 +869,21 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
         const int64_t g = gBase + lane;
         const bool valid = g < granules;
         float ub = 0.0F;
-        // Long words: eight granule-maximum rows in flight at a time.
+        // Long words: eight granule-maximum rows in flight at a time; the first eight stay in
+        // flight through the short words' searches and are added after them.
         unsigned long long longs = __ballot(static_cast<int>(lane) < n && info.y > 0 && static_cast<int32_t>(info.z) >= 0);
+        float vFirst[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            vFirst[u] = 0.0F;
+            if (longs) {
+                const int j = __ffsll(static_cast<long long>(longs)) - 1;
+                longs &= longs - 1;
+                const int64_t slotRow = static_cast<int64_t>(static_cast<int32_t>(ReadLane(info.z, j))) * granules;
+                if (valid)
+                    vFirst[u] = ix.bmax[slotRow + g];
+            }
+        }
         while (longs) {
             float v[8];
 #pragma unroll
@@ -936,6 +951,9 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
                     wk.cursor[(wq0 + j) * blocks + gBase / kGpb + lane] = ws + below;
             }
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            ub += vFirst[u];
         const unsigned long long liveG = __ballot(valid && ub >= thrLive);
         uint32_t bm = 0;
         if (static_cast<int>(lane) < kBpc)
