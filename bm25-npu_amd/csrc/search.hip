@@ -133,6 +133,22 @@ template <int NT, class T> __device__ void BitonicDesc(T *s, int n)
         }
 }
 
+// The same network over one wave's 64 lanes (one key per lane), in registers: the lower lane of
+// each compare-exchange pair keeps the larger key in a descending run, the smaller otherwise.
+__device__ __forceinline__ unsigned long long WaveBitonicDesc(unsigned long long v)
+{
+    const int lane = static_cast<int>(__lane_id());
+#pragma unroll
+    for (int size = 2; size <= kWave; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const unsigned long long o = __shfl_xor(v, stride);
+            const bool takeMax = ((lane & stride) == 0) == ((lane & size) == 0);
+            v = takeMax ? (v > o ? v : o) : (v < o ? v : o);
+        }
+    return v;
+}
+
 __device__ __forceinline__ uint32_t HistBucket(uint32_t bits)
 {
     const uint32_t b = bits >> 19;
@@ -1572,7 +1588,14 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
         return;                // on the plain path
     }
     const uint32_t m = c < k ? c : k;
-    if (c <= static_cast<uint32_t>(kSortCap)) {
+    if (c <= static_cast<uint32_t>(kWave)) {
+        // Up to 64 survivors (the usual case at small K): one wave sorts them without barriers.
+        if (tid < kWave) {
+            const unsigned long long v = tid < static_cast<int>(c) ? sKeys[tid] : 0ULL;
+            sKeys[tid] = WaveBitonicDesc(v);
+        }
+        __syncthreads();
+    } else if (c <= static_cast<uint32_t>(kSortCap)) {
         int n2 = 1;
         while (n2 < static_cast<int>(c))
             n2 <<= 1;
