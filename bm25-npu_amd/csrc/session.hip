@@ -387,14 +387,17 @@ void Session::Pack(const Query &query, Slot &slot)
 void Session::Launch(Slot &slot, bool timed)
 {
     slot.timed = timed;
-    BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
+    // Device-time events only for timed Searches: the two records cost about 2 us per call.
+    if (timed)
+        BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
     // Mixed batches launch directly: replaying the captured fork/join graph measured slower than
     // the same launches issued on the two streams (Q64 x 2-12 terms: +0.35 ms end to end).
     if (slot.exec[slot.mode] && !timed && slot.mode != 1)
         BM25MI_HIP(hipGraphLaunch(slot.exec[slot.mode], slot.stream), "replay Search graph");
     else
         Enqueue(slot, timed, slot.mode);
-    BM25MI_HIP(hipEventRecord(slot.ev[3], slot.stream), "record end");
+    if (timed)
+        BM25MI_HIP(hipEventRecord(slot.ev[3], slot.stream), "record end");
 }
 
 bool Session::Ready(Slot &slot)
@@ -410,8 +413,9 @@ void Session::Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *s
 {
     BM25MI_HIP(hipStreamSynchronize(slot.stream), "complete Search");
     bm25mi_stats &st = slot.last;
-    float ms = 0;
-    BM25MI_HIP(hipEventElapsedTime(&ms, slot.ev[0], slot.ev[3]), "device time");
+    float ms = -1;
+    if (slot.timed)
+        BM25MI_HIP(hipEventElapsedTime(&ms, slot.ev[0], slot.ev[3]), "device time");
     st.device_ms = ms;
     st.score_kernel_ms = st.final_ms = -1;
     if (slot.timed) {
@@ -668,7 +672,8 @@ static int SearchCommon(bm25mi_session *s, const Query &query, float *scores, in
                         std::memcpy(scores + prefix[q], sc.data() + rowOf[i], query.k[q] * 4);
                         std::memcpy(ids + prefix[q], id.data() + rowOf[i], query.k[q] * 4);
                     }
-                    ss.last.device_ms += first.device_ms;
+                    if (first.device_ms >= 0 && ss.last.device_ms >= 0)
+                        ss.last.device_ms += first.device_ms;
                     if (first.score_kernel_ms >= 0 && ss.last.score_kernel_ms >= 0)
                         ss.last.score_kernel_ms += first.score_kernel_ms;
                     ss.last.postings_scanned += first.postings_scanned;

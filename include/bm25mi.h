@@ -76,7 +76,8 @@ typedef struct {
 
 /* Per-Search counters, filled by bm25mi_last_stats. */
 typedef struct {
-    double device_ms;          /* HIP-event time of the device work of the last Search */
+    double device_ms;          /* HIP-event time of the device work of the last timed Search
+                                  (bm25mi_search_timed); -1 after an untimed one */
     double score_kernel_ms;    /* HIP-event time of the scoring kernel alone (timed Search only, else -1) */
     double host_ms;            /* wall time of the whole bm25mi_search* call */
     int64_t postings_scanned;  /* postings streamed from HBM by the scoring kernel */
