@@ -191,7 +191,9 @@ def test_million_docs_pruned(block, need_gpu, million):
     ks = np.array([10, 100, 1, 37] * 6, np.int32)
     want_s, want_i = orc.search(index, offsets, terms, ks, threads=8)
     kth = want_s[np.cumsum(ks) - 1].astype(np.float32)
-    with bm25mi.Session(index, max_queries=32, max_words=1024, max_k=100, block_docs=block) as s:
+    # use_graph: untimed calls replay the captured graphs, timed calls launch directly.
+    with bm25mi.Session(index, max_queries=32, max_words=1024, max_k=100, block_docs=block,
+                        use_graph=True) as s:
         for factor in (0.5, 0.9, 1.0):
             thr = (kth * np.float32(factor)).astype(np.float32)
             thr[::3] = 0.0
@@ -216,6 +218,28 @@ def million():
     index = corpus.index()
     queries = bm25mi.draw_queries(corpus.offsets, 8, 3, seed=4) + bm25mi.draw_queries(corpus.offsets, 8, 12, seed=5)
     return corpus, index, queries
+
+
+@pytest.mark.parametrize('graph', [True, False])
+def test_all_threshold_batch_launch_modes(graph, need_gpu, million):
+    """A batch whose queries all take the live path (planning and live kernels only): replayed
+    from its captured graph, then launched directly, several times in a row; results equal the
+    oracle bit for bit and no query reruns."""
+    corpus, index, queries = million
+    offsets, terms = bm25mi.pack(queries)
+    want_s, want_i = orc.search(index, offsets, terms, 10, threads=8)
+    kth = want_s[:, -1].astype(np.float32)
+    with bm25mi.Session(index, max_queries=16, max_words=512, max_k=10, use_graph=graph) as s:
+        live = takes_live_path(s, (offsets, terms), 10, kth)
+        keep = np.flatnonzero(live & (kth > 0))
+        assert len(keep) >= 4
+        sub = [queries[i] for i in keep]
+        for rep in range(3):
+            got_s, got_i, fb = s.search(sub, 10, thresholds=kth[keep])
+            assert_same(got_s, got_i, want_s[keep], want_i[keep], f'graph={graph} rep={rep}')
+            assert not fb.any()
+        st = s.stats()
+        assert st['blocks_skipped'] > 0 and st['postings_scanned'] < st['postings_total']
 
 
 def test_pruned_path_equals_plain(synth, synth_session):
