@@ -1041,15 +1041,28 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     do {                                                                                                               \
     } while (0)
 #endif
-    int ns = 0; // slots held (lane s < ns: slot s's query and granule), some carried over
-    uint32_t slotQ = 0, slotG = 0;
+    int ns = 0; // slots held (lane s < ns: slot s's query, granule and word range), some carried over
+    // Word offsets fit 32 bits (a batch holds at most 2^24 words).
+    uint32_t slotQ = 0, slotG = 0, slotW0 = 0, slotNw = 0;
+    uint32_t w0V = 0, nwV = 0; // lane l: the word range of the query of the position it tested
     for (;;) {
         BM25MI_LPHASE(0);
         // 1. Up to kLiveSlots live granules: lane l tests granule position listAt + l * grid.
         while (ns < kLiveSlots) {
             while (cand == 0 && listAt < listEnd) {
                 const uint64_t gl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
-                maskV = gl < listEnd ? wk.liveMask[gl / kGpb] : 0u;
+                maskV = 0u;
+                w0V = 0;
+                nwV = 0u;
+                if (gl < listEnd) {
+                    // With the mask, the word range of the position's query (known from the
+                    // position), so that a taken slot's words can be gathered at once.
+                    const uint32_t ql = static_cast<uint32_t>(gl / kGpb) / static_cast<uint32_t>(blocks);
+                    maskV = wk.liveMask[gl / kGpb];
+                    const int64_t a = bt.qoff[ql], e = bt.qoff[ql + 1];
+                    w0V = static_cast<uint32_t>(a);
+                    nwV = static_cast<uint32_t>(e - a);
+                }
                 cand = __ballot((maskV >> (gl % kGpb)) & 1u);
                 listAt += listStep;
             }
@@ -1063,9 +1076,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             scored += (ReadLane(maskV, i) & ((1u << b) - 1u)) == 0 ? 1u : 0u;
             const uint32_t q = task / static_cast<uint32_t>(blocks);
             const uint32_t blk = task - q * static_cast<uint32_t>(blocks);
+            const uint32_t w0i = ReadLane(w0V, i);
+            const uint32_t nwi = ReadLane(nwV, i);
             if (static_cast<int>(lane) == ns) {
                 slotQ = q;
                 slotG = blk * kGpb + b;
+                slotW0 = w0i;
+                slotNw = nwi;
             }
             ++ns;
         }
@@ -1075,11 +1092,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         BM25MI_LPHASE(1);
         // 2. The slots' query words and floors (the threshold, at least the start bound and tau).
         const bool isSlot = static_cast<int>(lane) < ns;
-        int64_t w0 = 0;
-        uint32_t nw = 0, floorS = 1, kS = 1;
+        const int64_t w0 = isSlot ? static_cast<int64_t>(slotW0) : 0;
+        const uint32_t nw = isSlot ? slotNw : 0u;
+        uint32_t floorS = 1, kS = 1;
         if (isSlot) {
-            w0 = bt.qoff[slotQ];
-            nw = static_cast<uint32_t>(bt.qoff[slotQ + 1] - w0);
             kS = static_cast<uint32_t>(bt.k[slotQ]);
             // tau only grows, so a stale (workgroup-scope) read is still a lower bound.
             const uint32_t tauS = __hip_atomic_load(wk.tau + slotQ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1420,6 +1436,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // Slots [m, ns) carry over to the next batch.
         slotQ = static_cast<uint32_t>(__shfl(static_cast<int>(slotQ), static_cast<int>(min(lane + m, 63u))));
         slotG = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), static_cast<int>(min(lane + m, 63u))));
+        slotW0 = static_cast<uint32_t>(__shfl(static_cast<int>(slotW0), static_cast<int>(min(lane + m, 63u))));
+        slotNw = static_cast<uint32_t>(__shfl(static_cast<int>(slotNw), static_cast<int>(min(lane + m, 63u))));
         ns -= m;
     }
     // This wave's counter row: row gridDim.x + blockIdx.x when the plain kernel runs beside this
