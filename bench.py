@@ -187,15 +187,18 @@ def learned_leg(args, index, offsets):
         fb = sum(int(prune_res[i][2].sum()) for i in (0, 1))
         scan = {'plain': 0, 'learned': 0}
         dev = {'plain': [], 'learned': []}
+        kern = {'plain': [], 'learned': []}  # timed scoring kernels: KScore / KPrune + KLive
         for i in (0, 1):
             s.search(batches[i], k, timed=True)
             st = s.stats()
             scan['plain'] += st['postings_scanned']
             dev['plain'].append(st['device_ms'])
+            kern['plain'].append(st['score_kernel_ms'])
             s.search(batches[i], k, thresholds=thr[i], timed=True)
             st = s.stats()
             scan['learned'] += st['postings_scanned']
             dev['learned'].append(st['device_ms'])
+            kern['learned'].append(st['score_kernel_ms'])
         nq = len(batches[0])
         p, lr = summarize(plain_ms, nq), summarize(prune_ms, nq)
         out['cases'].append({
@@ -205,7 +208,16 @@ def learned_leg(args, index, offsets):
             'plain_device_ms': statistics.mean(dev['plain']), 'learned_device_ms': statistics.mean(dev['learned']),
             'mismatches': mism, 'fallback_rate': fb / (2 * nq),
             'postings_plain': scan['plain'], 'postings_learned': scan['learned'],
-            'bytes_skipped_frac': 1 - scan['learned'] / max(1, scan['plain'])})
+            'bytes_skipped_frac': 1 - scan['learned'] / max(1, scan['plain']),
+            # HBM roofline of the scoring kernels: 8 B per posting streamed (the two A/B batches)
+            # over their timed duration; 'effective' counts the plain path's postings instead.
+            'roofline': {
+                'plain_kernels_ms': statistics.mean(kern['plain']),
+                'learned_kernels_ms': statistics.mean(kern['learned']),
+                'plain_frac': 8 * scan['plain'] / 2 / (statistics.mean(kern['plain']) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                'learned_frac': 8 * scan['learned'] / 2 / (statistics.mean(kern['learned']) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                'learned_effective_frac': 8 * scan['plain'] / 2 / (statistics.mean(kern['learned']) * 1e-3) / 1e9
+                                          / HBM_PEAK_GBS}})
 
     run_case('Q8x3 K10 (headline batches)', [bm25mi.draw_queries(offsets, 8, 3, seed=1),
                                             bm25mi.draw_queries(offsets, 8, 3, seed=2)], 10)
