@@ -96,7 +96,7 @@ struct WorkView {
     // (q * blocks + blk) the mask of the block's granules whose upper bound reaches the query's
     // threshold (0 for dead blocks and for plain queries), and per wave the dead blocks it found.
     uint32_t *liveMask;       // [maxQ * blocks]
-    uint32_t *dead;           // [grid]
+    uint32_t *dead;           // [2 * grid] (the planning kernel runs with twice the scoring grid)
     // Also per (query word w of the batch, block) for the short words of threshold queries: the
     // index of the word's first posting with doc at or after the block's first doc.
     uint32_t *cursor;         // [maxWords * blocks]

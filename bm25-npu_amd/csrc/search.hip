@@ -836,7 +836,9 @@ constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
 // in another order, which the factor (1 - 2^-16) on the threshold covers for up to 64 words.
 // A block with no granule reaching the threshold cannot hold a doc scoring at least it. The
 // kernel writes every block's mask of live granules (0 for dead blocks and for the blocks of
-// plain queries) and, per wave, the number of dead blocks; it runs with the scoring grid.
+// plain queries) and, per wave, the number of dead blocks. It runs with twice the scoring grid
+// (it needs 46 VGPRs and no LDS, so all its waves are resident): at Q8 every wave then takes at
+// most one item instead of two in a row.
 template <int B>
 __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
 {
@@ -1450,7 +1452,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             row[i] = 0;
         row[kStatPostings] = scanned;
         row[kStatScored] = scored;
-        row[kStatSkipped] = listEnd ? wk.dead[blockIdx.x] : 0u;
+        // The planning kernel ran with twice this grid: waves b and b + grid of it.
+        row[kStatSkipped] = listEnd ? wk.dead[blockIdx.x] + wk.dead[blockIdx.x + gridDim.x] : 0u;
         row[kStatCandidates] = emitted;
         row[kStatEmitting] = emitting;
         row[kStatUseful] = useful;

@@ -208,7 +208,7 @@ void Session::InitSlot(Slot &slot)
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
     slot.dStats = Alloc<unsigned long long>(2 * static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
     slot.dLive = Alloc<uint32_t>(q * static_cast<size_t>(blocks), "alloc live masks");
-    slot.dDead = Alloc<uint32_t>(static_cast<size_t>(gridCapacity), "alloc dead counts");
+    slot.dDead = Alloc<uint32_t>(2 * static_cast<size_t>(gridCapacity), "alloc dead counts");
     slot.dCursor = Alloc<uint32_t>(std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * static_cast<size_t>(blocks),
                                    "alloc short-list cursors");
     auto *pk = static_cast<unsigned char *>(slot.dPacket);
@@ -248,7 +248,7 @@ void Session::Enqueue(Slot &slot, bool timed, int mode)
     if (timed && mode != 0)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
     if (mode != 0)
-        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, 2 * grid, slot.stream), "Unsupported block size");
     if (mode == 1) {
         // Plain and threshold queries: the live kernel runs beside the plain one on the side
         // stream (they touch different queries' pools and counters and separate counter rows).
