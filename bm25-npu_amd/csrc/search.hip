@@ -1005,7 +1005,7 @@ template <int B>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KLive(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
 {
     constexpr int kGpb = B / kGranule;
-    constexpr int U = kColumns;
+    constexpr int U = kLiveColumns;
     constexpr int kLevel = 16; // samples per level of the short-list search (16 x 16 >= 4096 / 16)
     static_assert(kLevel * kLevel >= (kLongMin >> kSampleShift), "short lists must fit two levels");
     __shared__ __align__(16) float acc[kLiveSlots * kGranule];
