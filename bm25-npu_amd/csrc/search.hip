@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // issued after it, so every read stalls the task's first column round (measured on the
         // 10M-doc Q8 x 3-term K10 Search: 97 us reading every task, 94 us every 8th task and
         // after each append, 84 us every 16th task, with 3.4K instead of 1.1K candidates).
-        const bool readHist = histAge >= kHistEvery;
+        const bool readHist = histAge >= (k > kHistLargeK ? kHistEveryLargeK : kHistEvery);
         uint32_t hregs[kHistPerLane];
 #pragma unroll
         for (int i = 0; i < kHistPerLane; ++i)
