@@ -19,6 +19,9 @@ constexpr int kTaskCost = 1024;            // estimated fixed cost of a scoring 
 // FP32 sum of its terms' largest impacts (below it, most granules stay live; measured on the
 // 10M-doc corpus: 0.64 of the postings streamed under 0.3, 0.13-0.30 above it).
 constexpr float kLiveMinRatio = 0.3F;
+// ... and only when its K is at most kLiveMaxK: at K = 100 the live path (a cut of up to K docs per
+// live granule) measured 0.92-1.0x of the plain path on the 10M-doc corpus.
+constexpr int32_t kLiveMaxK = 64;
 #ifndef BM25MI_COLUMNS
 #define BM25MI_COLUMNS 14
 #endif

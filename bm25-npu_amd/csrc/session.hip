@@ -315,12 +315,13 @@ void Session::Pack(const Query &query, Slot &slot)
     for (int64_t w = 0; w < words; ++w)
         Require(query.terms[w] >= 0 && query.terms[w] < INT32_MAX, "Invalid public word ID");
     // A threshold below kLiveMinRatio of the query's score upper bound (the FP32 sum of its
-    // terms' largest impacts) leaves most granules live: such a query is scored on the plain
-    // path, with the same results.
+    // terms' largest impacts) leaves most granules live, and above K = kLiveMaxK the live path's
+    // per-granule cut costs more than it saves: such a query is scored on the plain path, with
+    // the same results.
     std::vector<float> thrUse(static_cast<size_t>(nq), 0.0F);
     int64_t poolNeed = 0;
     for (int32_t q = 0; q < nq; ++q) {
-        if (query.thr && query.thr[q] > 0.0F) {
+        if (query.thr && query.thr[q] > 0.0F && query.k[q] <= kLiveMaxK) {
             float ub = 0.0F;
             for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w)
                 if (query.terms[w] < vocab)

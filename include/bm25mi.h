@@ -132,7 +132,8 @@ int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offsets, const i
  * so results always equal bm25mi_search_batch; fallback_out[q] (optional) is
  * set to 1 for each rerun query. A threshold below 0.3 of the query's score
  * upper bound (the FP32 sum of its terms' largest impacts) would leave most
- * granules live: such a query is scored on the plain path directly. */
+ * granules live, and a query with K above 64 gains nothing from the live path:
+ * such queries are scored on the plain path directly. */
 int bm25mi_search_pruned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                          int32_t queries, const int32_t *k_per_query, const float *thresholds,
                          float *out_scores, int32_t *out_ids, int32_t *fallback_out);

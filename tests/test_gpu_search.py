@@ -28,9 +28,11 @@ def assert_same(got_scores, got_ids, want_scores, want_ids, label=''):
 
 def takes_live_path(session, queries, k, thr):
     """Threshold queries the library scores on the live path: a threshold of at least 0.3 of
-    the FP32 sum of the query's term maxima (the others are scored on the plain path)."""
+    the FP32 sum of the query's term maxima and K at most 64 (the others are scored on the
+    plain path)."""
     _, upper = session.bounds(queries, k)
-    return (thr > 0) & (thr >= np.float32(0.3) * upper)
+    ks = np.broadcast_to(np.asarray(k), np.shape(thr))
+    return (thr > 0) & (thr >= np.float32(0.3) * upper) & (ks <= 64)
 
 
 @pytest.fixture(scope='module')

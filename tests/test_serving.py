@@ -122,7 +122,8 @@ def test_pruned_requests_fall_back_and_match_plain(corpus, need_gpu):
             assert np.array_equal(r['scores'].view(np.uint32), want_s.view(np.uint32))
             c = srv.counters()
             # Requests scored on the live path (threshold at least 0.3 of the FP32 sum of the
-            # query's term maxima; the library scores the others on the plain path) rerun.
+            # query's term maxima, K at most 64; the library scores the others on the plain
+            # path) rerun.
             _, upper = s.bounds(queries, ks)
-            live = thr >= np.float32(0.3) * upper
+            live = (thr >= np.float32(0.3) * upper) & (np.asarray(ks) <= 64)
             assert c['fallbacks'] >= int(((factor > 1) & (kth > 0) & live).sum()) > 0
