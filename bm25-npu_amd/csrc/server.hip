@@ -74,8 +74,11 @@ public:
         Require(n == 0 || terms, "Null query terms");
         Require(k >= 1 && k <= session_.cfg.max_k && k <= session_.ndocs, "K out of range");
         Require(session_.PoolCapacity(k) <= session_.poolEntries, "K needs more pool memory than a slot holds");
-        if (session_.PoolCapacity(k, true) > session_.poolEntries)
-            thr = 0.0F; // a threshold pool would not fit a slot: the request runs on the plain path
+        // A request the session would score on the plain path anyway (K above kLiveMaxK), or
+        // whose threshold pool would not fit a slot, is queued as a plain request: admission
+        // then reserves only the plain pool for it.
+        if (k > kLiveMaxK || session_.PoolCapacity(k, true) > session_.poolEntries)
+            thr = 0.0F;
         for (int32_t i = 0; i < n; ++i)
             Require(terms[i] >= 0 && terms[i] < INT32_MAX, "Invalid public word ID");
         auto req = std::make_unique<Request>();
