@@ -331,7 +331,7 @@ def main():
     ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: headline leg only')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)',
+        print(json.dumps({'error': 'the library runs on one GPU; sharding a Search across GPUs is not implemented',
                           'n_gpus': args.gpus}))
         sys.exit(2)
 

@@ -45,7 +45,7 @@ enum {
     BM25MI_ERR_INNER = 3              /* DeviceError/other -> APP_ERR_INNER_ERROR */
 };
 
-/* Hard limits of this build. */
+/* Hard limits of the library. */
 #define BM25MI_MAX_K 1024            /* reference: 32 (FR/common/bm25/large_search_plan.h:34) */
 #define BM25MI_MAX_TERMS_PER_QUERY 64 /* FR/common/bm25/fixed_query_packet.h:32 */
 #define BM25MI_MAX_QUERIES 10000      /* fixed_query_packet.h:28 */

@@ -17,7 +17,7 @@
 //     the HIP error of a device failure, the message is cut to 1023 characters and cleared
 //     by every successful call.
 // Status classes follow FR/ascendhost/include/impl/Bm25ErrorAdapter.h:11-45. The numeric values
-// are this library's (bm25mi.h), since IndexSDK's ErrorCode.h is not part of this build.
+// are this library's (bm25mi.h), since IndexSDK's ErrorCode.h is not part of this library.
 // Differences by design: no SDK context/resources/vendor directory (the HIP device is named by
 // deviceId); topK may be up to 1024 (reference: 32); k1 and the scoring block size are config.
 #ifndef BM25MI_INDEX_HPP
