@@ -105,6 +105,16 @@ def test_query_files_and_packing():
     assert qa[0] == [783, 343, 370]
     offsets, terms = bm25mi.pack(qa + [[], [5, 5]])
     assert offsets.tolist()[-3:] == [24, 24, 26] and terms.dtype == np.int32
+    # Edge cases of the packing: no queries, empty rows only, numpy rows, and IDs that do not
+    # fit int32 (rejected rather than wrapped).
+    offsets, terms = bm25mi.pack([])
+    assert offsets.tolist() == [0] and offsets.dtype == np.int64 and terms.shape == (0,)
+    offsets, terms = bm25mi.pack([[], []])
+    assert offsets.tolist() == [0, 0, 0] and terms.shape == (0,) and terms.dtype == np.int32
+    offsets, terms = bm25mi.pack([np.array([7, 8], np.int64), [9]])
+    assert offsets.tolist() == [0, 2, 3] and terms.tolist() == [7, 8, 9]
+    with pytest.raises(OverflowError):
+        bm25mi.pack([[2**31]])
 
 
 def test_query_file_round_trip(tmp_path):
