@@ -184,6 +184,18 @@ def test_last_error_truncates_to_1023_chars():
     assert len(huge[:1023]) == 1023
 
 
+def test_config_defaults():
+    """Documented defaults of bm25mi_config_default: direct launches (no graph replay), 2048-doc
+    tiles, the full grid and the default task cost chosen by the library."""
+    import ctypes as C
+    from bm25mi import _lib
+    cfg = _lib.Config()
+    _lib.load().bm25mi_config_default(C.byref(cfg))
+    assert cfg.use_graph == 0 and cfg.block_docs == 2048
+    assert cfg.grid_blocks == 0 and cfg.task_cost == 0 and cfg.slots == 1 and cfg.pool_entries == 0
+    assert cfg.max_queries == 8 and cfg.max_words == 24 and cfg.max_k == 10 and cfg.k1 == 1.2
+
+
 def test_mlp_param_count():
     lib = _lib.load()
     assert lib.bm25mi_mlp_param_count(16) == 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1
