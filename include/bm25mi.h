@@ -192,7 +192,7 @@ int bm25mi_server_start(bm25mi_session *s, int32_t max_batch, bm25mi_server **ou
 int bm25mi_server_submit(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, uint64_t *ticket);
 /* Same with a pruning threshold (a predicted K-th score, as bm25mi_search_pruned): a query
  * that keeps fewer than K docs at or above it is rerun as a plain query, so the result is
- * the same. */
+ * the same. A request with K above 64 is queued as a plain one. */
 int bm25mi_server_submit_pruned(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, float threshold,
                                 uint64_t *ticket);
 /* Blocks until the query is done; writes its K scores and doc IDs (either may be NULL) and
