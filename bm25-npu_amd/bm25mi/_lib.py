@@ -99,6 +99,7 @@ SIGNATURES = {
     'bm25mi_last_stats': (C.c_int, [P, P]),
     'bm25mi_destroy': (C.c_int, [P]),
     'bm25mi_last_error': (i64, [P, C.c_char_p, C.c_size_t]),
+    'bm25mi_inject_failure': (C.c_int, [C.c_char_p, i64, C.c_char_p]),
     'bm25mi_device_info': (C.c_int, [i32, P, P, P]),
     'bm25mi_corpus_generate': (C.c_int, [P, P]),
     'bm25mi_corpus_view': (C.c_int, [P, P, P, P, P, P, P, P]),

@@ -123,7 +123,12 @@ class Session:
         base = _lib.ptr(out)
         scores, ids, fallback = out[:rows].view(np.float32), out[rows:2 * rows], out[2 * rows:]
         pScores, pIds, pFallback = base, base + 4 * rows, base + 8 * rows
-        pThr = None if thresholds is None else _lib.ptr(np.ascontiguousarray(thresholds, np.float32))
+        # The converted thresholds stay bound to a local until the call returns: the library
+        # reads them (twice: checks, then packing) while ctypes has released the GIL.
+        thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
+        if thr is not None and thr.shape != (nq,):
+            raise ValueError('one threshold per query required')
+        pThr = _lib.ptr(thr)
         h = self.handle
         po, pt = _lib.ptr(offsets), _lib.ptr(terms)
         if timed:
@@ -164,14 +169,44 @@ class Session:
         return st.as_dict()
 
 
+def check_request(queries, max_queries, max_words):
+    """Request checks of the reference's raw-ID wrapper (standalone/run_bm25s_index_search.py:38-46):
+    a list of 1..max_queries rows, each a list; at most max_words words in total; every word an
+    int (bool rejected) in [0, INT32_MAX-1]. Raises ValueError."""
+    if not (type(queries) is list and 1 <= len(queries) <= max_queries):
+        raise ValueError('Query batch is invalid or exceeds declared capacity')
+    total = 0
+    for row in queries:
+        if not (type(row) is list and len(row) <= max_words - total):
+            raise ValueError('Query words exceed declared capacity')
+        for word in row:
+            if not _is_int(word, 0, LIMIT - 1):
+                raise ValueError('Word IDs must be non-negative INT32 integers (bool rejected)')
+        total += len(row)
+
+
+def check_response(scores, ids, queries, k, num_docs):
+    """Response checks of the same wrapper (:56-75) under strict ranking: shape (Q, K), float32
+    scores and int32 IDs, finite non-negative scores, IDs in [0, num_docs) and unique per row,
+    scores descending, IDs ascending among equal scores. Raises RuntimeError."""
+    if not (scores.shape == ids.shape == (len(queries), k) and scores.dtype == np.float32
+            and ids.dtype == np.int32 and np.isfinite(scores).all() and (scores >= 0).all()
+            and (ids >= 0).all() and (ids < num_docs).all()):
+        raise RuntimeError('Search output shape/dtype/score/id invalid')
+    for s, i in zip(scores, ids):
+        if len(np.unique(i)) != k or not (s[:-1] >= s[1:]).all() \
+                or ((s[:-1] == s[1:]) & (i[:-1] >= i[1:])).any():
+            raise RuntimeError('Search returned duplicate IDs or broke the strict order')
+
+
 class SearchSession:
     """Search(query term IDs) -> (scores f32[Q,K], ids i32[Q,K]), K fixed per session.
 
-    Request checks follow standalone/run_bm25s_index_search.py:38-46 (a list of
-    1..max_queries rows, each a list; total words <= max_words; every word an int,
-    not bool, in [0, INT32_MAX-1]); response checks follow :56-75 (shape, dtype,
-    finite non-negative scores, in-range unique IDs, descending scores, ascending
-    IDs among equal scores).
+    The Python raw-ID Search surface of standalone/run_bm25s_index_search.py:15-76: the
+    configuration checks of its constructor (:21-29, with K up to the library's 1024 instead of
+    32), the request checks of check_request and the response checks of check_response, around a
+    resident-index Session. Term IDs at or above the vocabulary are skipped (no empty-token
+    column is needed: an empty query returns IDs 0..K-1 with score 0).
     """
 
     def __init__(self, index, k, max_queries=8, max_words=24, **kwargs):
@@ -183,29 +218,19 @@ class SearchSession:
         self.session = Session(index, max_queries=max_queries, max_words=max(1, max_words), max_k=k, **kwargs)
 
     def search(self, queries):
-        if not (type(queries) is list and 1 <= len(queries) <= self.maxq):
-            raise ValueError('Query batch is invalid or exceeds declared capacity')
-        total = 0
-        for row in queries:
-            if not (type(row) is list and len(row) <= self.maxwords - total):
-                raise ValueError('Query words exceed declared capacity')
-            for word in row:
-                if not _is_int(word, 0, LIMIT - 1):
-                    raise ValueError('Word IDs must be non-negative INT32 integers (bool rejected)')
-            total += len(row)
+        check_request(queries, self.maxq, self.maxwords)
         scores, ids, _ = self.session.search(queries, self.k)
-        if not (scores.shape == ids.shape == (len(queries), self.k) and scores.dtype == np.float32
-                and ids.dtype == np.int32 and np.isfinite(scores).all() and (scores >= 0).all()
-                and (ids >= 0).all() and (ids < self.num_docs).all()):
-            raise RuntimeError('Search output shape/dtype/score/id invalid')
-        for s, i in zip(scores, ids):
-            if len(np.unique(i)) != self.k or not (s[:-1] >= s[1:]).all() \
-                    or ((s[:-1] == s[1:]) & (i[:-1] >= i[1:])).any():
-                raise RuntimeError('Search returned duplicate IDs or broke the strict order')
+        check_response(scores, ids, queries, self.k, self.num_docs)
         return scores, ids
 
     def close(self):
         self.session.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # APP_ERR classes of the facade (Bm25ErrorAdapter.h:11-45).

@@ -62,6 +62,10 @@ class Server:
         ks = np.ascontiguousarray(ks, np.int32)
         thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
         n = len(ks)
+        if ks.ndim != 1 or offsets.shape != (n + 1,) or (thr is not None and thr.shape != (n,)):
+            raise ValueError('offsets, ks and thresholds must describe the same queries')
+        if n and (offsets[0] != 0 or offsets[-1] > len(terms) or (np.diff(offsets) < 0).any()):
+            raise ValueError('query offsets must start at 0, ascend and stay within the terms')
         lat = np.empty(n, np.float64)
         rows = int(ks.sum())
         scores = np.empty(rows, np.float32) if keep_results else None

@@ -64,6 +64,11 @@ public:
         work_.notify_all();
         if (worker_.joinable())
             worker_.join();
+        // Threads still inside Wait (woken by the last results) leave before the members go.
+        {
+            std::unique_lock<std::mutex> lock(mutex_);
+            idle_.wait(lock, [&] { return waiters_ == 0; });
+        }
         session_.serving = false;
     }
 
@@ -98,20 +103,29 @@ public:
         return ticket;
     }
 
-    // Blocks until the request is done; copies its K rows and removes it.
+    // Blocks until the request is done; copies its K rows and removes it. The server is not
+    // destroyed while a thread is inside (waiters_): the last member it touches is idle_,
+    // notified with the lock held, before it copies the rows it now owns.
     void Wait(uint64_t ticket, float *scores, int32_t *ids, double *latencyUs)
     {
-        std::unique_lock<std::mutex> lock(mutex_);
-        auto it = requests_.find(ticket);
-        Require(it != requests_.end() && !it->second->collected, "Unknown or already collected ticket");
-        // Wait on the request itself: map iterators do not survive inserts by Submit.
-        Request *pending = it->second.get();
-        pending->collected = true;
-        done_.wait(lock, [&] { return pending->done; });
-        auto at = requests_.find(ticket);
-        std::unique_ptr<Request> req = std::move(at->second);
-        requests_.erase(at);
-        lock.unlock();
+        std::unique_ptr<Request> req;
+        {
+            std::unique_lock<std::mutex> lock(mutex_);
+            auto it = requests_.find(ticket);
+            Require(it != requests_.end() && !it->second->collected, "Unknown or already collected ticket");
+            // Wait on the request itself: map iterators do not survive inserts by Submit.
+            Request *pending = it->second.get();
+            pending->collected = true;
+            ++waiters_;
+            done_.wait(lock, [&] { return pending->done; });
+            auto at = requests_.find(ticket);
+            req = std::move(at->second);
+            requests_.erase(at);
+            if (--waiters_ == 0)
+                idle_.notify_all();
+        }
+        if (req->status == BM25MI_ERR_INVALID_PARAM)
+            throw InvalidArgument(req->error);
         if (req->status != BM25MI_OK)
             throw DeviceError(0, req->error);
         if (scores)
@@ -249,6 +263,10 @@ private:
                         ++batches_;
                         queries_ += static_cast<int64_t>(ks.size());
                         maxInflight_ = std::max<int32_t>(maxInflight_, static_cast<int32_t>(inflight.size()));
+                    } catch (const InvalidArgument &e) {
+                        // A request the session refuses fails alone; the server keeps serving.
+                        free.push_back(slot);
+                        Fail(f, e.what(), false);
                     } catch (const std::exception &e) {
                         free.push_back(slot);
                         Fail(f, e.what());
@@ -292,21 +310,26 @@ private:
         }
     }
 
-    void Fail(InFlight &f, const char *what)
+    // Fails the batch's requests. A device failure (device = true) also ends the server and the
+    // session (large_search.cpp:208-210); a rejected request only fails its batch.
+    void Fail(InFlight &f, const char *what, bool device = true)
     {
-        {
-            std::lock_guard<std::mutex> g(mutex_);
-            failed_ = true; // a device failure ends the session (large_search.cpp:208-210)
+        if (device) {
+            {
+                std::lock_guard<std::mutex> g(mutex_);
+                failed_ = true;
+            }
+            session_.failed = true;
         }
-        session_.failed = true;
-        Finish(f, {}, {}, nullptr, BM25MI_ERR_INNER, what);
+        Finish(f, {}, {}, nullptr, device ? BM25MI_ERR_INNER : BM25MI_ERR_INVALID_PARAM, what);
         done_.notify_all();
     }
 
     Session &session_;
     const int32_t maxBatch_;
     std::mutex mutex_;
-    std::condition_variable work_, done_;
+    std::condition_variable work_, done_, idle_;
+    int32_t waiters_ = 0; // threads inside Wait
     std::deque<uint64_t> queue_;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> requests_;
     uint64_t next_ = 1;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -18,6 +19,47 @@
 namespace bm25mi {
 
 static size_t Align(size_t n, size_t a) { return (n + a - 1) / a * a; }
+
+// One-shot failures injected by bm25mi_inject_failure, per stage: the next call that reaches
+// the stage throws a DeviceError with the injected code and message (the reference injects the
+// same failures through mock ACL functions: FR/ut/cases/bm25/sdk_facade_tests.cpp:53-62).
+namespace {
+struct Injected {
+    bool pending = false;
+    int64_t code = 0;
+    std::string message;
+};
+std::mutex injectMutex;
+constexpr int kStages = 4;
+Injected injected[kStages]; // create, search, close, result
+int StageIndex(const char *stage)
+{
+    static const char *names[kStages] = {"create", "search", "close", "result"};
+    for (int i = 0; i < kStages; ++i)
+        if (std::strcmp(stage, names[i]) == 0)
+            return i;
+    return -1;
+}
+// Takes a pending injection of the stage, if any.
+bool TakeInjected(const char *stage)
+{
+    std::lock_guard<std::mutex> g(injectMutex);
+    Injected &f = injected[StageIndex(stage)];
+    const bool pending = f.pending;
+    f.pending = false;
+    return pending;
+}
+} // namespace
+
+static void FailIfInjected(const char *stage)
+{
+    std::lock_guard<std::mutex> g(injectMutex);
+    Injected &f = injected[StageIndex(stage)];
+    if (!f.pending)
+        return;
+    f.pending = false;
+    throw DeviceError(f.code, f.message);
+}
 
 int64_t Session::PoolCapacity(int32_t k, bool threshold) const
 {
@@ -438,6 +480,17 @@ void Session::Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *s
         throw DeviceError(0, "Candidate pool overflow");
     int32_t fallbacks = 0;
     const auto &out = slot.out;
+    if (TakeInjected("result") && slot.nq > 0 && !stq[0]) {
+        // Injected corruption of the first query's rows, for the checks below to reject.
+        auto *idw = reinterpret_cast<int32_t *>(res + resIds);
+        auto *scw = reinterpret_cast<float *>(res + resScores);
+        if (out[1] - out[0] >= 2) {
+            std::swap(idw[0], idw[1]);
+            std::swap(scw[0], scw[1]);
+        } else {
+            idw[0] = static_cast<int32_t>(ndocs);
+        }
+    }
     for (int32_t q = 0; q < slot.nq; ++q) {
         status[q] = stq[q];
         fallbacks += stq[q] ? 1 : 0;
@@ -478,6 +531,7 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
     Slot &slot = slots[0];
     Pack(query, slot); // input errors leave the session usable
     try {
+        FailIfInjected("search");
         Launch(slot, timed);
         Complete(slot, outScores, outIds, status);
         last = slot.last;
@@ -615,6 +669,7 @@ extern "C" int bm25mi_create(const int64_t *term_offsets, int64_t vocab, const i
         bm25mi_config_default(&c);
         if (cfg)
             c = *cfg;
+        FailIfInjected("create");
         auto s = std::make_unique<bm25mi_session>();
         s->impl = std::make_unique<Session>(term_offsets, vocab, doc_ids, tf, postings, doc_norm, docs, idf, c);
         *out = s.release();
@@ -728,6 +783,24 @@ extern "C" int bm25mi_search_timed(bm25mi_session *s, const int64_t *query_offse
     return SearchCommon(s, q, out_scores, out_ids, fallback_out, thresholds != nullptr, true);
 }
 
+// Request checks shared by the per-query statistics entry points (features, bounds): the rules
+// of Session::Pack that do not depend on the session's capacity (offsets start at 0 and ascend,
+// at most 64 terms per query, term IDs in [0, INT32_MAX), K >= 1). They read only tables that
+// are fixed at create time, so they may run while a server owns the session.
+static void CheckQueries(const int64_t *offsets, const int32_t *terms, int32_t queries, const int32_t *ks)
+{
+    Require(queries >= 1 && offsets && ks, "Null query offsets or K array");
+    Require(offsets[0] == 0, "query_offsets[0] must be 0");
+    for (int32_t q = 0; q < queries; ++q) {
+        const int64_t len = offsets[q + 1] - offsets[q];
+        Require(len >= 0 && len <= kMaxTerms, "Query exceeds the 64-term limit or has negative length");
+        Require(ks[q] >= 1, "K out of range");
+    }
+    Require(offsets[queries] == 0 || terms, "Null query terms");
+    for (int64_t w = 0; w < offsets[queries]; ++w)
+        Require(terms[w] >= 0 && terms[w] < INT32_MAX, "Invalid public word ID");
+}
+
 // The K-th largest term impact among the query's terms (a lower bound of the K-th score).
 static double LowerBound(const Session &ss, const int32_t *terms, int64_t n, int32_t k)
 {
@@ -748,17 +821,16 @@ extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_off
     return Invoke(
         [&] {
             Require(s->impl && !s->impl->closed, "Session closed");
-            Require(query_offsets && k_per_query && out_features && queries >= 1, "Null feature arguments");
+            Require(out_features != nullptr, "Null feature output");
+            CheckQueries(query_offsets, query_terms, queries, k_per_query);
             const Session &ss = *s->impl;
             const double n = static_cast<double>(ss.ndocs);
             for (int32_t q = 0; q < queries; ++q) {
-                Require(k_per_query[q] >= 1, "K out of range");
                 std::vector<double> m;
                 double sumMean = 0, sumIdf = 0, maxIdf = 0, sumDf = 0, maxDf = 0, minDf = -1, density = 0,
                        logAll = 0;
                 for (int64_t w = query_offsets[q]; w < query_offsets[q + 1]; ++w) {
                     const int32_t t = query_terms[w];
-                    Require(t >= 0, "Invalid public word ID");
                     if (t >= ss.vocab)
                         continue;
                     const double df = static_cast<double>(ss.toffHost[t + 1] - ss.toffHost[t]);
@@ -801,15 +873,12 @@ extern "C" int bm25mi_query_bounds(bm25mi_session *s, const int64_t *query_offse
     return Invoke(
         [&] {
             Require(s->impl && !s->impl->closed, "Session closed");
-            Require(query_offsets && k_per_query && queries >= 1 && (out_lower || out_upper), "Null bound arguments");
+            Require(out_lower || out_upper, "Null bound outputs");
+            CheckQueries(query_offsets, query_terms, queries, k_per_query);
             const Session &ss = *s->impl;
             for (int32_t q = 0; q < queries; ++q) {
                 const int64_t n = query_offsets[q + 1] - query_offsets[q];
-                Require(n >= 0 && (n == 0 || query_terms), "Invalid query offsets");
                 const int32_t *t = query_terms + query_offsets[q];
-                for (int64_t w = 0; w < n; ++w)
-                    Require(t[w] >= 0, "Invalid public word ID");
-                Require(k_per_query[q] >= 1, "K out of range");
                 if (out_lower)
                     out_lower[q] = static_cast<float>(LowerBound(ss, t, n, k_per_query[q]));
                 if (out_upper) {
@@ -885,6 +954,7 @@ extern "C" int bm25mi_destroy(bm25mi_session *s)
                 if (s->impl) {
                     if (s->impl->serving)
                         throw InvalidState("Stop the session's server before Close");
+                    FailIfInjected("close"); // the session stays intact: Close can be retried
                     s->impl->closed = true;
                     BM25MI_HIP(hipSetDevice(s->impl->device), "set device for close");
                     s->impl.reset();
@@ -895,6 +965,21 @@ extern "C" int bm25mi_destroy(bm25mi_session *s)
     if (rc == BM25MI_OK)
         delete s;
     return rc;
+}
+
+extern "C" int bm25mi_inject_failure(const char *stage, int64_t device_code, const char *message)
+{
+    return Invoke([&] {
+        std::lock_guard<std::mutex> g(injectMutex);
+        if (!stage) {
+            for (auto &f : injected)
+                f = Injected{};
+            return;
+        }
+        const int i = StageIndex(stage);
+        Require(i >= 0, "Unknown failure stage (create, search, close or result)");
+        injected[i] = Injected{true, device_code, message ? message : ""};
+    });
 }
 
 extern "C" int64_t bm25mi_last_error(const bm25mi_session *s, char *buf, size_t size)

@@ -169,6 +169,24 @@ int bm25mi_last_stats(const bm25mi_session *s, bm25mi_stats *out);
 /* Close: idempotent; frees device memory. The handle is freed on success. */
 int bm25mi_destroy(bm25mi_session *s);
 
+/* Failure injection for tests of the error contract (no counterpart in the reference's public
+ * API: its unit tests inject the same failures through mock ACL functions,
+ * FR/ut/cases/bm25/sdk_facade_tests.cpp:53-62, runtime_tests.cpp:19-23). The next library call
+ * that reaches `stage` fails once with BM25MI_ERR_INNER, device code device_code and the given
+ * message (kept whole here; bm25mi_last_error returns at most 1023 characters of it):
+ *   "create"  bm25mi_create, before any device work;
+ *   "search"  a Search on a session, after its request was accepted and before its device
+ *             work: the session fails for good and the outputs are left untouched;
+ *   "close"   bm25mi_destroy, before anything is released: the session stays usable and
+ *             Close may be retried;
+ *   "result"  the next Search's first query rows are corrupted on the host after the device
+ *             wrote them (its first two rows exchanged, or its first doc ID set to the document
+ *             count when K = 1), before the result checks of Session::Receive
+ *             (large_search.cpp:64-85) run: they reject it as a device error (code 0, device_code
+ *             and message unused), which fails the session.
+ * A NULL stage clears every pending injection. Injections are process-wide. */
+int bm25mi_inject_failure(const char *stage, int64_t device_code, const char *message);
+
 /* Copies the last error message of s (or of the calling thread when s is NULL)
  * into buf (truncated to size-1 chars, NUL-terminated) and returns the raw
  * HIP error code of a device failure, else 0 (ErrorInfo::aclCode). */

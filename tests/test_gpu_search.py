@@ -279,3 +279,111 @@ def test_timed_path_reports_kernel_time(synth, synth_session):
     st = synth_session.stats()
     assert st['score_kernel_ms'] > 0 and st['device_ms'] >= st['score_kernel_ms']
     assert st['postings_in_range'] == st['postings_total'] <= st['postings_scanned']
+
+
+def test_search_session_golden_vectors_and_rejects(need_gpu):
+    """The raw-ID Search surface (standalone/run_bm25s_index_search.py:15-76) on the golden
+    vectors the reference's CPU Search wrote, and its request rejections, which leave the
+    session usable."""
+    record = json.loads((GOLDEN / 'reference_cpu_golden.json').read_text())
+    by_corpus = {}
+    for case in record['cases']:
+        by_corpus.setdefault(case['corpus'], []).append(case)
+    for name, cases in by_corpus.items():
+        index = corpora.golden_index(name)
+        for k in sorted({c['k'] for c in cases}):
+            with bm25mi.SearchSession(index, k, max_queries=16, max_words=256) as ss:
+                for case in (c for c in cases if c['k'] == k):
+                    scores, ids = ss.search([list(map(int, q)) for q in case['queries']])
+                    assert ids.tolist() == case['ids'], case['name']
+                    assert scores.view(np.uint32).tolist() == case['score_bits'], case['name']
+                for bad in ([], [[1]] * 17, [[True]], [[-1]], [[2**31 - 1]], [(1,)], [[1] * 257]):
+                    with pytest.raises(ValueError):
+                        ss.search(bad)
+                case = next(c for c in cases if c['k'] == k)
+                scores, ids = ss.search([list(map(int, q)) for q in case['queries']])
+                assert ids.tolist() == case['ids']
+    with pytest.raises(ValueError):
+        bm25mi.SearchSession(corpora.golden_index('tiny129'), True)
+    with pytest.raises(ValueError):
+        bm25mi.SearchSession(corpora.golden_index('tiny129'), 130)
+
+
+def test_facade_device_failure_state_machine(need_gpu):
+    """sdk_facade_tests.cpp:38-62: an Init failure keeps its device code and a retry succeeds; a
+    device failure during Search returns INNER_ERROR with its code kept and the output
+    untouched, after which Search is an ILLEGAL_OPERATION; a failed Close keeps the state and
+    can be retried; Close is idempotent. The failures are injected from the host before any
+    device work of the failing call."""
+    lib = bm25mi._lib.load()
+    index = bm25mi.Index([0, 1], [0], [1], [1], [2])
+    cfg = bm25mi.Bm25Config(queryCount=1, maxQueryWords=1, topK=1)
+    model = bm25mi.AscendIndexBm25()
+    ok, inner, illegal = (bm25mi.search.APP_ERR_OK, bm25mi.search.APP_ERR_INNER_ERROR,
+                          bm25mi.search.APP_ERR_ILLEGAL_OPERATION)
+    try:
+        assert lib.bm25mi_inject_failure(b'create', 42, b'injected Init failure') == 0
+        assert model.Init(index, cfg) == inner and model.GetLastError().aclCode == 42
+        assert model.Init(index, cfg) == ok
+        assert model.Init(index, cfg) == illegal
+        out = bm25mi.Bm25SearchResult(np.array([9.0], np.float32), np.array([9], np.int64))
+        assert model.Search([[-1]], out) == bm25mi.search.APP_ERR_INVALID_PARAM
+        assert out.scores[0] == 9 and out.labels[0] == 9
+        assert model.Search([[0]], out) == ok and out.labels.tolist() == [0]
+        good = out.scores.copy()
+        assert lib.bm25mi_inject_failure(b'search', 161001, b'injected device failure') == 0
+        assert model.Search([[0]], out) == inner
+        err = model.GetLastError()
+        assert err.aclCode == 161001 and err.message == 'injected device failure'
+        assert np.array_equal(out.scores, good) and out.labels.tolist() == [0]
+        assert model.Search([[0]], out) == illegal
+        assert lib.bm25mi_inject_failure(b'close', 7, b'injected Close failure') == 0
+        assert model.Close() == inner and model.GetLastError().aclCode == 7
+        assert model.Close() == ok and model.Close() == ok
+        assert model.Search([[0]], out) == illegal
+    finally:
+        lib.bm25mi_inject_failure(None, 0, None)
+
+
+def test_result_checks_reject_corrupt_rows(synth, need_gpu):
+    """The result checks of Session::Receive (large_search.cpp:64-85) reject rows out of strict
+    order and doc IDs out of range as device errors, which fail the session."""
+    corpus, index = synth
+    lib = bm25mi._lib.load()
+    queries = bm25mi.draw_queries(corpus.offsets, 2, 3, seed=3)
+    for k, message in ((10, 'Strict global ranking failure'), (1, 'Invalid real-document result')):
+        with bm25mi.Session(index, max_queries=2, max_words=8, max_k=10) as s:
+            s.search(queries, k)
+            assert lib.bm25mi_inject_failure(b'result', 0, None) == 0
+            with pytest.raises(bm25mi.Bm25Error) as err:
+                s.search(queries, k)
+            assert err.value.code == bm25mi._lib.INNER and message in err.value.message
+            with pytest.raises(bm25mi.Bm25Error) as err:
+                s.search(queries, k)
+            assert err.value.code == bm25mi._lib.ILLEGAL_OPERATION
+
+
+@pytest.mark.parametrize('block', [2048, 8192])
+def test_final_kernel_beyond_sort_capacity(block, need_gpu):
+    """Tie mass that leaves more than 16,384 candidates at or above a query's final bound: the
+    final kernel selects the K-th key by bisection over the whole pool instead of sorting it in
+    LDS. 200,000 docs share one score (term 0) and every third one a second, higher one (term 1
+    added); merge_check.h:23-114 checks all-1.0f ties across lists the same way."""
+    n = 200_000
+    docs0 = np.arange(n, dtype=np.int32)
+    docs1 = np.arange(0, n, 3, dtype=np.int32)
+    offsets = np.array([0, n, n + len(docs1)], np.int64)
+    index = bm25mi.Index(offsets, np.concatenate([docs0, docs1]), np.ones(n + len(docs1), np.float32),
+                         np.ones(n, np.float32), np.array([1.0, 0.25], np.float32), k1=1.2)
+    queries = [[0], [0, 1], [1, 0], [0, 0]]
+    offs, terms = bm25mi.pack(queries)
+    with bm25mi.Session(index, max_queries=4, max_words=8, max_k=1024, block_docs=block) as s:
+        for k in (1000, 1024):
+            want_s, want_i = orc.search(index, offs, terms, k)
+            got_s, got_i, _ = s.search(queries, k)
+            assert_same(got_s, got_i, want_s, want_i, f'ties block={block} k={k}')
+            # Query [0] alone: every block keeps at least K of its tied docs, so its pool holds
+            # more candidates than the final kernel's LDS sort (16,384).
+            got_s, got_i, _ = s.search([[0]], k)
+            assert_same(got_s, got_i, want_s[:1], want_i[:1], f'ties [0] block={block} k={k}')
+            assert s.stats()['candidates'] > 16384

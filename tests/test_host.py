@@ -173,15 +173,51 @@ def test_create_without_gpu_reports_error_and_facade_state_machine():
 
 
 def test_last_error_truncates_to_1023_chars():
+    """ErrorInfo contract (FR/ut/cases/bm25/sdk_error_adapter_tests.cpp:34-39): a 2048-character
+    message reaches bm25mi_last_error cut to 1023 characters and NUL-terminated, the device code
+    is kept, and the next successful call clears both. The message is injected at the create
+    stage, which fails before any device work."""
     lib = _lib.load()
-    huge = 'x' * 5000
-    # An invalid k1 carries a short message; the buffer contract still NUL-terminates.
-    buf = C.create_string_buffer(8)
-    rc = lib.bm25mi_impacts(None, 1, None, None, None, None, 1.2, None, 0)
-    assert rc == _lib.INVALID_PARAM
-    lib.bm25mi_last_error(None, buf, len(buf))
-    assert len(buf.value) == 7
-    assert len(huge[:1023]) == 1023
+    long_message = b'x' * 2048
+    assert lib.bm25mi_inject_failure(b'create', 161001, long_message) == _lib.OK
+    cfg = _lib.Config()
+    lib.bm25mi_config_default(C.byref(cfg))
+    handle = C.c_void_p()
+    index = bm25mi.Index([0, 2], [0, 2], [1, 3], [1, 2, 1], [0.5])
+    rc = lib.bm25mi_create(_lib.ptr(index.offsets), index.vocab, _lib.ptr(index.docs), _lib.ptr(index.tf),
+                           index.postings, _lib.ptr(index.norm), index.num_docs, _lib.ptr(index.idf), C.byref(cfg),
+                           C.byref(handle))
+    assert rc == _lib.INNER and not handle.value
+    buf = C.create_string_buffer(4096)
+    assert lib.bm25mi_last_error(None, buf, len(buf)) == 161001
+    assert buf.value == b'x' * 1023
+    small = C.create_string_buffer(8)
+    lib.bm25mi_last_error(None, small, len(small))
+    assert small.value == b'x' * 7  # cut to the caller's buffer, NUL-terminated
+    # A success clears the message and the code.
+    assert lib.bm25mi_validate_index(_lib.ptr(index.offsets), index.vocab, _lib.ptr(index.docs), _lib.ptr(index.tf),
+                                     index.postings, _lib.ptr(index.norm), index.num_docs,
+                                     _lib.ptr(index.idf)) == _lib.OK
+    assert lib.bm25mi_last_error(None, buf, len(buf)) == 0 and buf.value == b''
+    # Unknown stages are rejected; a NULL stage clears pending injections.
+    assert lib.bm25mi_inject_failure(b'launch', 1, b'') == _lib.INVALID_PARAM
+    assert lib.bm25mi_inject_failure(b'search', 1, b'pending') == _lib.OK
+    assert lib.bm25mi_inject_failure(None, 0, None) == _lib.OK
+
+
+def test_facade_init_failure_keeps_device_code():
+    """The facade's Init failure keeps the device code (sdk_facade_tests.cpp:38-40) and leaves
+    the object uninitialised (a later Search is an ILLEGAL_OPERATION)."""
+    lib = _lib.load()
+    index = bm25mi.Index([0, 2], [0, 2], [1, 3], [1, 2, 1], [0.5])
+    facade = bm25mi.AscendIndexBm25()
+    assert lib.bm25mi_inject_failure(b'create', 42, b'injected init failure') == _lib.OK
+    assert facade.Init(index, bm25mi.Bm25Config(queryCount=1, maxQueryWords=1, topK=1)) == \
+        bm25mi.search.APP_ERR_INNER_ERROR
+    err = facade.GetLastError()
+    assert err.aclCode == 42 and err.message == 'injected init failure'
+    result = bm25mi.Bm25SearchResult()
+    assert facade.Search([[0]], result) == bm25mi.search.APP_ERR_ILLEGAL_OPERATION
 
 
 def test_config_defaults():
@@ -199,3 +235,30 @@ def test_config_defaults():
 def test_mlp_param_count():
     lib = _lib.load()
     assert lib.bm25mi_mlp_param_count(16) == 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def test_search_session_request_and_response_checks():
+    """The raw-ID Search surface's checks (standalone/run_bm25s_index_search.py:38-46, 56-75),
+    without a GPU: accepted and rejected requests, and responses that break the contract."""
+    from bm25mi.search import check_request, check_response
+    check_request([[1, 2], [], [2**31 - 2]], 3, 3)
+    check_request([[]], 1, 0)
+    for bad in ([], [[1]] * 4, ([1],), [(1,)], [[True]], [[-1]], [[2**31 - 1]], [[1.0]], [[np.int32(1)]],
+                [[1, 2], [3, 4]], 'abc', None):
+        with pytest.raises(ValueError):
+            check_request(bad, 3, 3)
+    scores = np.array([[3.0, 2.0, 2.0], [0.0, 0.0, 0.0]], np.float32)
+    ids = np.array([[5, 1, 4], [0, 1, 2]], np.int32)
+    check_response(scores, ids, [[1], [2]], 3, 10)
+    broken = [
+        (scores.astype(np.float64), ids), (scores, ids.astype(np.int64)), (scores[:, :2], ids[:, :2]),
+        (np.where(scores == 3.0, np.nan, scores).astype(np.float32), ids),
+        (np.where(scores == 3.0, -1.0, scores).astype(np.float32), ids),
+        (scores, np.array([[5, 1, 10], [0, 1, 2]], np.int32)),        # ID out of range
+        (scores, np.array([[5, 1, 1], [0, 1, 2]], np.int32)),         # duplicate ID
+        (scores, np.array([[5, 4, 1], [0, 1, 2]], np.int32)),         # tie not by ascending ID
+        (np.array([[2.0, 3.0, 2.0], [0, 0, 0]], np.float32), ids),   # not descending
+    ]
+    for s, i in broken:
+        with pytest.raises(RuntimeError):
+            check_response(s, i, [[1], [2]], 3, 10)

@@ -160,3 +160,32 @@ def test_oracle_equals_reference_binary():
             assert np.array_equal(rf.view(np.uint32), f.view(np.uint32))
             assert np.array_equal(ri, i)
             assert np.array_equal(rs.view(np.uint32), s.view(np.uint32))
+
+
+def test_bm25s_restatement_equals_oracle():
+    """The bm25s-algorithm CPU baseline (oracle_bm25s_search: dense FP32 row per query,
+    scatter-add of precomputed impacts in query-token order, bounded heap top-k) is pinned the
+    way the reference checks bm25s (standalone/bm25s_benchmark.py:145-154): its selected scores
+    equal the full-score rows at the selected IDs, its score multiset equals the exact top-K,
+    and its IDs equal the strict contract's. Impacts are precomputed exactly as the oracle
+    computes each contribution, so here the scores agree bit for bit."""
+    corpus = bm25mi.generate('synthetic8k')
+    index = corpus.index()
+    impacts = index.impacts()
+    queries = bm25mi.draw_queries(corpus.offsets, 10, 3, seed=21) + [[], [4999, 4999], [10**6, 3], [7, 7, 7, 1]]
+    offsets, terms = bm25mi.pack(queries)
+    for k in (1, 10, 100, 1000):
+        want_s, want_i, full = orc.search(index, offsets, terms, k, full=True)
+        got_s, got_i = orc.bm25s_search(index, impacts, offsets, terms, k, threads=3)
+        assert np.array_equal(np.take_along_axis(full, got_i.astype(np.int64), axis=1).view(np.uint32),
+                              got_s.view(np.uint32))
+        assert np.array_equal(got_s.view(np.uint32), want_s.view(np.uint32))
+        assert np.array_equal(got_i, want_i)
+    # A planted mistake in the impacts (one posting's impact scaled) is caught.
+    bad = impacts.copy()
+    hot = int(np.argmax(impacts))
+    bad[hot] *= np.float32(0.5)
+    got_s, got_i = orc.bm25s_search(index, bad, *bm25mi.pack([[int(np.searchsorted(index.offsets, hot, 'right') - 1)]]),
+                                    10)
+    want_s, want_i = orc.search(index, *bm25mi.pack([[int(np.searchsorted(index.offsets, hot, 'right') - 1)]]), 10)
+    assert not (np.array_equal(got_s, want_s) and np.array_equal(got_i, want_i))
