@@ -84,6 +84,7 @@ SIGNATURES = {
     'bm25mi_create': (C.c_int, [P, i64, P, P, i64, P, i64, P, P, P]),
     'bm25mi_search': (C.c_int, [P, P, P, i32, i32, P, P]),
     'bm25mi_search_batch': (C.c_int, [P, P, P, i32, P, P, P]),
+    'bm25mi_search_batch_noalloc': (C.c_int, [P, P, P, i32, P, P, P]),
     'bm25mi_search_pruned': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
     'bm25mi_search_timed': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
     'bm25mi_query_features': (C.c_int, [P, P, P, i32, P, P]),

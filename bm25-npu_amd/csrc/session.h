@@ -65,8 +65,14 @@ struct Slot {
     // The launched batch: query count, output row offsets, plain-path postings, timing mode.
     int32_t nq = 0;
     int mode = 0; // graph of the packed batch (index into exec)
-    std::vector<int64_t> out;
+    std::vector<int64_t> out; // [maxQ+1]
     int64_t total = 0;
+    // Per-query host state of Pack and the status rows of Complete, sized to the session's
+    // max_queries at create time so that a Search allocates no host memory.
+    std::vector<float> thrUse;
+    std::vector<int64_t> perQuery;
+    std::vector<uint8_t> listed;
+    std::vector<int32_t> status;
     bool timed = false;
     bm25mi_stats last{};
 };
@@ -94,6 +100,7 @@ public:
     size_t resStatus = 0, resScores = 0, resIds = 0;
     std::vector<Slot> slots;
     int device = 0;
+    void *scratchBase = nullptr; // device address of the no-allocation scratch on this device
 
     bool failed = false, closed = false, serving = false;
     bm25mi_stats last{};
@@ -115,10 +122,11 @@ public:
     // The wave plan: the prefix of per-query task costs (a task of query q costing taskCost +
     // ceil(postings of q / blocks), threshold queries nothing); every scoring wave cuts its own
     // contiguous range of equal estimated work out of the query-major task order (WaveRange).
-    void PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, int64_t *costPrefix) const;
+    void PlanWaves(const int64_t *postingsPerQuery, const uint8_t *listed, int32_t nq, int64_t *costPrefix) const;
     // Validates a whole request (throws InvalidArgument, touching nothing, when it is not
-    // acceptable), then writes it into the slot's packet.
-    void Pack(const Query &query, Slot &slot);
+    // acceptable), then writes it into the slot's packet. The batch's candidate pools must fit
+    // poolBudget entries (default: the slot's pool).
+    void Pack(const Query &query, Slot &slot, int64_t poolBudget = -1);
     // Queues the slot's Search on its stream (graph replay, or direct launches with events
     // around each kernel when timed). Returns at once.
     void Launch(Slot &slot, bool timed);
@@ -130,6 +138,10 @@ public:
     void Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *status);
     // Pack + Launch + Complete on slot 0; any device failure marks the session failed.
     void Search(const Query &query, float *outScores, int32_t *outIds, int32_t *status, bool timed);
+    // The same plain Search with its device scratch (packet, counters, histograms, counter rows,
+    // candidate pools) in the library's static scratch array instead of slot 0's buffers, and
+    // no allocation of any kind; a batch whose pools do not fit is rejected.
+    void SearchScratch(const Query &query, float *outScores, int32_t *outIds, int32_t *status);
     // Changes the scoring launch (waves, schedule task cost) and recaptures the slots' graphs.
     void SetLaunch(int32_t waves, int64_t cost);
 

@@ -51,6 +51,34 @@ bool TakeInjected(const char *stage)
 }
 } // namespace
 
+// Device scratch of bm25mi_search_batch_noalloc: the request packet, per-query counters, score
+// histograms and bounds, the per-wave counter rows and the candidate pools of one Search, at
+// most kScratchBytes. Each GPU holds its own copy from the library's load until the process ends;
+// its device address is taken once per device (ScratchAddress) and only that address is used.
+constexpr size_t kScratchBytes = BM25MI_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratchMutex;
+void *scratchAddress[kMaxDevices] = {}; // per device, from hipGetSymbolAddress
+} // namespace
+
+// The device address of `scratch` on `device`, which must be the current device. Called at
+// session creation.
+static void *ScratchAddress(int device)
+{
+    Require(device >= 0 && device < kMaxDevices, "device_id beyond the library's device table");
+    std::lock_guard<std::mutex> g(scratchMutex);
+    if (!scratchAddress[device]) {
+        void *p = nullptr;
+        BM25MI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "no-allocation scratch address");
+        Require(p != nullptr, "no-allocation scratch has no device address");
+        scratchAddress[device] = p;
+    }
+    return scratchAddress[device];
+}
+
 static void FailIfInjected(const char *stage)
 {
     std::lock_guard<std::mutex> g(injectMutex);
@@ -82,7 +110,7 @@ uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
     return bits;
 }
 
-void Session::PlanWaves(const int64_t *postingsPerQuery, const bool *listed, int32_t nq, int64_t *costPrefix) const
+void Session::PlanWaves(const int64_t *postingsPerQuery, const uint8_t *listed, int32_t nq, int64_t *costPrefix) const
 {
     costPrefix[0] = 0;
     for (int32_t q = 0; q < nq; ++q) {
@@ -139,6 +167,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     hipDeviceProp_t prop;
     BM25MI_HIP(hipGetDeviceProperties(&prop, device), "device properties");
     cus = prop.multiProcessorCount;
+    scratchBase = ScratchAddress(device);
     const int perCu = ScoreBlocksPerCu(blockDocs);
     Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
@@ -232,6 +261,11 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
 void Session::InitSlot(Slot &slot)
 {
     const size_t q = static_cast<size_t>(cfg.max_queries);
+    slot.out.assign(q + 1, 0);
+    slot.thrUse.assign(q, 0.0F);
+    slot.perQuery.assign(q, 0);
+    slot.listed.assign(q, 0);
+    slot.status.assign(q, 0);
     BM25MI_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking), "create stream");
     BM25MI_HIP(hipStreamCreateWithFlags(&slot.side, hipStreamNonBlocking), "create side stream");
     for (auto &e : slot.fork)
@@ -336,8 +370,10 @@ void Session::Capture(Slot &slot)
     }
 }
 
-void Session::Pack(const Query &query, Slot &slot)
+void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
 {
+    if (poolBudget < 0)
+        poolBudget = poolEntries;
     // Validate the complete request before modifying the reusable packet.
     const int32_t nq = query.count;
     Require(nq >= 1 && nq <= cfg.max_queries, "Query batch exceeds declared capacity or is empty");
@@ -360,7 +396,10 @@ void Session::Pack(const Query &query, Slot &slot)
     // terms' largest impacts) leaves most granules live, and above K = kLiveMaxK the live path's
     // per-granule cut costs more than it saves: such a query is scored on the plain path, with
     // the same results.
-    std::vector<float> thrUse(static_cast<size_t>(nq), 0.0F);
+    // Per-query host state lives in the slot's buffers, sized at create time: packing allocates
+    // nothing.
+    float *thrUse = slot.thrUse.data();
+    std::fill_n(thrUse, nq, 0.0F);
     int64_t poolNeed = 0;
     for (int32_t q = 0; q < nq; ++q) {
         if (query.thr && query.thr[q] > 0.0F && query.k[q] <= kLiveMaxK) {
@@ -373,7 +412,7 @@ void Session::Pack(const Query &query, Slot &slot)
         }
         poolNeed += PoolCapacity(query.k[q], thrUse[q] > 0.0F);
     }
-    Require(poolNeed <= poolEntries, "Batch needs more candidate-pool memory than a slot holds");
+    Require(poolNeed <= poolBudget, "Batch needs more candidate-pool memory than a slot (or the no-allocation scratch) holds");
 
     auto *pk = static_cast<unsigned char *>(slot.hPacket);
     reinterpret_cast<int32_t *>(pk)[0] = nq;
@@ -383,7 +422,7 @@ void Session::Pack(const Query &query, Slot &slot)
     auto *tau0 = reinterpret_cast<uint32_t *>(pk + offTau0);
     auto *pool = reinterpret_cast<int64_t *>(pk + offPool);
     auto *out = reinterpret_cast<int64_t *>(pk + offOut);
-    slot.out.assign(static_cast<size_t>(nq) + 1, 0);
+    std::fill_n(slot.out.begin(), nq + 1, int64_t(0));
     qoff[0] = pool[0] = out[0] = 0;
     for (int32_t q = 0; q < nq; ++q) {
         qoff[q + 1] = query.offsets[q + 1];
@@ -395,7 +434,8 @@ void Session::Pack(const Query &query, Slot &slot)
         tau0[q] = StartBound(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q]);
     }
     auto *info = reinterpret_cast<uint4 *>(pk + offTerms);
-    std::vector<int64_t> perQuery(static_cast<size_t>(nq), 0);
+    int64_t *perQuery = slot.perQuery.data();
+    std::fill_n(perQuery, nq, int64_t(0));
     int64_t total = 0;
     for (int32_t q = 0; q < nq; ++q)
         for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w) {
@@ -413,16 +453,16 @@ void Session::Pack(const Query &query, Slot &slot)
             }
         }
     // Threshold queries are planned by the planning kernel and scored from its live masks.
-    std::unique_ptr<bool[]> listed(new bool[static_cast<size_t>(nq)]);
+    uint8_t *listed = slot.listed.data();
     int32_t withThr = 0;
     for (int32_t q = 0; q < nq; ++q) {
-        listed[q] = thr[q] > 0.0F;
+        listed[q] = thr[q] > 0.0F ? 1 : 0;
         withThr += listed[q] ? 1 : 0;
     }
     reinterpret_cast<int32_t *>(pk)[1] = withThr;
     reinterpret_cast<int32_t *>(pk)[2] = nq - withThr;
     slot.mode = withThr == 0 ? 0 : withThr < nq ? 1 : 2;
-    PlanWaves(perQuery.data(), listed.get(), nq, reinterpret_cast<int64_t *>(pk + offCost));
+    PlanWaves(perQuery, listed, nq, reinterpret_cast<int64_t *>(pk + offCost));
     slot.nq = nq;
     slot.total = total;
 }
@@ -533,6 +573,64 @@ void Session::Search(const Query &query, float *outScores, int32_t *outIds, int3
     try {
         FailIfInjected("search");
         Launch(slot, timed);
+        Complete(slot, outScores, outIds, status);
+        last = slot.last;
+    } catch (...) {
+        failed = true;
+        throw;
+    }
+}
+
+void Session::SearchScratch(const Query &query, float *outScores, int32_t *outIds, int32_t *status)
+{
+    Require(outScores && outIds && status, "Null output array");
+    Require(!query.thr, "The no-allocation Search takes no thresholds");
+    // Scratch layout of this Search: packet | count | tau | histograms | counter rows | pools.
+    const size_t q = static_cast<size_t>(cfg.max_queries);
+    size_t at = Align(packetBytes, 256);
+    const size_t offCount = at;
+    at = Align(at + q * 4, 256);
+    const size_t offTau = at;
+    at = Align(at + q * 4, 256);
+    const size_t offHist = at;
+    at = Align(at + q * kHistBins * 4, 256);
+    const size_t offRows = at;
+    at = Align(at + static_cast<size_t>(grid) * kStats * 8, 256);
+    Require(at < kScratchBytes, "The session's packet and counters exceed the no-allocation scratch");
+    const int64_t poolCap = static_cast<int64_t>((kScratchBytes - at) / 8);
+    Slot &slot = slots[0];
+    Pack(query, slot, poolCap); // input errors (a batch whose pools exceed the scratch too) keep the session
+    auto *base = static_cast<unsigned char *>(scratchBase);
+    BatchView bt = slot.bt;
+    const auto rebase = [&](const void *p) {
+        return base + (static_cast<const unsigned char *>(p) - static_cast<const unsigned char *>(slot.dPacket));
+    };
+    bt.header = reinterpret_cast<const int32_t *>(rebase(bt.header));
+    bt.qoff = reinterpret_cast<const int64_t *>(rebase(bt.qoff));
+    bt.k = reinterpret_cast<const int32_t *>(rebase(bt.k));
+    bt.thr = reinterpret_cast<const float *>(rebase(bt.thr));
+    bt.tau0 = reinterpret_cast<const uint32_t *>(rebase(bt.tau0));
+    bt.poolOff = reinterpret_cast<const int64_t *>(rebase(bt.poolOff));
+    bt.outOff = reinterpret_cast<const int64_t *>(rebase(bt.outOff));
+    bt.words = reinterpret_cast<const uint4 *>(rebase(bt.words));
+    bt.costPrefix = reinterpret_cast<const int64_t *>(rebase(bt.costPrefix));
+    WorkView wk = slot.wk; // status, scores and IDs: the slot's mapped result block
+    wk.pool = reinterpret_cast<unsigned long long *>(base + at);
+    wk.count = reinterpret_cast<uint32_t *>(base + offCount);
+    wk.tau = reinterpret_cast<uint32_t *>(base + offTau);
+    wk.hist = reinterpret_cast<uint32_t *>(base + offHist);
+    wk.stats = reinterpret_cast<unsigned long long *>(base + offRows);
+    wk.liveMask = wk.dead = wk.cursor = nullptr; // threshold planning only
+    try {
+        slot.timed = false;
+        BM25MI_HIP(hipMemcpyAsync(base, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
+                   "packet H2D (scratch)");
+        // The scratch keeps the last call's contents, possibly another session's: the counters
+        // are set by a fill kernel before scoring.
+        LaunchInit(wk, cfg.max_queries, slot.stream);
+        Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream), "Unsupported block size");
+        LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev), grid, slot.stream);
+        BM25MI_HIP(hipGetLastError(), "launch Search kernels (scratch)");
         Complete(slot, outScores, outIds, status);
         last = slot.last;
     } catch (...) {
@@ -763,6 +861,29 @@ extern "C" int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offse
 {
     Query q{query_offsets, query_terms, queries, k_per_query, nullptr};
     return SearchCommon(s, q, out_scores, out_ids, nullptr, false, false);
+}
+
+extern "C" int bm25mi_search_batch_noalloc(bm25mi_session *s, const int64_t *query_offsets,
+                                           const int32_t *query_terms, int32_t queries, const int32_t *k_per_query,
+                                           float *out_scores, int32_t *out_ids)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            const auto start = std::chrono::steady_clock::now();
+            if (!s->impl || s->impl->closed || s->impl->failed)
+                throw InvalidState("BM25 session closed/failed");
+            if (s->impl->serving)
+                throw InvalidState("BM25 session is serving: submit through its server");
+            Session &ss = *s->impl;
+            Query q{query_offsets, query_terms, queries, k_per_query, nullptr};
+            ss.SearchScratch(q, out_scores, out_ids, ss.slots[0].status.data());
+            ss.last.fallbacks = 0;
+            ss.last.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+        },
+        &s->error);
 }
 
 extern "C" int bm25mi_search_pruned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,

@@ -124,6 +124,22 @@ int bm25mi_search(bm25mi_session *s, const int64_t *query_offsets, const int32_t
 int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                         int32_t queries, const int32_t *k_per_query, float *out_scores, int32_t *out_ids);
 
+/* Plain Search that allocates nothing (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving loops and harnesses
+ * that forbid hipMalloc). Same arguments and results as bm25mi_search_batch, but the Search's
+ * device scratch (request packet, per-query counters, score histograms and bounds, per-wave
+ * counter rows, candidate pools) lives in one static device array of
+ * BM25MI_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the process ends;
+ * the call makes no hipMalloc/hipFree and no host allocation. A batch whose candidate pools
+ * need more than the array holds (each query needs min(blocks x max(2K, 64), docs) 8-byte
+ * entries; e.g. 8 queries of K = 100 on 10M docs with 2048-doc tiles) is rejected with
+ * BM25MI_ERR_INVALID_PARAM. Every call on one GPU uses that GPU's array: calls that overlap
+ * on one GPU (two sessions on one device searched from two threads) race; calls on one
+ * session are serialised by its lock. */
+#define BM25MI_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int bm25mi_search_batch_noalloc(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                int32_t queries, const int32_t *k_per_query, float *out_scores, int32_t *out_ids);
+
 /* Threshold-pruned Search (no reference counterpart: the learned path).
  * thresholds[q] is a predicted K-th score (0 = plain). 256-doc granules whose
  * upper bound (the FP32 sum of the query words' granule maxima) is below it are
