@@ -156,84 +156,97 @@ def cpu_legs(args, index, packed, gpu_results):
 
 
 def learned_leg(args, index, offsets):
-    """Trains the K-th-score MLP on the GPU from exact plain-path outputs and compares the
-    pruned path with the plain path on held-out batches."""
+    """Trains the K-th-score MLP on the GPU from exact plain-path outputs, makes it resident on
+    the Search sessions and compares learned Searches (thresholds predicted on the GPU inside the
+    timed call) with plain Searches of the same held-out batches."""
     t0 = time.time()
-    s = bm25mi.Session(index, max_queries=64, max_words=64 * 12, max_k=100, block_docs=args.block_docs)
+    s = bm25mi.Session(index, max_queries=256, max_words=256 * 12, max_k=100, block_docs=args.block_docs)
     qs, ks = learned.make_queries(offsets, args.train_queries, seed=100)
-    data = learned.collect(s, qs, ks, batch=64)
+    t_draw = time.time() - t0
+    data = learned.collect(s, qs, ks, batch=256)
+    del qs
     t_collect = time.time() - t0
     t1 = time.time()
-    model = learned.ThresholdModel(seed=0).fit(data, epochs=args.train_epochs, batch=256, lr=2e-3, seed=0)
+    model = learned.ThresholdModel(seed=0).fit(data, epochs=args.train_epochs, batch=args.train_batch, lr=2e-3,
+                                               seed=0)
     t_fit = time.time() - t1
-    vq, vk = learned.make_queries(offsets, 2048, seed=200)
-    vdata = learned.collect(s, vq, vk, batch=64)
+    vq, vk = learned.make_queries(offsets, 4096, seed=200)
+    vdata = learned.collect(s, vq, vk, batch=256)
     model.calibrate(vdata, fallback_rate=args.fallback_target)
+    s.close()
     out = {'train_pairs': int(len(data.kth)), 'train_terms': '2-12 df-weighted', 'train_k': [10, 100],
-           'collect_s': round(t_collect, 2), 'fit_s': round(t_fit, 2), 'epochs': args.train_epochs,
+           'draw_s': round(t_draw, 2), 'collect_s': round(t_collect, 2), 'fit_s': round(t_fit, 2),
+           'epochs': args.train_epochs, 'batch': args.train_batch,
            'loss_first_last': [float(model.losses[0]), float(model.losses[-1])], 'safety': model.safety,
+           'inference': 'on the GPU inside every timed learned Search (threshold kernel before scoring)',
            'cases': []}
+    del data
 
-    def run_case(name, batches, k):
-        thr = []
-        for b in batches:
-            feats = s.features(b, k)
-            lower, upper = s.bounds(b, k)
-            thr.append(model.thresholds(feats, lower, upper))
-        plain_ms, plain_res = ab_protocol(lambda i: s.search(batches[i], k), [0, 1], warmup=3, cycles=args.cycles)
-        prune_ms, prune_res = ab_protocol(lambda i: s.search(batches[i], k, thresholds=thr[i]), [0, 1], warmup=3,
+    def run_case(name, batches, k, sess):
+        plain_ms, plain_res = ab_protocol(lambda i: sess.search(batches[i], k), [0, 1], warmup=3, cycles=args.cycles)
+        prune_ms, prune_res = ab_protocol(lambda i: sess.search(batches[i], k, learned=True), [0, 1], warmup=3,
                                           cycles=args.cycles)
         mism = sum(same(plain_res[i][:2], prune_res[i][:2]) for i in (0, 1))
         fb = sum(int(prune_res[i][2].sum()) for i in (0, 1))
         scan = {'plain': 0, 'learned': 0}
         dev = {'plain': [], 'learned': []}
-        kern = {'plain': [], 'learned': []}  # timed scoring kernels: KScore / KPrune + KLive
-        for i in (0, 1):
-            s.search(batches[i], k, timed=True)
-            st = s.stats()
-            scan['plain'] += st['postings_scanned']
-            dev['plain'].append(st['device_ms'])
-            kern['plain'].append(st['score_kernel_ms'])
-            s.search(batches[i], k, thresholds=thr[i], timed=True)
-            st = s.stats()
-            scan['learned'] += st['postings_scanned']
-            dev['learned'].append(st['device_ms'])
-            kern['learned'].append(st['score_kernel_ms'])
+        kern = {'plain': [], 'learned': []}  # timed scoring kernels: KScore / KThreshold .. KLive
+        for rep in range(5):
+            for i in (0, 1):
+                sess.search(batches[i], k, timed=True)
+                st = sess.stats()
+                scan['plain'] += st['postings_scanned'] if rep == 0 else 0
+                dev['plain'].append(st['device_ms'])
+                kern['plain'].append(st['score_kernel_ms'])
+                sess.search(batches[i], k, learned=True, timed=True)
+                st = sess.stats()
+                scan['learned'] += st['postings_scanned'] if rep == 0 else 0
+                dev['learned'].append(st['device_ms'])
+                kern['learned'].append(st['score_kernel_ms'])
         nq = len(batches[0])
         p, lr = summarize(plain_ms, nq), summarize(prune_ms, nq)
+        kp, kl = statistics.median(kern['plain']), statistics.median(kern['learned'])
         out['cases'].append({
             'case': name, 'queries': nq, 'k': k, 'plain_ms_median': p['ms_median'],
             'learned_ms_median': lr['ms_median'], 'speedup': p['ms_median'] / lr['ms_median'],
             'plain_qps': p['qps'], 'learned_qps': lr['qps'],
-            'plain_device_ms': statistics.mean(dev['plain']), 'learned_device_ms': statistics.mean(dev['learned']),
+            'plain_device_ms': statistics.median(dev['plain']), 'learned_device_ms': statistics.median(dev['learned']),
             'mismatches': mism, 'fallback_rate': fb / (2 * nq),
             'postings_plain': scan['plain'], 'postings_learned': scan['learned'],
             'bytes_skipped_frac': 1 - scan['learned'] / max(1, scan['plain']),
             # HBM roofline of the scoring kernels: 8 B per posting streamed (the two A/B batches)
             # over their timed duration; 'effective' counts the plain path's postings instead.
             'roofline': {
-                'plain_kernels_ms': statistics.mean(kern['plain']),
-                'learned_kernels_ms': statistics.mean(kern['learned']),
-                'plain_frac': 8 * scan['plain'] / 2 / (statistics.mean(kern['plain']) * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                'learned_frac': 8 * scan['learned'] / 2 / (statistics.mean(kern['learned']) * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                'learned_effective_frac': 8 * scan['plain'] / 2 / (statistics.mean(kern['learned']) * 1e-3) / 1e9
-                                          / HBM_PEAK_GBS}})
+                'plain_kernels_ms': kp, 'learned_kernels_ms': kl,
+                'plain_frac': 8 * scan['plain'] / 2 / (kp * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                'learned_frac': 8 * scan['learned'] / 2 / (kl * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                'learned_effective_frac': 8 * scan['plain'] / 2 / (kl * 1e-3) / 1e9 / HBM_PEAK_GBS}})
+        log(f'learned {name}: plain {p["ms_median"]:.4f} ms, learned {lr["ms_median"]:.4f} ms, '
+            f'{p["ms_median"] / lr["ms_median"]:.2f}x, device {kp:.4f}/{kl:.4f} ms, mismatches {mism}, fb {fb}')
 
-    run_case('Q8x3 K10 (headline batches)', [bm25mi.draw_queries(offsets, 8, 3, seed=1),
-                                            bm25mi.draw_queries(offsets, 8, 3, seed=2)], 10)
     for nq in (8, 64):
+        sess = bm25mi.Session(index, max_queries=nq, max_words=nq * 12, max_k=100, block_docs=args.block_docs,
+                              live_max_k=args.live_max_k)
+        model.upload(sess)
+        if nq == 8:
+            run_case('Q8x3 K10 (headline batches)', [bm25mi.draw_queries(offsets, 8, 3, seed=1),
+                                                    bm25mi.draw_queries(offsets, 8, 3, seed=2)], 10, sess)
         for k in (10, 100):
             batches = [learned.make_queries(offsets, nq, seed=900 + 10 * nq + k + j, k_choices=(k,))[0]
                        for j in (0, 1)]
-            run_case(f'Q{nq} x 2-12 terms K{k} (held out)', batches, k)
-    s.close()
+            run_case(f'Q{nq} x 2-12 terms K{k} (held out)', batches, k, sess)
+        sess.close()
     return out, model
 
 
 def serving_leg(args, index, offsets, model):
-    """Poisson arrivals through the continuous-batching server, plain and learned side by side."""
+    """Poisson arrivals through the continuous-batching server, plain and learned side by side.
+    Learned requests have their thresholds predicted on the GPU in the batch they join; every
+    learned request's rows are compared with the plain run's."""
     s = bm25mi.Session(index, max_queries=256, max_words=256 * 32, max_k=1000, slots=4,
-                       pool_entries=args.pool_entries, block_docs=args.block_docs)
+                       pool_entries=args.pool_entries, block_docs=args.block_docs, live_max_k=args.live_max_k)
+    if model is not None:
+        model.upload(s)
     out = {'slots': 4, 'max_batch': 256, 'pool_entries_per_slot': s.config.pool_entries or 'full',
            'terms': 'uniform 1-32, df-weighted', 'k_mix': {'10': 0.7, '100': 0.25, '1000': 0.05}, 'runs': []}
     rng = np.random.default_rng(55)
@@ -242,31 +255,21 @@ def serving_leg(args, index, offsets, model):
         lengths = rng.integers(1, 33, size=n)
         ks = rng.choice(np.array([10, 100, 1000], np.int32), size=n, p=[0.7, 0.25, 0.05]).astype(np.int32)
         queries = bm25mi.draw_queries(offsets, n, lengths, seed=int(qps))
-        thr = None
-        if model is not None:
-            feats = np.concatenate([s.features(queries[i:i + 256], ks[i:i + 256]) for i in range(0, n, 256)])
-            bounds = [s.bounds(queries[i:i + 256], ks[i:i + 256]) for i in range(0, n, 256)]
-            lower = np.concatenate([b[0] for b in bounds])
-            upper = np.concatenate([b[1] for b in bounds])
-            thr = model.thresholds(feats, lower, upper)
-        for mode in (['plain', 'learned'] if thr is not None else ['plain']):
+        plain = None
+        for mode in (['plain', 'learned'] if model is not None else ['plain']):
             with bm25mi.Server(s, max_batch=256) as srv:
-                r = srv.run_poisson(queries, ks, qps=qps, seed=int(qps) + 1, keep_results=(mode == 'learned'),
-                                    thresholds=thr if mode == 'learned' else None)
+                r = srv.run_poisson(queries, ks, qps=qps, seed=int(qps) + 1, keep_results=True,
+                                    learned=(mode == 'learned'))
                 c = srv.counters()
             rec = {'mode': mode, 'offered_qps': qps, 'requests': n, 'sustained_qps': r['sustained_qps'],
                    'p50_ms': r['p50_ms'], 'p95_ms': r['p95_ms'], 'p99_ms': r['p99_ms'], 'batches': c['batches'],
                    'mean_batch': c['queries'] / max(1, c['batches']), 'max_inflight': c['max_inflight'],
                    'fallbacks': c['fallbacks']}
-            if mode == 'learned':
-                # Spot-check results against the plain path on the first requests.
-                rows = np.concatenate([[0], np.cumsum(ks)])
-                mism = 0
-                for i in range(0, min(n, 64)):
-                    ws, wi, _ = s.search([queries[i]], int(ks[i]))
-                    mism += same((r['scores'][rows[i]:rows[i + 1]], r['ids'][rows[i]:rows[i + 1]]),
-                                 (ws.ravel(), wi.ravel()))
-                rec['mismatches_first_64'] = mism
+            if mode == 'plain':
+                plain = (r['scores'], r['ids'])
+            else:
+                rec['mismatches_vs_plain'] = same((r['scores'], r['ids']), plain)
+                rec['requests_checked'] = n
             out['runs'].append(rec)
             log(f'serving {mode} {qps} QPS: sustained {r["sustained_qps"]:.0f}, p50 {r["p50_ms"]:.2f} ms, '
                 f'p99 {r["p99_ms"]:.2f} ms')
@@ -320,8 +323,10 @@ def main():
     ap.add_argument('--no-learned', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
     ap.add_argument('--batched', action='store_true', help='add the 8.84M-passage uneven-batch runs')
-    ap.add_argument('--train-queries', type=int, default=65536, help='exact (query, K-th score) training pairs')
-    ap.add_argument('--train-epochs', type=int, default=40)
+    ap.add_argument('--train-queries', type=int, default=1_000_000, help='exact (query, K-th score) training pairs')
+    ap.add_argument('--train-epochs', type=int, default=20)
+    ap.add_argument('--train-batch', type=int, default=1024)
+    ap.add_argument('--live-max-k', type=int, default=0, help='largest K scored on the live path (0 = library default)')
     ap.add_argument('--fallback-target', type=float, default=0.005,
                     help='held-out rate of queries whose threshold exceeds the K-th score (plain reruns)')
     ap.add_argument('--serving-qps', type=float, nargs='*', default=[2000.0, 10000.0, 40000.0])

@@ -30,6 +30,7 @@ class Config(C.Structure):
         ('task_cost', C.c_int32),
         ('slots', C.c_int32),
         ('pool_entries', C.c_int64),
+        ('live_max_k', C.c_int32),
     ]
 
 
@@ -87,13 +88,17 @@ SIGNATURES = {
     'bm25mi_search_batch_noalloc': (C.c_int, [P, P, P, i32, P, P, P]),
     'bm25mi_search_pruned': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
     'bm25mi_search_timed': (C.c_int, [P, P, P, i32, P, P, P, P, P]),
+    'bm25mi_set_threshold_model': (C.c_int, [P, P, i64, f32, f32, f32]),
+    'bm25mi_search_learned': (C.c_int, [P, P, P, i32, P, i32, P, P, P]),
+    'bm25mi_learned_thresholds': (C.c_int, [P, P, P, i32, P, P]),
     'bm25mi_query_features': (C.c_int, [P, P, P, i32, P, P]),
     'bm25mi_query_bounds': (C.c_int, [P, P, P, i32, P, P, P]),
     'bm25mi_server_start': (C.c_int, [P, i32, P]),
     'bm25mi_server_submit': (C.c_int, [P, P, i32, i32, P]),
     'bm25mi_server_wait': (C.c_int, [P, C.c_uint64, P, P, P]),
     'bm25mi_server_submit_pruned': (C.c_int, [P, P, i32, i32, f32, P]),
-    'bm25mi_server_run_poisson': (C.c_int, [P, P, P, i32, P, P, f64, C.c_uint64, P, P, P, P]),
+    'bm25mi_server_run_poisson': (C.c_int, [P, P, P, i32, P, P, i32, f64, C.c_uint64, P, P, P, P]),
+    'bm25mi_server_submit_learned': (C.c_int, [P, P, i32, i32, P]),
     'bm25mi_server_counters': (C.c_int, [P, P, P, P, P]),
     'bm25mi_server_stop': (C.c_int, [P]),
     'bm25mi_set_launch': (C.c_int, [P, i32, i32]),

@@ -1,15 +1,17 @@
 """Learned K-th-score threshold for safe block pruning (no reference counterpart).
 
-A 3-layer MLP (16 -> 64 -> 64 -> 1, ReLU, FP32) predicts log(K-th score) of a query from
-per-term index statistics (bm25mi_query_features). Training and inference run on the GPU
-through the library's own HIP kernels (bm25mi_mlp_train / bm25mi_mlp_predict); training pairs
-are exact Search outputs of the plain GPU path over seeded queries.
+A 3-layer MLP (16 -> 64 -> 64 -> 1, ReLU, FP32) predicts, from per-query statistics of the
+query's terms (bm25mi_query_features), log(K-th score / upper), where upper is the FP32 sum of
+the terms' largest impacts (an upper bound of every score). Training and inference run on the
+GPU through the library's own HIP kernels (bm25mi_mlp_train / bm25mi_mlp_predict); training
+pairs are exact Search outputs of the plain GPU path over seeded queries.
 
 The prediction, scaled by a safety factor calibrated on held-out queries and clamped to the
-query's provable bounds [lower, upper] (bm25mi_query_bounds), is passed to the pruned Search as
-its threshold. Blocks whose granule upper bound is below it are skipped and only docs scoring
-at least it are kept; a query with fewer than K such docs is rerun on the plain path, so
-results always equal the plain path.
+query's provable bounds [lower, upper] (bm25mi_query_bounds), is the query's threshold. Uploaded
+to a session (ThresholdModel.upload), the model predicts thresholds on the device inside every
+learned Search (Session.search(..., learned=True)). Granules whose upper bound is below the
+threshold are skipped and only docs scoring at least it are kept; a query with fewer than K such
+docs is rerun on the plain path, so results always equal the plain path.
 """
 
 import ctypes as C
@@ -98,7 +100,7 @@ def collect(session, queries, ks, batch):
 
 
 class ThresholdModel:
-    """Predicts log(K-th score); thresholds = clamp(safety * exp(prediction), lower, upper)."""
+    """Predicts log(K-th score / upper); thresholds = clamp(safety * upper * exp(prediction), lower, upper)."""
 
     def __init__(self, seed=0):
         self.params = init_params(seed)
@@ -111,35 +113,44 @@ class ThresholdModel:
     def _x(self, features):
         return ((np.asarray(features, np.float32) - self.mean) / self.scale).astype(np.float32)
 
+    @staticmethod
+    def _usable(data):
+        return (data.kth > 0) & (data.upper > 0)
+
     def fit(self, data, epochs=60, batch=256, lr=2e-3, seed=0):
         """Trains on the queries whose K-th score is positive (the others need no threshold)."""
-        keep = data.kth > 0
-        x, kth = data.features[keep], data.kth[keep]
+        keep = self._usable(data)
+        x = data.features[keep]
         self.mean = x.mean(axis=0).astype(np.float32)
         self.scale = np.maximum(x.std(axis=0), 1e-6).astype(np.float32)
-        y = np.log(kth.astype(np.float64))
+        y = np.log(data.kth[keep].astype(np.float64) / data.upper[keep].astype(np.float64))
         self.y_mean, self.y_scale = float(y.mean()), float(max(y.std(), 1e-6))
         target = ((y - self.y_mean) / self.y_scale).astype(np.float32)
         self.params, self.losses = train(self.params, self._x(x), target, epochs, min(batch, len(target)), lr, seed)
         return self
 
-    def predict(self, features):
-        """Predicted K-th score per query (float64)."""
+    def predict_ratio(self, features):
+        """Predicted K-th score / upper bound per query (float64)."""
         z = forward(self.params, self._x(features)).astype(np.float64)
         return np.exp(z * self.y_scale + self.y_mean)
 
     def thresholds(self, features, lower, upper, safety=None):
         s = self.safety if safety is None else safety
-        t = np.clip(s * self.predict(features), np.asarray(lower, np.float64), np.asarray(upper, np.float64))
+        upper = np.asarray(upper, np.float64)
+        t = np.clip(s * upper * self.predict_ratio(features), np.asarray(lower, np.float64), upper)
         return t.astype(np.float32)
 
     def calibrate(self, data, fallback_rate=0.02):
         """Largest safety factor whose thresholds exceed the true K-th score (forcing a
         plain-path rerun) on at most `fallback_rate` of the held-out queries."""
-        keep = data.kth > 0
-        ratio = data.kth[keep].astype(np.float64) / self.predict(data.features[keep])
+        keep = self._usable(data)
+        ratio = data.kth[keep].astype(np.float64) / (data.upper[keep] * self.predict_ratio(data.features[keep]))
         self.safety = float(np.quantile(ratio, fallback_rate)) if len(ratio) else 1.0
         return self.safety
+
+    def upload(self, session):
+        """Makes the model resident on the session's GPU for learned Searches."""
+        session.set_threshold_model(self.params, self.mean, self.scale, self.y_mean, self.y_scale, self.safety)
 
 
 def compare(session, queries, k, thresholds):

@@ -30,7 +30,7 @@ class Session:
     """A resident index on one GPU. All searches go through libbm25mi."""
 
     def __init__(self, index, max_queries=8, max_words=24, max_k=10, block_docs=0, k1=None, device=0,
-                 use_graph=False, grid_blocks=0, task_cost=0, slots=1, pool_entries=0):
+                 use_graph=False, grid_blocks=0, task_cost=0, slots=1, pool_entries=0, live_max_k=0):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.bm25mi_config_default(C.byref(cfg))
@@ -45,6 +45,7 @@ class Session:
         cfg.task_cost = task_cost
         cfg.slots = slots
         cfg.pool_entries = pool_entries
+        cfg.live_max_k = live_max_k
         self.config = cfg
         self._kcache = {}
         self.num_docs, self.vocab = index.num_docs, index.vocab
@@ -93,12 +94,14 @@ class Session:
             raise ValueError('one K per query required')
         return offsets, terms, nq, ks
 
-    def search(self, queries, k, thresholds=None, timed=False):
+    def search(self, queries, k, thresholds=None, timed=False, learned=False):
         """Exact top-K per query. queries: list of term-ID lists or (offsets, terms).
 
         k: int or per-query array. thresholds: optional per-query predicted K-th
-        scores (learned path; results are still exact). Returns (scores, ids,
-        fallback) with rows concatenated per query (shape (Q, K) when k is an int).
+        scores (threshold path; results are still exact). learned: thresholds predicted on
+        the GPU by the session's resident model (set_threshold_model) inside the Search.
+        Returns (scores, ids, fallback) with rows concatenated per query (shape (Q, K) when
+        k is an int).
         """
         offsets, terms = self._pack(queries)
         nq = len(offsets) - 1
@@ -131,7 +134,11 @@ class Session:
         pThr = _lib.ptr(thr)
         h = self.handle
         po, pt = _lib.ptr(offsets), _lib.ptr(terms)
-        if timed:
+        if learned:
+            if pThr is not None:
+                raise ValueError('a learned Search predicts its own thresholds')
+            rc = self._lib.bm25mi_search_learned(h, po, pt, nq, pk, 1 if timed else 0, pScores, pIds, pFallback)
+        elif timed:
             rc = self._lib.bm25mi_search_timed(h, po, pt, nq, pk, pThr, pScores, pIds, pFallback)
         elif pThr is not None:
             rc = self._lib.bm25mi_search_pruned(h, po, pt, nq, pk, pThr, pScores, pIds, pFallback)
@@ -158,6 +165,24 @@ class Session:
         _lib.check(self._lib.bm25mi_query_bounds(self.handle, _lib.ptr(offsets), _lib.ptr(terms), nq, _lib.ptr(ks),
                                                  _lib.ptr(lower), _lib.ptr(upper)), self.handle)
         return lower, upper
+
+    def set_threshold_model(self, model, mean, scale, y_mean, y_scale, safety):
+        """Makes a K-th-score model resident on the session's GPU for learned Searches: MLP
+        parameters (layout of the bm25mi_mlp_* kernels), feature mean and scale, and the output
+        transform threshold = safety * upper * exp(z * y_scale + y_mean)."""
+        blob = np.ascontiguousarray(np.concatenate([np.asarray(model, np.float32), np.asarray(mean, np.float32),
+                                                    np.asarray(scale, np.float32)]), np.float32)
+        _lib.check(self._lib.bm25mi_set_threshold_model(self.handle, _lib.ptr(blob), len(blob), float(y_mean),
+                                                        float(y_scale), float(safety)), self.handle)
+
+    def learned_thresholds(self, queries, k):
+        """float32[Q]: the thresholds a learned Search of these queries would use, predicted on
+        the GPU by the resident model (0 = the query takes the plain path)."""
+        offsets, terms, nq, ks = self._prepare(queries, k)
+        out = np.empty(nq, np.float32)
+        _lib.check(self._lib.bm25mi_learned_thresholds(self.handle, _lib.ptr(offsets), _lib.ptr(terms), nq,
+                                                       _lib.ptr(ks), _lib.ptr(out)), self.handle)
+        return out
 
     def set_launch(self, grid_waves=0, task_cost=0):
         """Re-plans the scoring launch (waves, schedule task cost); 0 keeps a value."""

@@ -31,12 +31,17 @@ class Server:
         _lib.check(self._lib.bm25mi_server_start(session.handle, max_batch, C.byref(handle)))
         self._h = handle
 
-    def submit(self, terms, k, threshold=None):
-        """Queues one query; with a threshold it runs pruned (and reruns plain if it must)."""
+    def submit(self, terms, k, threshold=None, learned=False):
+        """Queues one query; with a threshold (or learned: predicted on the device) it runs
+        pruned, and reruns plain if it must."""
         terms = np.ascontiguousarray(terms, np.int32)
         ticket = C.c_uint64()
         tp = _lib.ptr(terms) if terms.size else None
-        if threshold is None:
+        if learned:
+            if threshold is not None:
+                raise ValueError('a learned request predicts its own threshold')
+            _lib.check(self._lib.bm25mi_server_submit_learned(self._h, tp, len(terms), k, C.byref(ticket)))
+        elif threshold is None:
             _lib.check(self._lib.bm25mi_server_submit(self._h, tp, len(terms), k, C.byref(ticket)))
         else:
             _lib.check(self._lib.bm25mi_server_submit_pruned(self._h, tp, len(terms), k, float(threshold),
@@ -52,7 +57,7 @@ class Server:
                                                 C.byref(lat)))
         return scores, ids, lat.value
 
-    def run_poisson(self, queries, ks, qps, seed=0, keep_results=True, thresholds=None):
+    def run_poisson(self, queries, ks, qps, seed=0, keep_results=True, thresholds=None, learned=False):
         """Open-loop load: query i is submitted at the i-th arrival of a seeded Poisson process
         of rate qps. Returns latencies (us), sustained QPS, percentiles and, when asked, the
         concatenated result rows."""
@@ -72,7 +77,8 @@ class Server:
         ids = np.empty(rows, np.int32) if keep_results else None
         elapsed = C.c_double()
         _lib.check(self._lib.bm25mi_server_run_poisson(self._h, _lib.ptr(offsets), _lib.ptr(terms), n, _lib.ptr(ks),
-                                                       _lib.ptr(thr), float(qps), C.c_uint64(seed), _lib.ptr(lat),
+                                                       _lib.ptr(thr), 1 if learned else 0, float(qps),
+                                                       C.c_uint64(seed), _lib.ptr(lat),
                                                        _lib.ptr(scores), _lib.ptr(ids), C.byref(elapsed)))
         out = {'latency_us': lat, 'elapsed_s': elapsed.value, 'offered_qps': float(qps),
                'sustained_qps': n / elapsed.value, **percentiles(lat)}

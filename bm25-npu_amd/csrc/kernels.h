@@ -7,8 +7,12 @@ namespace bm25mi {
 
 constexpr int kWave = 64;                  // CDNA wavefront
 constexpr int kMaxTerms = 64;              // terms per query (fixed_query_packet.h:32): one lane per term
-constexpr int kGranule = 256;              // docs per directory / block-max granule
-constexpr int kGranuleShift = 8;
+// Docs per directory / block-max granule: the unit the threshold path skips or scores. 128 docs:
+// on the 10M-doc corpus at thresholds of 0.93 x the K-th score, the live granules of Q8 x 3-term
+// batches hold 5.3% (seed 1) / 2.6% (seed 2) of the query terms' postings, against 13% / 6.6%
+// with 256-doc granules, and are fewer (25.8K against 31.9K).
+constexpr int kGranule = 128;
+constexpr int kGranuleShift = 7;
 constexpr int kLongMin = 4096;             // lists this long use the granule directory; shorter ones
                                            // are located through the posting samples
 constexpr int kSampleShift = 4;            // samples[i] = doc ID of posting 16 * i (global CSC index)
@@ -93,6 +97,35 @@ struct BatchView {
     // equivalents (0 for threshold queries). Scoring wave w takes the tasks of the query-major
     // task order (task = q * blocks + blk) from cost costPrefix[nq] * w / grid on.
     const int64_t *costPrefix; // [maxQ+1]
+    // Learned batches (header[3] = 1), whose thresholds the threshold kernel predicts and writes
+    // into thr[], together with costPrefix and header[1..2]: per query its model features, the
+    // FP32 sum of its terms' largest impacts (an upper bound of every score) and its plain-path
+    // cost (task cost * blocks), used when the query is routed to the plain path.
+    const float *feat;         // [maxQ][kFeatures]
+    const float *upper;        // [maxQ]
+    const int64_t *planCost;   // [maxQ]
+};
+
+// Ranges of a request packet to stage, in 16-byte units: the packet's arrays, each up to the
+// batch's queries (or words).
+constexpr int kStageRanges = 12;
+struct StageRanges {
+    uint32_t off[kStageRanges];
+    uint32_t len[kStageRanges];
+    int32_t n;
+};
+
+constexpr int kFeatures = 16;  // model inputs per query
+constexpr int kHidden = 64;    // width of the model's two hidden layers
+
+// The resident K-th-score model of learned batches and the routing rules of threshold queries.
+// model: W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1], then the features' mean[16] and
+// scale[16]; nullptr when the session has none.
+struct ModelView {
+    const float *model;
+    float yMean, yScale, safety; // threshold = safety * upper * exp(z * yScale + yMean), clamped
+    float minRatio;              // a threshold below minRatio * upper routes the query to the plain path,
+    int32_t liveMaxK;            // as does a K above liveMaxK
 };
 
 // Per-Search device scratch and results.
@@ -102,11 +135,15 @@ struct WorkView {
     uint32_t *hist;           // [maxQ * kHistBins] histogram of kept candidates' scores
     uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
     unsigned long long *stats;// [grid][kStats]: one counter row per scoring wave
-    // Written by the planning kernel for batches with a threshold query: per task
-    // (q * blocks + blk) the mask of the block's granules whose upper bound reaches the query's
-    // threshold (0 for dead blocks and for plain queries), and per wave the dead blocks it found.
-    uint32_t *liveMask;       // [maxQ * blocks]
+    // Written by the planning kernel for batches with a threshold query: per item (q * items per
+    // query + c, the query's granules 64c .. 64c+63) the mask of granules whose upper bound
+    // reaches the query's threshold (0 for plain queries), and per wave the dead blocks it found.
+    unsigned long long *itemMask; // [maxQ * ceil(granules / 64)]
     uint32_t *dead;           // [2 * grid] (the planning kernel runs with twice the scoring grid)
+    // Live plan of the live kernel's waves, from the item masks (KPlan): wave w takes `count`
+    // live granules in item order, starting at the `skip`-th live granule of item `item`:
+    // {item, skip, count, 0}. The waves' shares differ by at most one granule.
+    uint4 *plan;              // [grid]
     // Also per (query word w of the batch, block) for the short words of threshold queries: the
     // index of the word's first posting with doc at or after the block's first doc.
     uint32_t *cursor;         // [maxWords * blocks]

@@ -202,21 +202,24 @@ __global__ __launch_bounds__(kMlpThreads) void KMlpBackward(const float *__restr
         lossPartial[blockIdx.x] = sRed[0] + sRed[1] + sRed[2] + sRed[3];
 }
 
+// Sums the workgroup partials in a fixed order. train = 0: grad += sum, lossOut[0] = mean loss;
+// train = 1 (training steps): grad = sum, lossOut[0] += mean loss (the epoch's running sum).
 __global__ void KMlpReduce(const float *__restrict__ partial, const float *__restrict__ lossPartial, int blocks,
-                           int64_t count, int n, float *__restrict__ grad, float *__restrict__ lossOut)
+                           int64_t count, int n, float *__restrict__ grad, float *__restrict__ lossOut, int train)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < count) {
         float s = 0.0F;
         for (int b = 0; b < blocks; ++b)
             s += partial[int64_t(b) * count + i];
-        grad[i] += s;
+        grad[i] = train ? s : grad[i] + s;
     }
     if (i == 0 && lossOut) {
         float s = 0.0F;
         for (int b = 0; b < blocks; ++b)
             s += lossPartial[b];
-        lossOut[0] = s / static_cast<float>(n);
+        const float mean = s / static_cast<float>(n);
+        lossOut[0] = train ? lossOut[0] + mean : mean;
     }
 }
 
@@ -293,7 +296,7 @@ extern "C" int bm25mi_mlp_backward(const float *params, const float *x, const fl
         BM25MI_MLP_HIP(hipGetLastError(), "MLP backward launch");
         const int rgrid = static_cast<int>((count + 255) / 256);
         hipLaunchKernelGGL(KMlpReduce, dim3(rgrid), dim3(256), 0, s, partial, lossPartial, blocks, count, n, grad,
-                           loss_out);
+                           loss_out, 0);
         BM25MI_MLP_HIP(hipGetLastError(), "MLP reduce launch");
         // The scratch buffer is reused by the next call: finish this one first.
         BM25MI_MLP_HIP(hipStreamSynchronize(s), "MLP backward sync");
@@ -440,7 +443,9 @@ extern "C" int bm25mi_mlp_predict(const float *params, const float *x, int32_t n
 
 // Minibatch Adam training on the GPU. params (in/out) start from the caller's values. The
 // sample order of every epoch is a seeded shuffle; epoch_loss[e] (optional) receives the
-// mean minibatch loss of epoch e.
+// mean minibatch loss of epoch e. The steps of an epoch are queued without waiting on the host:
+// each step's gradient partials live in this call's own buffers and the epoch's loss is summed
+// on the device, read once per epoch.
 extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *target, int32_t n, int32_t in_features,
                                 int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss)
 {
@@ -450,8 +455,10 @@ extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *targ
                 "Invalid MLP training arguments");
         const int in = in_features;
         const int64_t count = MlpParams(in);
+        const int blocks = static_cast<int>(std::min<int64_t>(kBackwardGrid, (int64_t(batch) + kTile - 1) / kTile));
         Stream st;
         DeviceBuffer dp(count), dg(count), dm(count), dv(count), dx(static_cast<size_t>(n) * in), dt(n), dl(1);
+        DeviceBuffer partial(static_cast<size_t>(blocks) * count), lossPartial(static_cast<size_t>(blocks));
         dp.Upload(params, count, st.s);
         BM25MI_MLP_HIP(hipMemsetAsync(dm.p, 0, count * sizeof(float), st.s), "Adam m clear");
         BM25MI_MLP_HIP(hipMemsetAsync(dv.p, 0, count * sizeof(float), st.s), "Adam v clear");
@@ -461,6 +468,8 @@ extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *targ
         std::vector<float> xs(static_cast<size_t>(n) * in), ts(n);
         uint64_t rng = seed;
         int32_t step = 0;
+        const int rgrid = static_cast<int>((count + 255) / 256);
+        const int agrid = rgrid;
         for (int e = 0; e < epochs; ++e) {
             for (int32_t i = n - 1; i > 0; --i) // Fisher-Yates
                 std::swap(order[i], order[static_cast<int32_t>(SplitMix64(rng) % static_cast<uint64_t>(i + 1))]);
@@ -471,23 +480,26 @@ extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *targ
             }
             dx.Upload(xs.data(), xs.size(), st.s);
             dt.Upload(ts.data(), ts.size(), st.s);
-            double sum = 0;
+            BM25MI_MLP_HIP(hipMemsetAsync(dl.p, 0, sizeof(float), st.s), "MLP epoch loss clear");
             int batches = 0;
             for (int32_t b0 = 0; b0 + batch <= n; b0 += batch) {
-                BM25MI_MLP_HIP(hipMemsetAsync(dg.p, 0, count * sizeof(float), st.s), "MLP grad clear");
-                if (bm25mi_mlp_backward(dp.p, dx.p + static_cast<size_t>(b0) * in, dt.p + b0, batch, in, dg.p, dl.p,
-                                        st.s) != BM25MI_OK)
-                    throw DeviceError(0, "MLP backward failed");
-                if (bm25mi_mlp_adam(dp.p, dg.p, dm.p, dv.p, count, lr, 0.9F, 0.999F, 1e-8F, ++step, st.s) != BM25MI_OK)
-                    throw DeviceError(0, "Adam step failed");
-                float loss = 0;
-                dl.Download(&loss, 1, st.s);
-                BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP step sync");
-                sum += loss;
+                hipLaunchKernelGGL(KMlpBackward, dim3(blocks), dim3(kMlpThreads), 0, st.s, dp.p,
+                                   dx.p + static_cast<size_t>(b0) * in, dt.p + b0, batch, in, partial.p, lossPartial.p);
+                hipLaunchKernelGGL(KMlpReduce, dim3(rgrid), dim3(256), 0, st.s, partial.p, lossPartial.p, blocks,
+                                   count, batch, dg.p, dl.p, 1);
+                ++step;
+                const float c1 = 1.0F - std::pow(0.9F, static_cast<float>(step));
+                const float c2 = 1.0F - std::pow(0.999F, static_cast<float>(step));
+                hipLaunchKernelGGL(KMlpAdam, dim3(agrid), dim3(256), 0, st.s, dp.p, dg.p, dm.p, dv.p, count, lr, 0.9F,
+                                   0.999F, 1e-8F, c1, c2);
                 ++batches;
             }
+            BM25MI_MLP_HIP(hipGetLastError(), "MLP training launches");
+            float sum = 0;
+            dl.Download(&sum, 1, st.s);
+            BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP epoch loss");
             if (epoch_loss)
-                epoch_loss[e] = static_cast<float>(sum / std::max(1, batches));
+                epoch_loss[e] = sum / static_cast<float>(std::max(1, batches));
         }
         dp.Download(params, count, st.s);
         BM25MI_MLP_HIP(hipStreamSynchronize(st.s), "MLP train sync");

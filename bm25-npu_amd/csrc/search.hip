@@ -25,6 +25,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include "../../include/bm25mi.h"
 #include "kernels.h"
 
 namespace bm25mi {
@@ -240,22 +241,33 @@ __device__ __forceinline__ uint32_t WaveKthLargest(uint32_t v, uint32_t k)
 constexpr int Log2(int x) { return x <= 1 ? 0 : 1 + Log2(x / 2); }
 
 typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t U32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) const U32x4 LdsUint4; // 16-byte LDS words
+typedef __attribute__((address_space(3))) const U32x2 LdsUint2; // 8-byte LDS words
+typedef __attribute__((address_space(3))) const uint32_t LdsU32;
 
-// Keys (score bits << log2(B) | (B-1-e)) of accumulator entries e at least (xb, xe), counted
-// by each lane over its entries.
+// Keys (score bits << log2(B) | (B-1-e)) of accumulator entries e (B of them, lane l holding
+// entries l * B / 64 ...) at least (xb, xe), counted by each lane over its entries.
 template <int B>
-__device__ __forceinline__ uint32_t CountKeys(LdsUint4 *bits4, uint32_t xb, uint32_t xe)
+__device__ __forceinline__ uint32_t CountKeys(LdsU32 *bits, uint32_t xb, uint32_t xe)
 {
-    constexpr int kVec = B / (4 * kWave);
     const uint32_t lane = __lane_id();
     uint32_t c = 0;
+    if constexpr (B == 2 * kWave) {
+        const U32x2 v = reinterpret_cast<LdsUint2 *>(bits)[lane];
+        const uint32_t e0 = static_cast<uint32_t>(B - 1) - lane * 2;
+        c = (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) + (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u);
+    } else {
+        constexpr int kVec = B / (4 * kWave);
+        static_assert(kVec >= 1 && B % (4 * kWave) == 0, "accumulator size");
+        LdsUint4 *bits4 = reinterpret_cast<LdsUint4 *>(bits);
 #pragma unroll 1
-    for (int r = 0; r < kVec; ++r) {
-        const U32x4 v = bits4[r * kWave + lane];
-        const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
-        c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) + (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
-             (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) + (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
+        for (int r = 0; r < kVec; ++r) {
+            const U32x4 v = bits4[r * kWave + lane];
+            const uint32_t e0 = static_cast<uint32_t>(B - 1) - (r * kWave + lane) * 4;
+            c += (v.x > xb || (v.x == xb && e0 >= xe) ? 1u : 0u) + (v.y > xb || (v.y == xb && e0 - 1 >= xe) ? 1u : 0u) +
+                 (v.z > xb || (v.z == xb && e0 - 2 >= xe) ? 1u : 0u) + (v.w > xb || (v.w == xb && e0 - 3 >= xe) ? 1u : 0u);
+        }
     }
     return c;
 }
@@ -273,7 +285,7 @@ struct CutResult {
 // block score at least it. kept/mine are the wave and lane counts of docs scoring at least
 // floor, mx the lane's largest score bits.
 template <int B>
-__device__ __attribute__((noinline)) CutResult SelectCut(LdsUint4 *bits4, uint32_t floor, uint32_t k, uint32_t cap,
+__device__ __attribute__((noinline)) CutResult SelectCut(LdsU32 *bits4, uint32_t floor, uint32_t k, uint32_t cap,
                                                          uint32_t mx)
 {
     constexpr int kIdxBits = Log2(B);
@@ -350,8 +362,10 @@ __device__ __forceinline__ void WaveRange(const BatchView &bt, int32_t blocks, u
 }
 
 template <int B>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t addRows)
 {
+    // addRows: the live kernel ran before this one in the same Search and wrote counter rows
+    // [0, addRows); a wave below that adds its counters to its row.
     constexpr int kVec = B / (4 * kWave);   // float4 accumulators per lane
     constexpr int kGpb = B / kGranule;      // granules per block
     constexpr int U = kColumns;
@@ -723,7 +737,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // Cut (rare), then append the kept docs to the pool as composite keys (score bits << 32 |
         // ~doc) and count them in the histogram: one global atomic reserves the pool slots,
         // one per non-empty bucket counts them, and the cut raises tau.
-        LdsUint4 *bitsL = (LdsUint4 *)bits4; // acc is in LDS
+        LdsU32 *bitsL = (LdsU32 *)bits4; // acc is in LDS
         uint32_t tb = floor, te = 0, tauNew = 0;
         const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
         if (kept > cap) {
@@ -756,7 +770,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             uint32_t slot = slot0 + before;
 #pragma unroll 1
             for (int r = 0; r < kVec; ++r) {
-                const U32x4 v = bitsL[r * kWave + lane];
+                const U32x4 v = reinterpret_cast<LdsUint4 *>(bitsL)[r * kWave + lane];
                 const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
                 const uint32_t e0 = (r * kWave + lane) * 4;
 #pragma unroll
@@ -789,11 +803,25 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     }
     BM25MI_PHASE(0);
 #undef BM25MI_PHASE
-    // Counters go to this wave's own row (plain stores); the final kernel sums the rows.
+    // Counters go to this wave's own row (plain stores; added to the row the live kernel's wave
+    // of the same index wrote when addRows); the final kernel sums the rows.
     useful = WaveSum64(useful);
     overflow = WaveSum64(overflow);
     if (lane == 0) {
         unsigned long long *row = wk.stats + static_cast<int64_t>(blockIdx.x) * kStats;
+        const bool covered = static_cast<int32_t>(blockIdx.x) < addRows;
+        if (covered && tBegin == tEnd)
+            return; // no tasks: the live kernel's row already holds this Search's counters
+        if (covered) {
+            scanned += row[kStatPostings];
+            scored += row[kStatScored];
+            skipped += row[kStatSkipped];
+            emitted += row[kStatCandidates];
+            overflow += row[kStatOverflow];
+            emitting += row[kStatEmitting];
+            cuts += row[kStatExact];
+            useful += row[kStatUseful];
+        }
         row[kStatPostings] = scanned;
         row[kStatScored] = scored;
         row[kStatSkipped] = skipped;
@@ -816,6 +844,66 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// Threshold of query q of a learned batch (no reference counterpart), computed by one wave
+// (every lane returns it): the MLP kFeatures -> 64 -> 64 -> 1 (ReLU, FP32) maps the query's
+// standardised features to z, an estimate of (log(K-th score / upper) - yMean) / yScale, where
+// upper is the FP32 sum of its terms' largest impacts; the threshold is
+// clamp(safety * upper * exp(z * yScale + yMean), lower, upper), lower = tau0 (the K-th largest
+// impact of any of its terms). A threshold below minRatio * upper, or K above liveMaxK, routes
+// the query to the plain path (0). Lane j computes hidden unit j of both layers; the second
+// layer's weights are read in four rows of 16 to keep few registers live. The computation is the
+// same instruction sequence on the same inputs wherever it runs.
+__device__ __forceinline__ float PredictThreshold(const ModelView &mv, const BatchView &bt, int q)
+{
+    const int lane = static_cast<int>(__lane_id());
+    const float *W1 = mv.model, *B1 = W1 + kHidden * kFeatures, *W2 = B1 + kHidden, *B2 = W2 + kHidden * kHidden;
+    const float *W3 = B2 + kHidden, *B3 = W3 + kHidden, *mean = B3 + 1, *scale = mean + kFeatures;
+    const float ub = bt.upper[q];
+    const float lower = __uint_as_float(bt.tau0[q]);
+    const int32_t k = bt.k[q];
+    const float4 *w1 = reinterpret_cast<const float4 *>(W1 + lane * kFeatures);
+    float z1 = B1[lane];
+#pragma unroll
+    for (int v = 0; v < kFeatures / 4; ++v) {
+        const float4 w = w1[v];
+        const float *f = bt.feat + static_cast<int64_t>(q) * kFeatures + 4 * v;
+        z1 = fmaf(w.x, (f[0] - mean[4 * v]) / scale[4 * v], z1);
+        z1 = fmaf(w.y, (f[1] - mean[4 * v + 1]) / scale[4 * v + 1], z1);
+        z1 = fmaf(w.z, (f[2] - mean[4 * v + 2]) / scale[4 * v + 2], z1);
+        z1 = fmaf(w.w, (f[3] - mean[4 * v + 3]) / scale[4 * v + 3], z1);
+    }
+    const float h1 = fmaxf(z1, 0.0F);
+    float z2 = B2[lane];
+    const float4 *w2 = reinterpret_cast<const float4 *>(W2 + lane * kHidden);
+#pragma unroll 1
+    for (int c = 0; c < kHidden / 16; ++c) {
+        float4 w[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            w[v] = w2[c * 4 + v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = c * 16 + 4 * v;
+            z2 = fmaf(w[v].x, __shfl(h1, i), z2);
+            z2 = fmaf(w[v].y, __shfl(h1, i + 1), z2);
+            z2 = fmaf(w[v].z, __shfl(h1, i + 2), z2);
+            z2 = fmaf(w[v].w, __shfl(h1, i + 3), z2);
+        }
+    }
+    float y = W3[lane] * fmaxf(z2, 0.0F);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        y += __shfl_xor(y, o);
+    y += B3[0];
+    float thr = 0.0F;
+    if (ub > 0.0F && k <= mv.liveMaxK) {
+        thr = fminf(fmaxf(mv.safety * ub * expf(y * mv.yScale + mv.yMean), lower), ub);
+        if (!(thr >= mv.minRatio * ub))
+            thr = 0.0F;
+    }
+    return thr;
+}
+
 __device__ __forceinline__ unsigned long long WaveOr64(unsigned long long v)
 {
 #pragma unroll
@@ -835,8 +923,8 @@ constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
 // summed in query order, is at most the same sum of these bounds; the sum here may be taken
 // in another order, which the factor (1 - 2^-16) on the threshold covers for up to 64 words.
 // A block with no granule reaching the threshold cannot hold a doc scoring at least it. The
-// kernel writes every block's mask of live granules (0 for dead blocks and for the blocks of
-// plain queries) and, per wave, the number of dead blocks. It runs with twice the scoring grid
+// kernel writes every item's mask of live granules (0 for the items of plain queries) and, per
+// wave, the number of dead blocks. It runs with twice the scoring grid
 // (it needs 46 VGPRs and no LDS, so all its waves are resident): at Q8 every wave then takes at
 // most one item instead of two in a row.
 template <int B>
@@ -845,7 +933,7 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
     constexpr int kGpb = B / kGranule;
     constexpr int kBpc = kPruneSpan / kGpb; // blocks per item
     constexpr int kSampleRounds = (kLongMin >> kSampleShift) / kWave;
-    static_assert(kPruneSpan % kGpb == 0 && kGpb <= 32, "block size");
+    static_assert(kPruneSpan % kGpb == 0 && kGpb <= kPruneSpan, "block size");
     const uint32_t lane = __lane_id();
     const int32_t nq = bt.header[0];
     const int64_t granules = ix.granules;
@@ -878,10 +966,9 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
         }
         const int64_t blk = gBase / kGpb + lane; // lane b < kBpc: block gBase / kGpb + b of this chunk
         const bool inRange = static_cast<int>(lane) < kBpc && blk < blocks;
-        uint32_t *maskOut = wk.liveMask + static_cast<int64_t>(q) * blocks;
         if (!(thrLive > 0.0F)) {
-            if (inRange)
-                maskOut[blk] = 0u; // a plain query
+            if (lane == 0)
+                wk.itemMask[it] = 0ULL; // a plain query
             continue;
         }
         const int64_t g = gBase + lane;
@@ -973,52 +1060,218 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
         for (int u = 0; u < 8; ++u)
             ub += vFirst[u];
         const unsigned long long liveG = __ballot(valid && ub >= thrLive);
-        uint32_t bm = 0;
-        if (static_cast<int>(lane) < kBpc)
-            bm = static_cast<uint32_t>((liveG >> (lane * kGpb)) & ((1ULL << kGpb) - 1ULL));
-        if (inRange)
-            maskOut[blk] = bm;
+        unsigned long long bm = 0; // lane b < kBpc: the live granules of block b of the item
+        if (static_cast<int>(lane) < kBpc) {
+            if constexpr (kGpb == kPruneSpan)
+                bm = liveG;
+            else
+                bm = (liveG >> (lane * kGpb)) & ((1ULL << kGpb) - 1ULL);
+        }
+        if (lane == 0)
+            wk.itemMask[it] = liveG;
         dead += static_cast<uint32_t>(__popcll(__ballot(inRange && bm == 0)));
     }
     if (lane == 0)
         wk.dead[blockIdx.x] = dead;
 }
 
-constexpr int kLiveSlots = 8; // granules a live batch scores together (8 x 256 FP32 = 8 KB of LDS)
+#ifndef BM25MI_LIVE_SLOTS
+#define BM25MI_LIVE_SLOTS 16
+#endif
+// Granules a live batch scores together (16 x 128 FP32 = 8 KB of LDS per wave).
+constexpr int kLiveSlots = BM25MI_LIVE_SLOTS;
+static_assert(kLiveSlots >= 2 && kLiveSlots <= 32 && (kLiveSlots & (kLiveSlots - 1)) == 0, "live batch size");
+constexpr int kPlanThreads = 1024;
+#ifndef BM25MI_PLAN_GROUPS
+#define BM25MI_PLAN_GROUPS 16
+#endif
+#ifndef BM25MI_PLAN_LOADS
+#define BM25MI_PLAN_LOADS 8
+#endif
+constexpr int kPlanGroups = BM25MI_PLAN_GROUPS; // workgroups of the live plan, each assigning a portion
+constexpr int kPlanLoads = BM25MI_PLAN_LOADS;   // item masks in flight per thread
+
+// ceil(a / b) and floor(a / b) for a < 2^53 and 1 <= b < 2^32, from the reciprocal r = 1 / b (a
+// double): the estimate is within one of the true value and an integer check corrects it.
+__device__ __forceinline__ uint64_t CeilDiv(uint64_t a, uint64_t b, double r)
+{
+    uint64_t q = static_cast<uint64_t>(ceil(static_cast<double>(a) * r));
+    if (q > 0 && (q - 1) * b >= a)
+        --q;
+    else if (q * b < a)
+        ++q;
+    return q;
+}
+__device__ __forceinline__ uint64_t FloorDiv(uint64_t a, uint64_t b, double r)
+{
+    uint64_t q = static_cast<uint64_t>(floor(static_cast<double>(a) * r));
+    if (q * b > a)
+        --q;
+    else if ((q + 1) * b <= a)
+        ++q;
+    return q;
+}
+
+// Live plan (the learned path; no reference counterpart). kPlanGroups workgroups turn the
+// planning kernel's item masks into equal shares of the live granules for the `waves` waves of
+// the live kernel. A live granule's rank is its position in item order (items query-major,
+// granules ascending); wave w takes ranks [T * w / waves, T * (w + 1) / waves) of the T live
+// granules, so the shares differ by at most one granule. Every workgroup counts all items (kPlanLoads
+// masks in flight per thread) for T and for the live granules before its portion of the items,
+// then scans its portion (a contiguous run per thread) for every item's rank prefix P_i, and each
+// item writes the plan of every wave whose first rank falls in it: the item, the live granules of
+// it before that rank, and the wave's share. The intervals [ceil(P_i * waves / T),
+// ceil(P_{i+1} * waves / T)) of the items partition [0, waves), so every wave gets exactly one
+// plan. For learned batches (header[3] = 1), whose routing the threshold kernel decided, wave 0
+// of workgroup 0 then writes the plain queries' cost prefix and the header counts.
+__global__ __launch_bounds__(kPlanThreads) void KPlan(BatchView bt, WorkView wk, uint32_t itemsPerQuery, int32_t waves)
+{
+    __shared__ uint32_t red[kPlanThreads / 64];
+    const int32_t nq = bt.header[0];
+    const uint32_t items = static_cast<uint32_t>(nq) * itemsPerQuery;
+    const uint32_t p0 = static_cast<uint32_t>(static_cast<uint64_t>(items) * blockIdx.x / gridDim.x);
+    const uint32_t p1 = static_cast<uint32_t>(static_cast<uint64_t>(items) * (blockIdx.x + 1) / gridDim.x);
+    uint32_t all = 0, beforeP = 0;
+    for (uint32_t b = threadIdx.x; b < items; b += kPlanThreads * kPlanLoads) {
+        uint32_t c[kPlanLoads];
+#pragma unroll
+        for (int k = 0; k < kPlanLoads; ++k) {
+            const uint32_t i = b + k * kPlanThreads;
+            c[k] = i < items ? static_cast<uint32_t>(__popcll(wk.itemMask[i])) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kPlanLoads; ++k) {
+            all += c[k];
+            beforeP += b + k * kPlanThreads < p0 ? c[k] : 0u;
+        }
+    }
+    const uint32_t total = BlockSum<kPlanThreads>(all, red);
+    const uint32_t before = BlockSum<kPlanThreads>(beforeP, red);
+    // This thread's run of the portion: items [r0, r1), up to per of them.
+    const uint32_t per = (p1 - p0 + kPlanThreads - 1) / kPlanThreads;
+    const uint32_t r0 = min(p0 + threadIdx.x * per, p1), r1 = min(r0 + per, p1);
+    uint32_t run = 0;
+    for (uint32_t j = r0; j < r1; j += 16) {
+        uint32_t c[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            c[k] = j + k < r1 ? static_cast<uint32_t>(__popcll(wk.itemMask[j + k])) : 0u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            run += c[k];
+    }
+    uint32_t portion = 0;
+    uint64_t P = static_cast<uint64_t>(before) + BlockExclusive<kPlanThreads>(run, red, &portion);
+    if (total == 0) {
+        if (blockIdx.x == 0)
+            for (int w = static_cast<int>(threadIdx.x); w < waves; w += kPlanThreads)
+                wk.plan[w] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        const uint64_t G = static_cast<uint64_t>(waves), T = total;
+        const double rT = 1.0 / static_cast<double>(T), rG = 1.0 / static_cast<double>(G);
+        for (uint32_t j = r0; j < r1; j += 16) {
+            uint32_t cnt[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                cnt[k] = j + k < r1 ? static_cast<uint32_t>(__popcll(wk.itemMask[j + k])) : 0u;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (cnt[k]) {
+                    const uint64_t wLo = CeilDiv(P * G, T, rT);
+                    const uint64_t wHi = min(CeilDiv((P + cnt[k]) * G, T, rT), G);
+                    uint64_t s0 = FloorDiv(T * wLo, G, rG);
+                    for (uint64_t w = wLo; w < wHi; ++w) {
+                        const uint64_t s1 = FloorDiv(T * (w + 1), G, rG);
+                        wk.plan[w] = make_uint4(j + k, static_cast<uint32_t>(s0 - P), static_cast<uint32_t>(s1 - s0), 0u);
+                        s0 = s1;
+                    }
+                }
+                P += cnt[k];
+            }
+        }
+    }
+    if (blockIdx.x != 0 || threadIdx.x >= kWave || bt.header[3] == 0)
+        return;
+    // Learned batch: the wave plan of the plain kernel over the queries routed to the plain path.
+    const int lane = static_cast<int>(threadIdx.x);
+    auto *cost = const_cast<int64_t *>(bt.costPrefix);
+    auto *header = const_cast<int32_t *>(bt.header);
+    int64_t acc = 0;
+    int32_t live = 0;
+    for (int base = 0; base < nq; base += kWave) {
+        const int q = base + lane;
+        const bool in = q < nq;
+        const bool isLive = in && bt.thr[q] > 0.0F;
+        int64_t c = in && !isLive ? bt.planCost[q] : 0;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int64_t y = __shfl_up(c, o);
+            if (lane >= o)
+                c += y;
+        }
+        if (in)
+            cost[q + 1] = acc + c;
+        acc += __shfl(c, kWave - 1);
+        live += __popcll(__ballot(isLive));
+    }
+    if (lane == 0) {
+        cost[0] = 0;
+        header[1] = live;
+        header[2] = nq - live;
+    }
+}
+
+// Index (0..63) of the n-th set bit (n counted from 0) of m, which has more than n set bits.
+__device__ __forceinline__ uint32_t NthSetBit(unsigned long long m, uint32_t n)
+{
+    uint32_t pos = 0;
+#pragma unroll
+    for (int width = 32; width > 0; width >>= 1) {
+        const uint32_t low = static_cast<uint32_t>(__popcll(m & ((1ULL << width) - 1ULL)));
+        if (n >= low) {
+            n -= low;
+            m >>= width;
+            pos += static_cast<uint32_t>(width);
+        }
+    }
+    return pos;
+}
 
 // Live kernel of threshold queries (the learned path; no reference counterpart). Every wave
-// walks the granule positions blockIdx.x + i * gridDim.x of the live masks (position = task *
-// granules per block + granule, task = q * blocks + blk), 64 per round, and takes the live
-// ones kLiveSlots at a time, from any query. Live granules cluster in the blocks that hold a
-// query's top docs; striding over granules rather than blocks spreads a block's live granules
-// over consecutive waves (striding over blocks gave some waves six live blocks of eight live
-// granules while the median wave took about one batch). A batch
-// resolves all its (granule, word) pairs together: one gather of the query words, one load of
-// two directory entries per long word (or a two-level 16-ary search of a short list's
-// samples), then it streams the pairs' postings as segments in (granule, query word) order
-// into its granules' FP32 accumulators, so every doc's score is the same sequential FP32 sum
-// in query-word order as on the plain path. Docs scoring at least the query's threshold (and
-// its start bound) are appended to the query's pool; a pool that overflows makes the final
-// kernel rerun the query on the plain path. Batching turns the per-block chain of dependent
-// loads into one chain per kLiveSlots granules.
+// scores its share of the live granules (KPlan: a contiguous run in item order, so the waves'
+// work differs by at most one granule), kLiveSlots at a time, from any query. It locates up to
+// 64 of its granules at once from a window of 64 item masks (one load per lane), then takes them
+// into slots. A batch resolves all its (granule, word) pairs together: one gather of the query
+// words, one load of two directory entries per long word (or a window of the short list's
+// postings from the planning kernel's cursor, with a two-level 16-ary search of its samples as
+// the fallback), then it streams the pairs' postings as segments in (granule, query word) order
+// into its granules' FP32 accumulators, so every doc's score is the same sequential FP32 sum in
+// query-word order as on the plain path. Docs scoring at least the query's threshold (and its
+// start bound) are appended to the query's pool; a pool that overflows makes the final kernel
+// rerun the query on the plain path. Batching turns the per-granule chain of dependent loads
+// into one chain per kLiveSlots granules. Counter rows go to row rowBase + wave.
 template <int B>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KLive(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KLive(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t rowBase, int32_t pruneGrid)
 {
     constexpr int kGpb = B / kGranule;
     constexpr int U = kLiveColumns;
     constexpr int kLevel = 16; // samples per level of the short-list search (16 x 16 >= 4096 / 16)
     static_assert(kLevel * kLevel >= (kLongMin >> kSampleShift), "short lists must fit two levels");
+    static_assert(64 % kGpb == 0, "blocks tile the 64 granules of an item");
     __shared__ __align__(16) float acc[kLiveSlots * kGranule];
     float4 *acc4 = reinterpret_cast<float4 *>(acc);
     const uint32_t lane = __lane_id();
-    const int32_t nq = bt.header[0];
-    // Granule positions task * kGpb + bit of the live masks (task = q * blocks + blk).
-    const uint64_t listEnd =
-        bt.header[1] > 0 ? static_cast<uint64_t>(nq) * static_cast<uint32_t>(blocks) * static_cast<uint32_t>(kGpb) : 0;
-    const uint64_t listStep = static_cast<uint64_t>(gridDim.x) * kWave;
-    uint64_t listAt = blockIdx.x;
-    unsigned long long cand = 0; // lanes whose loaded granule is live and still to take
-    uint32_t maskV = 0;
+    const uint32_t itemsPerQuery = static_cast<uint32_t>((ix.granules + kWave - 1) / kWave);
+    const uint32_t items = static_cast<uint32_t>(bt.header[0]) * itemsPerQuery;
+    // This wave's share of the live granules.
+    const uint4 share = wk.plan[blockIdx.x];
+    uint32_t winItem = share.x; // first item of the window of item masks (lane l: item winItem + l)
+    uint32_t skip = share.y;    // live granules of the window before the next one to take
+    uint32_t left = share.z;    // granules still to take
+    // Located granules not yet in a slot: lane i < listN holds granule i of the list as
+    // item << 6 | bit (items < 2^26: the session bounds granules per Search by 2^32) and its
+    // query's word range as first word << 7 | words (first word <= 2^24, words <= 64).
+    uint32_t listIB = 0, listWN = 0, listN = 0, listAt = 0;
     unsigned long long scanned = 0, scored = 0, emitted = 0, emitting = 0, useful = 0;
 
     // Per segment of a batch in position-major order (lane r: segment r): first posting, count,
@@ -1046,47 +1299,78 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     int ns = 0; // slots held (lane s < ns: slot s's query, granule and word range), some carried over
     // Word offsets fit 32 bits (a batch holds at most 2^24 words).
     uint32_t slotQ = 0, slotG = 0, slotW0 = 0, slotNw = 0;
-    uint32_t w0V = 0, nwV = 0; // lane l: the word range of the query of the position it tested
     for (;;) {
         BM25MI_LPHASE(0);
-        // 1. Up to kLiveSlots live granules: lane l tests granule position listAt + l * grid.
+        // 1. Up to kLiveSlots granules of the share.
         while (ns < kLiveSlots) {
-            while (cand == 0 && listAt < listEnd) {
-                const uint64_t gl = listAt + static_cast<uint64_t>(lane) * gridDim.x;
-                maskV = 0u;
-                w0V = 0;
-                nwV = 0u;
-                if (gl < listEnd) {
-                    // With the mask, the word range of the position's query (known from the
-                    // position), so that a taken slot's words can be gathered at once.
-                    const uint32_t ql = static_cast<uint32_t>(gl / kGpb) / static_cast<uint32_t>(blocks);
-                    maskV = wk.liveMask[gl / kGpb];
-                    const int64_t a = bt.qoff[ql], e = bt.qoff[ql + 1];
-                    w0V = static_cast<uint32_t>(a);
-                    nwV = static_cast<uint32_t>(e - a);
+            if (listAt == listN) {
+                if (left == 0)
+                    break;
+                // Locate the next min(left, 64) granules: load the window of item masks, moving
+                // it on while it holds fewer than skip + 1 live granules; then lane i finds the
+                // granule of rank skip + i in it.
+                unsigned long long winMask = 0;
+                uint32_t winIncl = 0, winTotal = 0; // inclusive live-granule counts over the lanes
+                for (;;) {
+                    if (winItem >= items) {
+                        left = 0; // past the last item (the plan counts only granules of the masks)
+                        break;
+                    }
+                    const uint32_t it = winItem + lane;
+                    winMask = it < items ? wk.itemMask[it] : 0ULL;
+                    winIncl = static_cast<uint32_t>(__popcll(winMask));
+#pragma unroll
+                    for (int o = 1; o < kWave; o <<= 1) {
+                        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(winIncl), o));
+                        if (lane >= static_cast<uint32_t>(o))
+                            winIncl += y;
+                    }
+                    winTotal = ReadLane(winIncl, kWave - 1);
+                    if (skip < winTotal)
+                        break;
+                    skip -= winTotal;
+                    winItem += kWave;
                 }
-                cand = __ballot((maskV >> (gl % kGpb)) & 1u);
-                listAt += listStep;
+                if (left == 0)
+                    break;
+                const uint32_t take = min(left, min(static_cast<uint32_t>(kWave), winTotal - skip));
+                const uint32_t r = skip + min(lane, take - 1u);
+                uint32_t idx = 0; // first window lane whose inclusive count exceeds r
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(winIncl), static_cast<int>(idx) + step - 1));
+                    if (v <= r)
+                        idx += static_cast<uint32_t>(step);
+                }
+                const unsigned long long m = __shfl(winMask, static_cast<int>(idx));
+                const uint32_t incl = static_cast<uint32_t>(__shfl(static_cast<int>(winIncl), static_cast<int>(idx)));
+                const uint32_t bit = NthSetBit(m, r - (incl - static_cast<uint32_t>(__popcll(m))));
+                const uint32_t item = winItem + idx;
+                const uint32_t q = item / itemsPerQuery;
+                listIB = item << 6 | bit;
+                // The query's word range, loaded with the granule so that slots gather at once.
+                const uint32_t w0 = static_cast<uint32_t>(bt.qoff[q]);
+                listWN = w0 << 7 | (static_cast<uint32_t>(bt.qoff[q + 1]) - w0);
+                // A live block counts once, at its first live granule.
+                const uint32_t b0 = bit & ~static_cast<uint32_t>(kGpb - 1);
+                scored += static_cast<unsigned long long>(
+                    __popcll(__ballot(lane < take && ((m >> b0) & ((1ULL << (bit - b0)) - 1ULL)) == 0)));
+                listN = take;
+                listAt = 0;
+                skip += take;
+                left -= take;
             }
-            if (cand == 0)
-                break;
-            const int i = __ffsll(static_cast<long long>(cand)) - 1;
-            cand &= cand - 1;
-            const uint64_t gp = listAt - listStep + static_cast<uint64_t>(i) * gridDim.x;
-            const uint32_t task = static_cast<uint32_t>(gp / kGpb), b = static_cast<uint32_t>(gp % kGpb);
-            // A live block counts once, for its first live granule.
-            scored += (ReadLane(maskV, i) & ((1u << b) - 1u)) == 0 ? 1u : 0u;
-            const uint32_t q = task / static_cast<uint32_t>(blocks);
-            const uint32_t blk = task - q * static_cast<uint32_t>(blocks);
-            const uint32_t w0i = ReadLane(w0V, i);
-            const uint32_t nwi = ReadLane(nwV, i);
+            const int i = static_cast<int>(listAt);
+            const uint32_t ib = ReadLane(listIB, i), wn = ReadLane(listWN, i);
+            const uint32_t qi = (ib >> 6) / itemsPerQuery;
             if (static_cast<int>(lane) == ns) {
-                slotQ = q;
-                slotG = blk * kGpb + b;
-                slotW0 = w0i;
-                slotNw = nwi;
+                slotQ = qi;
+                slotG = ((ib >> 6) - qi * itemsPerQuery) * kWave + (ib & 63u);
+                slotW0 = wn >> 7;
+                slotNw = wn & 127u;
             }
             ++ns;
+            ++listAt;
         }
         if (ns == 0)
             break;
@@ -1115,8 +1399,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         const int m = __popcll(__ballot(isSlot && pairEnd <= static_cast<uint32_t>(kWave)));
         const uint32_t pairs = ReadLane(pairEnd, m - 1);
         const uint32_t pairStart = pairEnd - nw;
+        static_assert((kLiveSlots * kGranule) % (4 * kWave) == 0, "live accumulator size");
 #pragma unroll
-        for (int r = 0; r < kLiveSlots; ++r)
+        for (int r = 0; r < kLiveSlots * kGranule / (4 * kWave); ++r)
             acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
 
         // 3. Lane p < pairs: pair p = (slot s, word j), slot-major. Its postings in the slot's
@@ -1369,16 +1654,33 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 
         BM25MI_LPHASE(3);
         // 5. Per slot: append the docs scoring at least its floor to its query's pool, at most
-        // min(K, 256) of them: past the granule's K-th key a doc has K docs of its query above
-        // it, so the cut keeps exactly the granule's top K and raises the query's bound tau
-        // (a threshold query's pool holds min(K, 256) entries per granule).
-        const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
-        LdsUint4 *bitsL = (LdsUint4 *)bits4; // acc is in LDS
+        // min(K, kGranule) of them: past the granule's K-th key a doc has K docs of its query
+        // above it, so the cut keeps exactly the granule's top K and raises the query's bound tau
+        // (a threshold query's pool holds min(K, kGranule) entries per granule). Lane l holds the
+        // slot's entries l * kVpl .. l * kVpl + kVpl - 1.
+        constexpr int kVpl = kGranule / kWave;
+        static_assert(kVpl == 2 || kVpl == 4, "values per lane of a live accumulator");
+        const auto *accBits = reinterpret_cast<const uint32_t *>(acc);
         for (int r = 0; r < m; ++r) {
             const uint32_t fl = ReadLane(floorS, r);
-            const uint4 v = bits4[r * kWave + lane];
-            uint32_t mine = (v.x >= fl ? 1u : 0u) + (v.y >= fl ? 1u : 0u) + (v.z >= fl ? 1u : 0u) +
-                            (v.w >= fl ? 1u : 0u);
+            uint32_t vals[kVpl];
+            if constexpr (kVpl == 2) {
+                const uint2 v2 = reinterpret_cast<const uint2 *>(accBits + r * kGranule)[lane];
+                vals[0] = v2.x;
+                vals[1] = v2.y;
+            } else {
+                const uint4 v4 = reinterpret_cast<const uint4 *>(accBits + r * kGranule)[lane];
+                vals[0] = v4.x;
+                vals[1] = v4.y;
+                vals[2] = v4.z;
+                vals[3] = v4.w;
+            }
+            uint32_t mine = 0, mx = 0;
+#pragma unroll
+            for (int c = 0; c < kVpl; ++c) {
+                mine += vals[c] >= fl ? 1u : 0u;
+                mx = max(mx, vals[c]);
+            }
             uint32_t kept = WaveSum(mine);
             if (kept == 0)
                 continue;
@@ -1387,8 +1689,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             const uint32_t capG = min(kq, static_cast<uint32_t>(kGranule));
             uint32_t tb = fl, te = 0;
             if (kept > capG) {
-                const uint32_t mx = max(max(v.x, v.y), max(v.z, v.w));
-                const CutResult cr = SelectCut<kGranule>(bitsL + r * kWave, fl, kq, capG, mx);
+                const CutResult cr = SelectCut<kGranule>((LdsU32 *)(accBits + r * kGranule), fl, kq, capG, mx);
                 tb = cr.tb;
                 te = cr.te;
                 kept = cr.kept;
@@ -1396,7 +1697,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 if (lane == 0 && cr.tauNew)
                     atomicMax(wk.tau + q, cr.tauNew);
             }
-            const uint32_t doc0 = (ReadLane(slotG, r) << kGranuleShift) + lane * 4;
+            const uint32_t doc0 = (ReadLane(slotG, r) << kGranuleShift) + lane * kVpl;
             uint32_t slot0 = 0;
             if (lane == 0)
                 slot0 = atomicAdd(wk.count + q, kept);
@@ -1410,13 +1711,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             uint32_t slot = ReadLane(slot0, 0) + before - mine;
             unsigned long long *pool = wk.pool + bt.poolOff[q];
             const int64_t capq = bt.poolOff[q + 1] - bt.poolOff[q];
-            const uint32_t vals[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int i = 0; i < kHistPerLane; ++i)
                 sHist[lane * kHistPerLane + i] = 0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (vals[c] > tb || (vals[c] == tb && static_cast<uint32_t>(kGranule - 1) - (lane * 4 + c) >= te)) {
+            for (int c = 0; c < kVpl; ++c)
+                if (vals[c] > tb ||
+                    (vals[c] == tb && static_cast<uint32_t>(kGranule - 1) - (lane * kVpl + c) >= te)) {
                     if (static_cast<int64_t>(slot) < capq)
                         pool[slot] = (static_cast<unsigned long long>(vals[c]) << 32) |
                                      static_cast<unsigned long long>(0xFFFFFFFFu - (doc0 + c));
@@ -1442,18 +1743,22 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         slotNw = static_cast<uint32_t>(__shfl(static_cast<int>(slotNw), static_cast<int>(min(lane + m, 63u))));
         ns -= m;
     }
-    // This wave's counter row: row gridDim.x + blockIdx.x when the plain kernel runs beside this
-    // one (it writes rows [0, gridDim.x)), else row blockIdx.x.
+    // This wave's counter row: rowBase + blockIdx.x (rowBase = gridDim.x when the plain kernel's
+    // waves write rows [0, gridDim.x) of the same Search).
     useful = WaveSum64(useful);
     if (lane == 0) {
-        const int64_t at = (bt.header[2] > 0 ? gridDim.x : 0u) + blockIdx.x;
+        const int64_t at = static_cast<int64_t>(rowBase) + blockIdx.x;
         unsigned long long *row = wk.stats + at * kStats;
         for (int i = 0; i < kStats; ++i)
             row[i] = 0;
         row[kStatPostings] = scanned;
         row[kStatScored] = scored;
         // The planning kernel ran with twice this grid: waves b and b + grid of it.
-        row[kStatSkipped] = listEnd ? wk.dead[blockIdx.x] + wk.dead[blockIdx.x + gridDim.x] : 0u;
+        // The planning kernel's dead-block counts (its grid differs from this one).
+        unsigned long long deadBlocks = 0;
+        for (int32_t b = static_cast<int32_t>(blockIdx.x); b < pruneGrid; b += static_cast<int32_t>(gridDim.x))
+            deadBlocks += wk.dead[b];
+        row[kStatSkipped] = deadBlocks;
         row[kStatCandidates] = emitted;
         row[kStatEmitting] = emitting;
         row[kStatUseful] = useful;
@@ -1468,6 +1773,66 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 #endif
     }
 #undef BM25MI_LPHASE
+}
+
+constexpr int kStageThreads = 256;
+constexpr int kStageBatch = 8; // 16-byte loads in flight per thread
+
+// Staging of a Search's request packet (no reference counterpart): the workgroups copy the
+// packet's ranges in use from pinned, device-mapped, coherent host memory to the packet's device
+// copy, in stream order before the kernels that read it. Unit u of the ranges' concatenation goes
+// to thread u % (threads of the grid); each thread issues kStageBatch loads before its stores, so
+// a packet of up to threads * kStageBatch units costs one host-memory round trip. For learned
+// batches (predict != 0; the host packet's thresholds are not staged) wave w of workgroup b then
+// predicts the threshold of query 4b + w (PredictThreshold, reading the query's inputs from the
+// host packet) and writes it into the device packet.
+__global__ __launch_bounds__(kStageThreads) void KStage(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                        StageRanges r, BatchView host, float *thrOut, ModelView mv,
+                                                        int32_t predict)
+{
+    uint32_t total = 0;
+    for (int k = 0; k < r.n; ++k)
+        total += r.len[k];
+    const uint32_t threads = gridDim.x * kStageThreads, me = blockIdx.x * kStageThreads + threadIdx.x;
+    for (uint32_t u0 = 0; u0 < total; u0 += threads * kStageBatch) {
+        uint4 v[kStageBatch];
+        uint32_t at[kStageBatch];
+#pragma unroll
+        for (int j = 0; j < kStageBatch; ++j) {
+            const uint32_t u = u0 + j * threads + me;
+            at[j] = 0xFFFFFFFFu;
+            if (u < total) {
+                uint32_t rest = u;
+                int k = 0;
+                while (rest >= r.len[k]) {
+                    rest -= r.len[k];
+                    ++k;
+                }
+                at[j] = r.off[k] + rest;
+                v[j] = src[at[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kStageBatch; ++j)
+            if (at[j] != 0xFFFFFFFFu)
+                dst[at[j]] = v[j];
+    }
+    if (!predict)
+        return;
+    const int q = static_cast<int>(blockIdx.x * (kStageThreads / kWave) + (threadIdx.x >> 6));
+    if (q >= host.header[0])
+        return;
+    const float thr = PredictThreshold(mv, host, q);
+    if ((threadIdx.x & 63) == 0)
+        thrOut[q] = thr;
+}
+
+void LaunchStage(const uint4 *src, uint4 *dst, const StageRanges &ranges, const BatchView &host, float *thrOut,
+                 const ModelView &mv, int32_t predictQueries, hipStream_t stream)
+{
+    const int grid = predictQueries > 0 ? (predictQueries + kStageThreads / kWave - 1) / (kStageThreads / kWave) : 1;
+    hipLaunchKernelGGL(KStage, dim3(grid), dim3(kStageThreads), 0, stream, src, dst, ranges, host, thrOut, mv,
+                       predictQueries > 0 ? 1 : 0);
 }
 
 // Initial state of the per-query counters; run once at session creation. After that
@@ -1690,19 +2055,19 @@ void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 }
 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, int32_t addRows)
 {
     // With events, hipExtLaunchKernelGGL stamps them at the kernel's own start and end (the
     // interval the profiler reports), not around the launch.
     switch (blockDocs) {
     case 2048:
-        hipExtLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks);
+        hipExtLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows);
         return true;
     case 4096:
-        hipExtLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks);
+        hipExtLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows);
         return true;
     case 8192:
-        hipExtLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks);
+        hipExtLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows);
         return true;
     default:
         return false;
@@ -1728,21 +2093,69 @@ bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const 
 }
 
 bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                int32_t grid, hipStream_t stream)
+                int32_t grid, int32_t rowBase, int32_t pruneGrid, hipStream_t stream)
 {
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KLive<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KLive<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneGrid);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KLive<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KLive<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneGrid);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KLive<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KLive<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneGrid);
         return true;
     default:
         return false;
     }
+}
+
+void LaunchPlan(const BatchView &bt, const WorkView &wk, int64_t granules, int32_t liveGrid, hipStream_t stream)
+{
+    const uint32_t itemsPerQuery = static_cast<uint32_t>((granules + kWave - 1) / kWave);
+    hipLaunchKernelGGL(KPlan, dim3(kPlanGroups), dim3(kPlanThreads), 0, stream, bt, wk, itemsPerQuery, liveGrid);
+}
+
+// Resident planning-kernel waves per CU, or 0.
+int PruneBlocksPerCu(int blockDocs)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (blockDocs) {
+    case 2048:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<2048>, kWave, 0);
+        break;
+    case 4096:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<4096>, kWave, 0);
+        break;
+    case 8192:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<8192>, kWave, 0);
+        break;
+    default:
+        break;
+    }
+    return e == hipSuccess ? n : 0;
+}
+
+// Resident live-kernel waves per CU for a block size, or 0.
+int LiveBlocksPerCu(int blockDocs)
+{
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (blockDocs) {
+    case 2048:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KLive<2048>, kWave, 0);
+        break;
+    case 4096:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KLive<4096>, kWave, 0);
+        break;
+    case 8192:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KLive<8192>, kWave, 0);
+        break;
+    default:
+        break;
+    }
+    return e == hipSuccess ? n : 0;
 }
 
 // Resident scoring waves per CU for a block size (registers and LDS), or 0.
@@ -1766,10 +2179,10 @@ int ScoreBlocksPerCu(int blockDocs)
     return e == hipSuccess ? n : 0;
 }
 
-void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
+void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t workgroups, unsigned long long *hostStats,
                  int32_t scoreGrid, hipStream_t stream)
 {
-    hipLaunchKernelGGL(KFinal, dim3(maxQ), dim3(kFinalThreads), 0, stream, bt, wk, hostStats, scoreGrid);
+    hipLaunchKernelGGL(KFinal, dim3(workgroups), dim3(kFinalThreads), 0, stream, bt, wk, hostStats, scoreGrid);
 }
 
 } // namespace bm25mi

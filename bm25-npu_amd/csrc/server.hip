@@ -37,6 +37,7 @@ struct Request {
     int32_t k = 0;
     float thr = 0.0F;       // pruning threshold (0 = plain); a pruned run that keeps fewer than K
                             // docs is requeued at the head as a plain request
+    bool learned = false;   // threshold predicted on the device by the session's model
     bool fellBack = false;
     Clock::time_point submitted, finished;
     std::vector<float> scores;
@@ -72,9 +73,11 @@ public:
         session_.serving = false;
     }
 
-    uint64_t Submit(const int32_t *terms, int32_t n, int32_t k, float thr = 0.0F)
+    uint64_t Submit(const int32_t *terms, int32_t n, int32_t k, float thr = 0.0F, bool learned = false)
     {
         Require(std::isfinite(thr) && thr >= 0.0F, "Threshold must be finite and >= 0");
+        if (learned && !session_.dModel)
+            throw InvalidState("Learned requests need a threshold model (bm25mi_set_threshold_model)");
         Require(n >= 0 && n <= kMaxTerms && n <= session_.cfg.max_words, "Query exceeds word capacity or 64 terms");
         Require(n == 0 || terms, "Null query terms");
         Require(k >= 1 && k <= session_.cfg.max_k && k <= session_.ndocs, "K out of range");
@@ -82,14 +85,17 @@ public:
         // A request the session would score on the plain path anyway (K above kLiveMaxK), or
         // whose threshold pool would not fit a slot, is queued as a plain request: admission
         // then reserves only the plain pool for it.
-        if (k > kLiveMaxK || session_.PoolCapacity(k, true) > session_.poolEntries)
+        if (k > session_.liveMaxK || session_.PoolCapacity(k, true) > session_.poolEntries) {
             thr = 0.0F;
+            learned = false;
+        }
         for (int32_t i = 0; i < n; ++i)
             Require(terms[i] >= 0 && terms[i] < INT32_MAX, "Invalid public word ID");
         auto req = std::make_unique<Request>();
         req->terms.assign(terms, terms + n);
         req->k = k;
         req->thr = thr;
+        req->learned = learned;
         uint64_t ticket;
         {
             std::lock_guard<std::mutex> g(mutex_);
@@ -157,21 +163,31 @@ private:
     };
 
     // Admits the oldest waiting requests into a batch for a free slot (called with the lock).
+    // A batch holds learned and plain requests, or threshold and plain requests: a learned
+    // request and one with a threshold of its own wait for different batches.
     bool TakeBatch(std::vector<uint64_t> &tickets, std::vector<int64_t> &offsets, std::vector<int32_t> &terms,
-                   std::vector<int32_t> &ks, std::vector<float> &thr)
+                   std::vector<int32_t> &ks, std::vector<float> &thr, std::vector<uint8_t> &predict, bool &learned)
     {
         tickets.clear();
         offsets.assign(1, 0);
         terms.clear();
         ks.clear();
         thr.clear();
+        predict.clear();
+        learned = false;
+        bool withThr = false;
         int64_t pool = 0;
         while (!queue_.empty() && static_cast<int32_t>(tickets.size()) < maxBatch_) {
             const Request &r = *requests_.at(queue_.front());
-            const int64_t need = session_.PoolCapacity(r.k, r.thr > 0.0F);
+            const int64_t need = session_.PoolCapacity(r.k, r.thr > 0.0F || r.learned);
             if (static_cast<int64_t>(terms.size() + r.terms.size()) > session_.cfg.max_words ||
                 pool + need > session_.poolEntries)
                 break; // memory admission: the rest waits for the next free slot
+            if ((r.learned && withThr) || (r.thr > 0.0F && learned))
+                break;
+            learned = learned || r.learned;
+            withThr = withThr || r.thr > 0.0F;
+            predict.push_back(r.learned ? 1 : 0);
             pool += need;
             tickets.push_back(queue_.front());
             terms.insert(terms.end(), r.terms.begin(), r.terms.end());
@@ -195,6 +211,7 @@ private:
             Request &r = *requests_.at(f.tickets[i]);
             if (code == BM25MI_OK && status && (*status)[i]) {
                 r.thr = 0.0F;
+                r.learned = false;
                 r.fellBack = true;
                 ++fallbacks_;
                 queue_.push_front(f.tickets[i]);
@@ -223,7 +240,9 @@ private:
         std::vector<int64_t> offsets;
         std::vector<int32_t> terms, ks, status;
         std::vector<float> scores, thr;
+        std::vector<uint8_t> predict;
         std::vector<int32_t> ids;
+        bool learned = false;
         for (;;) {
             bool launched = false;
             {
@@ -246,13 +265,15 @@ private:
                     return;
                 if (queue_.empty() && inflight.empty())
                     continue;
-                if (!free.empty() && !failed_ && TakeBatch(tickets, offsets, terms, ks, thr)) {
+                if (!free.empty() && !failed_ && TakeBatch(tickets, offsets, terms, ks, thr, predict, learned)) {
                     lock.unlock();
                     Slot *slot = free.back();
                     free.pop_back();
                     const bool anyThr = std::any_of(thr.begin(), thr.end(), [](float t) { return t > 0.0F; });
                     Query q{offsets.data(), terms.data(), static_cast<int32_t>(ks.size()), ks.data(),
                             anyThr ? thr.data() : nullptr};
+                    q.learned = learned;
+                    q.predict = learned ? predict.data() : nullptr;
                     InFlight f{slot, tickets, ks};
                     try {
                         session_.Pack(q, *slot);
@@ -380,6 +401,15 @@ extern "C" int bm25mi_server_submit_pruned(bm25mi_server *srv, const int32_t *te
     });
 }
 
+extern "C" int bm25mi_server_submit_learned(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k,
+                                            uint64_t *ticket)
+{
+    return Invoke([&] {
+        Require(srv && srv->impl && ticket, "Null server or ticket");
+        *ticket = srv->impl->Submit(terms, n, k, 0.0F, true);
+    });
+}
+
 extern "C" int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *scores, int32_t *ids,
                                   double *latency_us)
 {
@@ -394,12 +424,13 @@ extern "C" int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *sc
 // second one. latency_us[i] = submit to completion; scores/ids (optional) receive the rows
 // of request i at offset sum_{j<i} ks[j].
 extern "C" int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offsets, const int32_t *terms,
-                                         int32_t n, const int32_t *ks, const float *thresholds, double qps,
-                                         uint64_t seed, double *latency_us, float *scores, int32_t *ids,
+                                         int32_t n, const int32_t *ks, const float *thresholds, int32_t learned,
+                                         double qps, uint64_t seed, double *latency_us, float *scores, int32_t *ids,
                                          double *elapsed_s)
 {
     return Invoke([&] {
         Require(srv && srv->impl && offsets && ks && latency_us && n >= 1 && qps > 0, "Invalid Poisson run");
+        Require(!(learned && thresholds), "A learned run predicts its own thresholds");
         std::vector<int64_t> rowOf(static_cast<size_t>(n) + 1, 0);
         for (int32_t i = 0; i < n; ++i)
             rowOf[i + 1] = rowOf[i] + ks[i];
@@ -434,7 +465,7 @@ extern "C" int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offs
                 std::this_thread::yield();
             try {
                 tickets[i] = srv->impl->Submit(terms + offsets[i], static_cast<int32_t>(offsets[i + 1] - offsets[i]),
-                                               ks[i], thresholds ? thresholds[i] : 0.0F);
+                                               ks[i], thresholds ? thresholds[i] : 0.0F, learned != 0);
             } catch (const std::exception &e) {
                 std::lock_guard<std::mutex> g(failMutex);
                 if (failure.empty())

@@ -43,30 +43,34 @@ struct Query {
     int32_t count;
     const int32_t *k;   // per query
     const float *thr;   // per query or nullptr
+    bool learned = false; // thresholds predicted on the device by the resident model
+    const uint8_t *predict = nullptr; // learned batches: per query 1 = predict, 0 = plain (nullptr: all)
 };
 
 // One in-flight Search: request packet, candidate pools, per-query bound state, per-wave
 // counters, result block, stream and captured graph.
 struct Slot {
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr, *dLive = nullptr, *dDead = nullptr, *dCursor = nullptr;
-    void *hPacket = nullptr, *hResult = nullptr, *hResultDev = nullptr;
+    void *dHist = nullptr, *dItems = nullptr, *dDead = nullptr, *dCursor = nullptr, *dPlan = nullptr;
+    void *hPacket = nullptr, *hPacketDev = nullptr, *hResult = nullptr, *hResultDev = nullptr;
+    StageRanges ranges{}; // the packet ranges the staging kernel copies for the packed batch
+    BatchView btHost{};   // the packet's arrays in its mapped host memory (staging kernel)
     BatchView bt{};
     WorkView wk{};
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;          // live scoring beside plain scoring (mixed batches)
-    hipEvent_t fork[2] = {nullptr, nullptr};
     // Captured Searches by batch mode: [0] plain queries only; [2] threshold queries only
-    // (planning, live scoring). Mode 1 (both kinds: planning, then plain scoring beside live
-    // scoring on the side stream) is launched directly, [1] stays empty.
-    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
-    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    // (planning, live plan, live scoring); [3] learned (captured once the session has a model).
+    // Mode 1 (both kinds) is launched directly, [1] stays empty.
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // The launched batch: query count, output row offsets, plain-path postings, timing mode.
     int32_t nq = 0;
-    int mode = 0; // graph of the packed batch (index into exec)
+    int mode = 0; // 0 plain, 1 mixed, 2 threshold queries only, 3 learned (thresholds predicted on the
+                  // device); modes 0, 2 and 3 may replay a captured graph (exec[mode])
     std::vector<int64_t> out; // [maxQ+1]
     int64_t total = 0;
+    int32_t finalGrid = 0; // final-kernel workgroups of the launched Search (its counter rows)
     // Per-query host state of Pack and the status rows of Complete, sized to the session's
     // max_queries at create time so that a Search allocates no host memory.
     std::vector<float> thrUse;
@@ -83,6 +87,8 @@ public:
     int64_t ndocs = 0, vocab = 0, postings = 0;
     int32_t blockDocs = 2048, granule = kGranule, blocks = 0, grid = 0, cus = 0;
     int32_t gridCapacity = 0; // the packet and counter rows hold this many scoring waves
+    int32_t liveGrid = 0;     // live-kernel waves (as many as are resident at once)
+    int32_t pruneCapacity = 0; // planning-kernel waves resident at once (its largest grid)
     int64_t poolEntries = 0; // candidate-pool entries of each slot
     int64_t taskCost = kTaskCost;
     std::vector<int64_t> toffHost;
@@ -97,6 +103,12 @@ public:
     IndexView ix{};
     size_t packetBytes = 0, resultBytes = 0;
     size_t offQoff = 0, offK = 0, offThr = 0, offTau0 = 0, offPool = 0, offOut = 0, offTerms = 0, offCost = 0;
+    size_t offFeat = 0, offUpper = 0, offPlanCost = 0;
+    int64_t itemsPerQuery = 0; // 64-granule items per query (planning kernel)
+    int32_t liveMaxK = kLiveMaxK; // largest K a threshold query takes the live path with
+    // The resident threshold model of learned Searches (bm25mi_set_threshold_model).
+    void *dModel = nullptr;
+    float modelYMean = 0.0F, modelYScale = 1.0F, modelSafety = 1.0F;
     size_t resStatus = 0, resScores = 0, resIds = 0;
     std::vector<Slot> slots;
     int device = 0;
@@ -122,11 +134,16 @@ public:
     // The wave plan: the prefix of per-query task costs (a task of query q costing taskCost +
     // ceil(postings of q / blocks), threshold queries nothing); every scoring wave cuts its own
     // contiguous range of equal estimated work out of the query-major task order (WaveRange).
+    // The learned model's kFeatures inputs for one query (index statistics of its terms), and
+    // the FP32 sum of its terms' largest impacts in query order (upperOut, optional).
+    void Features(const int32_t *terms, int64_t n, int32_t k, float *out, float *upperOut) const;
     void PlanWaves(const int64_t *postingsPerQuery, const uint8_t *listed, int32_t nq, int64_t *costPrefix) const;
     // Validates a whole request (throws InvalidArgument, touching nothing, when it is not
     // acceptable), then writes it into the slot's packet. The batch's candidate pools must fit
     // poolBudget entries (default: the slot's pool).
     void Pack(const Query &query, Slot &slot, int64_t poolBudget = -1);
+    // A learned batch's query q whose threshold the device predicts (the others take the plain path).
+    bool Predicted(const Query &query, int32_t q) const;
     // Queues the slot's Search on its stream (graph replay, or direct launches with events
     // around each kernel when timed). Returns at once.
     void Launch(Slot &slot, bool timed);
@@ -142,14 +159,23 @@ public:
     // candidate pools) in the library's static scratch array instead of slot 0's buffers, and
     // no allocation of any kind; a batch whose pools do not fit is rejected.
     void SearchScratch(const Query &query, float *outScores, int32_t *outIds, int32_t *status);
+    // The thresholds a learned Search of the batch would use, predicted on the device (0 = the
+    // query takes the plain path).
+    void PredictThresholds(const Query &query, float *out);
     // Changes the scoring launch (waves, schedule task cost) and recaptures the slots' graphs.
     void SetLaunch(int32_t waves, int64_t cost);
 
+    // Captures the slot's graphs that do not exist yet (modes 0 and 2; 3 once a model is set).
+    void Capture(Slot &slot);
+
 private:
     void InitSlot(Slot &slot);
-    void Capture(Slot &slot);
-    void Enqueue(Slot &slot, bool timed, int mode);
+    void Enqueue(Slot &slot, bool timed, int mode, bool capture = false);
     void Bootstrap();
+    ModelView Model() const;
+    StageRanges FullRanges(bool learned) const;
+    BatchView PacketView(const unsigned char *packet) const;
+    int32_t FinalGrid(int32_t nq) const;
     template <class T> T *Alloc(size_t count, const char *what)
     {
         void *p = nullptr;
@@ -159,16 +185,29 @@ private:
 };
 
 void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream);
+// Copies the ranges of a request packet from mapped host memory to its device copy.
+// Learned batches (predictQueries > 0) also get their thresholds predicted, from the host packet
+// view `host`, into thrOut (one wave per query).
+void LaunchStage(const uint4 *src, uint4 *dst, const StageRanges &ranges, const BatchView &host, float *thrOut,
+                 const ModelView &mv, int32_t predictQueries, hipStream_t stream);
 // start/stop (optional): events stamped at the kernel's own start and end.
+// addRows: add the counters to the rows of the live kernel that ran before it in the same Search.
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
-// Live granules of threshold queries, after LaunchPrune and LaunchScore with the same grid.
+                 int32_t grid, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
+                 int32_t addRows = 0);
+// Live granules of threshold queries, after LaunchPrune (with pruneGrid waves) and LaunchPlan
+// (for this grid); counter rows from rowBase on.
 bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                int32_t grid, hipStream_t stream);
+                int32_t grid, int32_t rowBase, int32_t pruneGrid, hipStream_t stream);
+int LiveBlocksPerCu(int blockDocs);
+int PruneBlocksPerCu(int blockDocs);
+// The live kernel's per-wave shares of the live granules, after LaunchPrune.
+void LaunchPlan(const BatchView &bt, const WorkView &wk, int64_t granules, int32_t liveGrid, hipStream_t stream);
 bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream);
 int ScoreBlocksPerCu(int blockDocs);
-void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t maxQ, unsigned long long *hostStats,
+// One workgroup per query slot (workgroups beyond the batch's queries only sum counter rows).
+void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t workgroups, unsigned long long *hostStats,
                  int32_t scoreGrid, hipStream_t stream);
 
 } // namespace bm25mi

@@ -110,6 +110,71 @@ uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
     return bits;
 }
 
+void Session::Features(const int32_t *terms, int64_t n, int32_t k, float *out, float *upperOut) const
+{
+    // Per-term statistics of the query's known terms (IDs >= vocab skipped, repeats counted),
+    // pooled per query. kth = the term's K-th largest impact when the session keeps it (0
+    // otherwise); the largest kth is the provable lower bound of the K-th score.
+    constexpr int kTop = 3;
+    float m[kTop] = {0.0F, 0.0F, 0.0F}; // largest term maxima, descending
+    float upper = 0.0F, lower = 0.0F, second = 0.0F;
+    double sumKth = 0, sumMean = 0, sumIdf = 0, sumDf = 0, maxDf = 0, minDf = -1, density = 0, logAll = 0;
+    int known = 0;
+    const double nd = static_cast<double>(ndocs);
+    for (int64_t w = 0; w < n; ++w) {
+        const int32_t t = terms[w];
+        if (t >= vocab)
+            continue;
+        ++known;
+        const float mx = tmaxHost[t];
+        upper += mx; // sequential FP32 sum in query order: no doc scores above it
+        for (int i = 0; i < kTop; ++i)
+            if (mx > m[i]) {
+                for (int j = kTop - 1; j > i; --j)
+                    m[j] = m[j - 1];
+                m[i] = mx;
+                break;
+            }
+        const float kth = topOffHost[t + 1] - topOffHost[t] >= k ? topValHost[topOffHost[t] + k - 1] : 0.0F;
+        if (kth > lower) {
+            second = lower;
+            lower = kth;
+        } else if (kth > second) {
+            second = kth;
+        }
+        const double df = static_cast<double>(toffHost[t + 1] - toffHost[t]);
+        sumKth += kth;
+        sumMean += tmeanHost[t];
+        sumIdf += idfHost[t];
+        sumDf += df;
+        maxDf = std::max(maxDf, df);
+        minDf = minDf < 0 ? df : std::min(minDf, df);
+        density += mx * df / nd;
+        logAll += std::log(std::max(df, 0.5) / nd);
+    }
+    const auto lg = [](double v) { return std::log(std::max(v, 1e-30)); };
+    const double vals[kFeatures] = {std::log(static_cast<double>(k)),
+                                    static_cast<double>(known),
+                                    lg(upper),
+                                    lg(lower),
+                                    lg(second),
+                                    m[0],
+                                    m[1],
+                                    m[2],
+                                    sumKth,
+                                    sumMean,
+                                    sumIdf,
+                                    std::log1p(sumDf),
+                                    std::log1p(maxDf),
+                                    std::log1p(std::max(minDf, 0.0)),
+                                    density,
+                                    std::max(-40.0, logAll)};
+    for (int i = 0; i < kFeatures; ++i)
+        out[i] = static_cast<float>(vals[i]);
+    if (upperOut)
+        *upperOut = upper;
+}
+
 void Session::PlanWaves(const int64_t *postingsPerQuery, const uint8_t *listed, int32_t nq, int64_t *costPrefix) const
 {
     costPrefix[0] = 0;
@@ -132,6 +197,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(cfg.task_cost >= 0, "task_cost must be >= 0");
     Require(cfg.slots >= 0 && cfg.slots <= 16, "slots must be in [0, 16]");
     Require(cfg.pool_entries >= 0, "pool_entries must be >= 0");
+    Require(cfg.live_max_k >= 0 && cfg.live_max_k <= BM25MI_MAX_K, "live_max_k must be in [0, 1024]");
     if (cfg.task_cost > 0)
         taskCost = cfg.task_cost;
     blockDocs = cfg.block_docs ? cfg.block_docs : 2048;
@@ -172,8 +238,20 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
     Require(grid >= 1 && grid <= (1 << 20), "grid_blocks out of range");
-    gridCapacity = std::max(grid, 2 * cus * perCu); // counter rows: 2 x grid (plain and live kernels)
+    const int prunePerCu = PruneBlocksPerCu(blockDocs);
+    Require(prunePerCu >= 1, "The planning kernel does not fit this device");
+    pruneCapacity = cus * prunePerCu;
+    const int livePerCu = LiveBlocksPerCu(blockDocs);
+    Require(livePerCu >= 1, "The live kernel does not fit this device");
+    liveGrid = cus * livePerCu;
+    // Counter rows, live plans and dead counts: 2 x the largest grid.
+    gridCapacity = std::max(std::max(grid, liveGrid), 2 * cus * perCu);
     Require(static_cast<int64_t>(cfg.max_queries) * blocks < (int64_t(1) << 32), "Too many scoring tasks per Search");
+    itemsPerQuery = (granules + kWave - 1) / kWave;
+    // Live granules are ranked in 32 bits (planning and live kernels).
+    Require(static_cast<int64_t>(cfg.max_queries) * itemsPerQuery * kWave < (int64_t(1) << 32),
+            "Too many granules per Search");
+    liveMaxK = cfg.live_max_k > 0 ? cfg.live_max_k : kLiveMaxK;
 
     // Candidate pools: by default room for max_queries queries of K = max_k in every slot;
     // a smaller pool_entries budget makes callers admit fewer large-K queries per batch.
@@ -187,7 +265,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     const uint64_t known = static_cast<uint64_t>(m) * 8 + static_cast<uint64_t>(m) / 4 + 4 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
                            static_cast<uint64_t>(nslots) * static_cast<uint64_t>(poolEntries) * 8 +
-                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks * 4 +
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks +
                            static_cast<uint64_t>(nslots) * static_cast<uint64_t>(std::max<int64_t>(1, cfg.max_words)) * blocks * 4;
     // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
     Require(known <= static_cast<uint64_t>(freeBytes) / 5 * 4, "Index, pools and arenas exceed 80% of free HBM");
@@ -226,25 +304,32 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
                    static_cast<const uint32_t *>(dDir), static_cast<const float *>(dBmax), granules,
                    static_cast<int32_t>(n)};
 
-    // Request packet layout (the same in every slot).
+    // Request packet layout (the same in every slot); every array starts on 16 bytes (staging).
     const size_t q = static_cast<size_t>(cfg.max_queries);
     size_t at = 16;
     offQoff = at;
-    at = Align(at + (q + 1) * 8, 8);
+    at = Align(at + (q + 1) * 8, 16);
     offK = at;
-    at = Align(at + q * 4, 8);
+    at = Align(at + q * 4, 16);
     offThr = at;
-    at = Align(at + q * 4, 8);
+    at = Align(at + q * 4, 16);
     offTau0 = at;
-    at = Align(at + q * 4, 8);
+    at = Align(at + q * 4, 16);
     offPool = at;
-    at = Align(at + (q + 1) * 8, 8);
+    at = Align(at + (q + 1) * 8, 16);
     offOut = at;
-    at = Align(at + (q + 1) * 8, 8);
+    at = Align(at + (q + 1) * 8, 16);
     offTerms = at;
     at = Align(at + std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * 16, 16);
     offCost = at;
-    at = Align(at + (q + 1) * 8, 256);
+    at = Align(at + (q + 1) * 8, 16);
+    // Learned batches only: per-query model features, score upper bounds and plain-path costs.
+    offFeat = at;
+    at = Align(at + q * kFeatures * 4, 16);
+    offUpper = at;
+    at = Align(at + q * 4, 16);
+    offPlanCost = at;
+    at = Align(at + q * 8, 256);
     packetBytes = at;
     // Result block layout: per final-workgroup counter rows, status, scores, IDs.
     resStatus = q * kStats * 8;
@@ -267,13 +352,14 @@ void Session::InitSlot(Slot &slot)
     slot.listed.assign(q, 0);
     slot.status.assign(q, 0);
     BM25MI_HIP(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking), "create stream");
-    BM25MI_HIP(hipStreamCreateWithFlags(&slot.side, hipStreamNonBlocking), "create side stream");
-    for (auto &e : slot.fork)
-        BM25MI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "create fork event");
     for (auto &e : slot.ev)
         BM25MI_HIP(hipEventCreate(&e), "create event");
-    BM25MI_HIP(hipHostMalloc(&slot.hPacket, packetBytes, hipHostMallocDefault), "alloc pinned packet");
+    // Coherent (fine-grained) and mapped: the staging kernel reads the packet from it directly.
+    BM25MI_HIP(hipHostMalloc(&slot.hPacket, packetBytes, hipHostMallocMapped | hipHostMallocCoherent),
+               "alloc pinned packet");
     std::memset(slot.hPacket, 0, packetBytes);
+    BM25MI_HIP(hipHostGetDevicePointer(&slot.hPacketDev, slot.hPacket, 0), "map pinned packet");
+    slot.ranges = FullRanges(false);
     slot.dPacket = Alloc<unsigned char>(packetBytes, "alloc device packet");
     BM25MI_HIP(hipHostMalloc(&slot.hResult, resultBytes, hipHostMallocMapped), "alloc pinned result");
     std::memset(slot.hResult, 0, resultBytes);
@@ -283,28 +369,22 @@ void Session::InitSlot(Slot &slot)
     slot.dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
     slot.dStats = Alloc<unsigned long long>(2 * static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
-    slot.dLive = Alloc<uint32_t>(q * static_cast<size_t>(blocks), "alloc live masks");
-    slot.dDead = Alloc<uint32_t>(2 * static_cast<size_t>(gridCapacity), "alloc dead counts");
+    slot.dItems = Alloc<unsigned long long>(q * static_cast<size_t>(itemsPerQuery), "alloc live item masks");
+    slot.dPlan = Alloc<uint4>(static_cast<size_t>(gridCapacity), "alloc live plan");
+    slot.dDead = Alloc<uint32_t>(static_cast<size_t>(pruneCapacity), "alloc dead counts");
     slot.dCursor = Alloc<uint32_t>(std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * static_cast<size_t>(blocks),
                                    "alloc short-list cursors");
-    auto *pk = static_cast<unsigned char *>(slot.dPacket);
-    slot.bt = BatchView{reinterpret_cast<const int32_t *>(pk),
-                        reinterpret_cast<const int64_t *>(pk + offQoff),
-                        reinterpret_cast<const int32_t *>(pk + offK),
-                        reinterpret_cast<const float *>(pk + offThr),
-                        reinterpret_cast<const uint32_t *>(pk + offTau0),
-                        reinterpret_cast<const int64_t *>(pk + offPool),
-                        reinterpret_cast<const int64_t *>(pk + offOut),
-                        reinterpret_cast<const uint4 *>(pk + offTerms),
-                        reinterpret_cast<const int64_t *>(pk + offCost)};
+    slot.bt = PacketView(static_cast<unsigned char *>(slot.dPacket));
+    slot.btHost = PacketView(static_cast<unsigned char *>(slot.hPacketDev));
     auto *rs = static_cast<unsigned char *>(slot.hResultDev);
     slot.wk = WorkView{static_cast<unsigned long long *>(slot.dPool),
                        static_cast<uint32_t *>(slot.dCount),
                        static_cast<uint32_t *>(slot.dHist),
                        static_cast<uint32_t *>(slot.dTau),
                        static_cast<unsigned long long *>(slot.dStats),
-                       static_cast<uint32_t *>(slot.dLive),
+                       static_cast<unsigned long long *>(slot.dItems),
                        static_cast<uint32_t *>(slot.dDead),
+                       static_cast<uint4 *>(slot.dPlan),
                        static_cast<uint32_t *>(slot.dCursor),
                        reinterpret_cast<int32_t *>(rs + resStatus),
                        reinterpret_cast<float *>(rs + resScores),
@@ -315,49 +395,109 @@ void Session::InitSlot(Slot &slot)
         Capture(slot);
 }
 
-void Session::Enqueue(Slot &slot, bool timed, int mode)
+StageRanges Session::FullRanges(bool learned) const
 {
-    BM25MI_HIP(hipMemcpyAsync(slot.dPacket, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
-               "packet H2D");
+    // The whole packet; for learned batches without the thresholds, which the staging kernel
+    // predicts on the device.
+    StageRanges r{};
+    if (!learned) {
+        r.off[0] = 0;
+        r.len[0] = static_cast<uint32_t>(packetBytes / 16);
+        r.n = 1;
+    } else {
+        r.off[0] = 0;
+        r.len[0] = static_cast<uint32_t>(offThr / 16);
+        r.off[1] = static_cast<uint32_t>(offTau0 / 16);
+        r.len[1] = static_cast<uint32_t>((packetBytes - offTau0) / 16);
+        r.n = 2;
+    }
+    return r;
+}
+
+BatchView Session::PacketView(const unsigned char *pk) const
+{
+    return BatchView{reinterpret_cast<const int32_t *>(pk),
+                     reinterpret_cast<const int64_t *>(pk + offQoff),
+                     reinterpret_cast<const int32_t *>(pk + offK),
+                     reinterpret_cast<const float *>(pk + offThr),
+                     reinterpret_cast<const uint32_t *>(pk + offTau0),
+                     reinterpret_cast<const int64_t *>(pk + offPool),
+                     reinterpret_cast<const int64_t *>(pk + offOut),
+                     reinterpret_cast<const uint4 *>(pk + offTerms),
+                     reinterpret_cast<const int64_t *>(pk + offCost),
+                     reinterpret_cast<const float *>(pk + offFeat),
+                     reinterpret_cast<const float *>(pk + offUpper),
+                     reinterpret_cast<const int64_t *>(pk + offPlanCost)};
+}
+
+int32_t Session::FinalGrid(int32_t nq) const
+{
+    return std::min(cfg.max_queries, std::max(nq, 8));
+}
+
+ModelView Session::Model() const
+{
+    return ModelView{static_cast<const float *>(dModel), modelYMean, modelYScale, modelSafety, kLiveMinRatio,
+                     liveMaxK};
+}
+
+void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
+{
+    // Learned batches: the staging kernel also predicts the thresholds (one wave per query).
+    const int32_t predict = mode == 3 ? (capture ? cfg.max_queries : slot.nq) : 0;
+    LaunchStage(static_cast<const uint4 *>(slot.hPacketDev), static_cast<uint4 *>(slot.dPacket),
+                capture ? FullRanges(mode == 3) : slot.ranges, slot.btHost,
+                reinterpret_cast<float *>(static_cast<unsigned char *>(slot.dPacket) + offThr), Model(), predict,
+                slot.stream);
     // Timed plain batches: events stamped by the scoring kernel itself; otherwise recorded
     // around the scoring kernels (planning, plain and live scoring).
     if (timed && mode != 0)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
-    if (mode != 0)
-        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, 2 * grid, slot.stream), "Unsupported block size");
-    if (mode == 1) {
-        // Plain and threshold queries: the live kernel runs beside the plain one on the side
-        // stream (they touch different queries' pools and counters and separate counter rows).
-        BM25MI_HIP(hipEventRecord(slot.fork[0], slot.stream), "fork live scoring");
-        BM25MI_HIP(hipStreamWaitEvent(slot.side, slot.fork[0], 0), "fork live scoring");
-        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.side), "Unsupported block size");
-        BM25MI_HIP(hipEventRecord(slot.fork[1], slot.side), "join live scoring");
-        Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
-        BM25MI_HIP(hipStreamWaitEvent(slot.stream, slot.fork[1], 0), "join live scoring");
-    } else if (mode == 0) {
+    if (mode == 0) {
         Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, timed ? slot.ev[1] : nullptr,
                             timed ? slot.ev[2] : nullptr),
                 "Unsupported block size");
     } else {
-        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream), "Unsupported block size");
+        // One stream: threshold prediction (learned batches), planning, the live plan (with the
+        // plain queries' wave plan of learned batches), live scoring, then plain scoring of the
+        // queries routed to the plain path (mixed and learned batches), which adds its counters to
+        // the live kernel's rows.
+        // One planning wave per item of the batch, up to the waves resident at once.
+        const int32_t pruneGrid = capture ? pruneCapacity
+                                          : static_cast<int32_t>(std::min<int64_t>(
+                                                std::max<int64_t>(1, static_cast<int64_t>(slot.nq) * itemsPerQuery),
+                                                pruneCapacity));
+        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, pruneGrid, slot.stream), "Unsupported block size");
+        LaunchPlan(slot.bt, slot.wk, granules, liveGrid, slot.stream);
+        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, liveGrid, 0, pruneGrid, slot.stream),
+                "Unsupported block size");
+        if (mode != 2)
+            Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, nullptr, nullptr, liveGrid),
+                    "Unsupported block size");
     }
     if (timed && mode != 0)
         BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
-    // Counter rows: [0, grid) from the plain or the only scoring kernel, [grid, 2 grid) from the
-    // live kernel when both ran.
-    LaunchFinal(slot.bt, slot.wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev),
-                mode == 1 ? 2 * grid : grid,
+    // Counter rows: [0, grid) of the plain kernel, [0, liveGrid) of the live kernel (the plain
+    // kernel adds into the live kernel's rows where both wrote).
+    const int32_t rows = mode == 0 ? grid : mode == 2 ? liveGrid : std::max(grid, liveGrid);
+    // One final workgroup per query of the batch, and at least 8 (up to the session's capacity)
+    // to share the counter rows; a captured graph has one per query of the session's capacity.
+    // Workgroups beyond the batch's queries only sum counter rows.
+    slot.finalGrid = capture ? cfg.max_queries : FinalGrid(slot.nq);
+    LaunchFinal(slot.bt, slot.wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), rows,
                 slot.stream);
 }
 
 void Session::Capture(Slot &slot)
 {
-    for (int g = 0; g < 3; ++g) {
-        if (g == 1)
-            continue; // mixed batches launch directly (Launch)
+    for (int g = 0; g < 4; ++g) {
+        // Mixed batches launch directly (Launch); learned ones are captured once the session has
+        // a model (its address and output transform are part of the graph).
+        if (g == 1 || (g == 3 && !dModel) || slot.exec[g])
+            continue;
         BM25MI_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
         try {
-            Enqueue(slot, false, g);
+            Enqueue(slot, false, g, true);
         } catch (...) {
             hipGraph_t discarded = nullptr;
             (void)hipStreamEndCapture(slot.stream, &discarded);
@@ -368,6 +508,11 @@ void Session::Capture(Slot &slot)
         BM25MI_HIP(hipStreamEndCapture(slot.stream, &slot.graph[g]), "end graph capture");
         BM25MI_HIP(hipGraphInstantiate(&slot.exec[g], slot.graph[g], nullptr, nullptr, 0), "instantiate graph");
     }
+}
+
+bool Session::Predicted(const Query &query, int32_t q) const
+{
+    return query.learned && (!query.predict || query.predict[q]) && query.k[q] <= liveMaxK;
 }
 
 void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
@@ -393,16 +538,17 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
     for (int64_t w = 0; w < words; ++w)
         Require(query.terms[w] >= 0 && query.terms[w] < INT32_MAX, "Invalid public word ID");
     // A threshold below kLiveMinRatio of the query's score upper bound (the FP32 sum of its
-    // terms' largest impacts) leaves most granules live, and above K = kLiveMaxK the live path's
+    // terms' largest impacts) leaves most granules live, and above K = liveMaxK the live path's
     // per-granule cut costs more than it saves: such a query is scored on the plain path, with
-    // the same results.
+    // the same results. Learned batches are routed on the device (KThreshold) by the same rules;
+    // their queries with K <= liveMaxK get pools that fit either path.
     // Per-query host state lives in the slot's buffers, sized at create time: packing allocates
     // nothing.
     float *thrUse = slot.thrUse.data();
     std::fill_n(thrUse, nq, 0.0F);
     int64_t poolNeed = 0;
     for (int32_t q = 0; q < nq; ++q) {
-        if (query.thr && query.thr[q] > 0.0F && query.k[q] <= kLiveMaxK) {
+        if (query.thr && query.thr[q] > 0.0F && query.k[q] <= liveMaxK) {
             float ub = 0.0F;
             for (int64_t w = query.offsets[q]; w < query.offsets[q + 1]; ++w)
                 if (query.terms[w] < vocab)
@@ -410,7 +556,7 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
             if (query.thr[q] >= kLiveMinRatio * ub)
                 thrUse[q] = query.thr[q];
         }
-        poolNeed += PoolCapacity(query.k[q], thrUse[q] > 0.0F);
+        poolNeed += PoolCapacity(query.k[q], thrUse[q] > 0.0F || Predicted(query, q));
     }
     Require(poolNeed <= poolBudget, "Batch needs more candidate-pool memory than a slot (or the no-allocation scratch) holds");
 
@@ -428,7 +574,7 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
         qoff[q + 1] = query.offsets[q + 1];
         kk[q] = query.k[q];
         thr[q] = thrUse[q];
-        pool[q + 1] = pool[q] + PoolCapacity(query.k[q], thr[q] > 0.0F);
+        pool[q + 1] = pool[q] + PoolCapacity(query.k[q], thr[q] > 0.0F || Predicted(query, q));
         out[q + 1] = out[q] + query.k[q];
         slot.out[q + 1] = out[q + 1];
         tau0[q] = StartBound(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q]);
@@ -461,8 +607,50 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
     }
     reinterpret_cast<int32_t *>(pk)[1] = withThr;
     reinterpret_cast<int32_t *>(pk)[2] = nq - withThr;
-    slot.mode = withThr == 0 ? 0 : withThr < nq ? 1 : 2;
+    reinterpret_cast<int32_t *>(pk)[3] = query.learned ? 1 : 0;
+    slot.mode = query.learned ? 3 : withThr == 0 ? 0 : withThr < nq ? 1 : 2;
+    // Learned batches: the plan with every query plain, the most the device-side routing can
+    // keep (checked here), and what the threshold kernel needs to route and re-plan.
     PlanWaves(perQuery, listed, nq, reinterpret_cast<int64_t *>(pk + offCost));
+    // The packet bytes the kernels read: each array's first nq (or words) entries; a learned
+    // batch's thresholds are predicted on the device instead. A captured graph stages the whole
+    // packet (FullRanges: its ranges are fixed at capture).
+    {
+        StageRanges &r = slot.ranges;
+        r.n = 0;
+        const auto add = [&](size_t off, size_t bytes) {
+            if (bytes)
+                r.off[r.n] = static_cast<uint32_t>(off / 16), r.len[r.n++] = static_cast<uint32_t>((bytes + 15) / 16);
+        };
+        const size_t n = static_cast<size_t>(nq);
+        add(0, 16);
+        add(offQoff, (n + 1) * 8);
+        add(offK, n * 4);
+        if (!query.learned)
+            add(offThr, n * 4);
+        add(offTau0, n * 4);
+        add(offPool, (n + 1) * 8);
+        add(offOut, (n + 1) * 8);
+        add(offTerms, static_cast<size_t>(words) * 16);
+        add(offCost, (n + 1) * 8);
+        if (query.learned) {
+            add(offFeat, n * kFeatures * 4);
+            add(offUpper, n * 4);
+            add(offPlanCost, n * 8);
+        }
+    }
+    if (query.learned) {
+        auto *feat = reinterpret_cast<float *>(pk + offFeat);
+        auto *upper = reinterpret_cast<float *>(pk + offUpper);
+        auto *planCost = reinterpret_cast<int64_t *>(pk + offPlanCost);
+        for (int32_t q = 0; q < nq; ++q) {
+            Features(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q],
+                     feat + static_cast<int64_t>(q) * kFeatures, upper + q);
+            if (!Predicted(query, q))
+                upper[q] = 0.0F; // routed to the plain path by the threshold kernel
+            planCost[q] = (taskCost + (perQuery[q] + blocks - 1) / blocks) * blocks;
+        }
+    }
     slot.nq = nq;
     slot.total = total;
 }
@@ -475,10 +663,13 @@ void Session::Launch(Slot &slot, bool timed)
         BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
     // Mixed batches launch directly: replaying the captured fork/join graph measured slower than
     // the same launches issued on the two streams (Q64 x 2-12 terms: +0.35 ms end to end).
-    if (slot.exec[slot.mode] && !timed && slot.mode != 1)
+    // Learned batches (mode 3) launch directly as well.
+    if (slot.mode != 1 && slot.exec[slot.mode] && !timed) {
         BM25MI_HIP(hipGraphLaunch(slot.exec[slot.mode], slot.stream), "replay Search graph");
-    else
+        slot.finalGrid = cfg.max_queries;
+    } else {
         Enqueue(slot, timed, slot.mode);
+    }
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[3], slot.stream), "record end");
 }
@@ -510,7 +701,7 @@ void Session::Complete(Slot &slot, float *outScores, int32_t *outIds, int32_t *s
     auto *res = static_cast<unsigned char *>(slot.hResult);
     const auto *rows = reinterpret_cast<const unsigned long long *>(res);
     unsigned long long stats[kStats] = {};
-    for (int32_t r = 0; r < cfg.max_queries; ++r)
+    for (int32_t r = 0; r < slot.finalGrid; ++r)
         for (int i = 0; i < kStats; ++i)
             stats[i] = i >= kStatMax ? std::max(stats[i], rows[r * kStats + i]) : stats[i] + rows[r * kStats + i];
     const auto *stq = reinterpret_cast<const int32_t *>(res + resStatus);
@@ -620,19 +811,42 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
     wk.tau = reinterpret_cast<uint32_t *>(base + offTau);
     wk.hist = reinterpret_cast<uint32_t *>(base + offHist);
     wk.stats = reinterpret_cast<unsigned long long *>(base + offRows);
-    wk.liveMask = wk.dead = wk.cursor = nullptr; // threshold planning only
+    wk.itemMask = nullptr; // threshold planning only
+    wk.dead = wk.cursor = nullptr;
+    wk.plan = nullptr;
     try {
         slot.timed = false;
-        BM25MI_HIP(hipMemcpyAsync(base, slot.hPacket, packetBytes, hipMemcpyHostToDevice, slot.stream),
-                   "packet H2D (scratch)");
+        LaunchStage(static_cast<const uint4 *>(slot.hPacketDev), reinterpret_cast<uint4 *>(base), slot.ranges,
+                    slot.btHost, nullptr, Model(), 0, slot.stream);
         // The scratch keeps the last call's contents, possibly another session's: the counters
         // are set by a fill kernel before scoring.
         LaunchInit(wk, cfg.max_queries, slot.stream);
         Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream), "Unsupported block size");
-        LaunchFinal(bt, wk, cfg.max_queries, static_cast<unsigned long long *>(slot.hResultDev), grid, slot.stream);
+        slot.finalGrid = FinalGrid(slot.nq);
+        LaunchFinal(bt, wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), grid, slot.stream);
         BM25MI_HIP(hipGetLastError(), "launch Search kernels (scratch)");
         Complete(slot, outScores, outIds, status);
         last = slot.last;
+    } catch (...) {
+        failed = true;
+        throw;
+    }
+}
+
+void Session::PredictThresholds(const Query &query, float *out)
+{
+    Require(out != nullptr, "Null threshold output");
+    Require(dModel != nullptr, "Learned thresholds need a threshold model (bm25mi_set_threshold_model)");
+    Slot &slot = slots[0];
+    Pack(query, slot);
+    try {
+        LaunchStage(static_cast<const uint4 *>(slot.hPacketDev), static_cast<uint4 *>(slot.dPacket), slot.ranges,
+                    slot.btHost, reinterpret_cast<float *>(static_cast<unsigned char *>(slot.dPacket) + offThr),
+                    Model(), query.count, slot.stream);
+        BM25MI_HIP(hipMemcpyAsync(out, slot.bt.thr, static_cast<size_t>(query.count) * sizeof(float),
+                                  hipMemcpyDeviceToHost, slot.stream),
+                   "thresholds D2H");
+        BM25MI_HIP(hipStreamSynchronize(slot.stream), "predict thresholds");
     } catch (...) {
         failed = true;
         throw;
@@ -648,7 +862,7 @@ void Session::SetLaunch(int32_t waves, int64_t cost)
     grid = waves;
     taskCost = cost;
     for (auto &slot : slots) {
-        for (int g = 0; g < 3; ++g) {
+        for (int g = 0; g < 4; ++g) {
             if (slot.exec[g])
                 BM25MI_HIP(hipGraphExecDestroy(slot.exec[g]), "destroy graph exec");
             if (slot.graph[g])
@@ -687,7 +901,7 @@ void Session::Bootstrap()
 void Session::Release() noexcept
 {
     for (auto &slot : slots) {
-        for (int g = 0; g < 3; ++g) {
+        for (int g = 0; g < 4; ++g) {
             if (slot.exec[g])
                 (void)hipGraphExecDestroy(slot.exec[g]);
             if (slot.graph[g])
@@ -705,18 +919,8 @@ void Session::Release() noexcept
             (void)hipStreamDestroy(slot.stream);
             slot.stream = nullptr;
         }
-        if (slot.side) {
-            (void)hipStreamSynchronize(slot.side);
-            (void)hipStreamDestroy(slot.side);
-            slot.side = nullptr;
-        }
-        for (auto &e : slot.fork)
-            if (e) {
-                (void)hipEventDestroy(e);
-                e = nullptr;
-            }
-        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dLive,
-                         &slot.dDead, &slot.dCursor})
+        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dItems,
+                         &slot.dDead, &slot.dCursor, &slot.dPlan})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
@@ -727,7 +931,7 @@ void Session::Release() noexcept
                 *p = nullptr;
             }
     }
-    for (void **p : {&dPost, &dSamples, &dDir, &dBmax})
+    for (void **p : {&dPost, &dSamples, &dDir, &dBmax, &dModel})
         if (*p) {
             (void)hipFree(*p);
             *p = nullptr;
@@ -754,6 +958,7 @@ extern "C" void bm25mi_config_default(bm25mi_config *cfg)
     cfg->task_cost = 0;
     cfg->slots = 1;
     cfg->pool_entries = 0;
+    cfg->live_max_k = 0;
 }
 
 extern "C" int bm25mi_create(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,
@@ -788,6 +993,8 @@ static int SearchCommon(bm25mi_session *s, const Query &query, float *scores, in
             if (s->impl->serving)
                 throw InvalidState("BM25 session is serving: submit through its server");
             Session &ss = *s->impl;
+            if (query.learned && !ss.dModel)
+                throw InvalidState("Learned Search needs a threshold model (bm25mi_set_threshold_model)");
             std::vector<int32_t> status(std::max(0, query.count));
             ss.Search(query, scores, ids, status.data(), timed);
             int32_t reruns = 0;
@@ -863,6 +1070,89 @@ extern "C" int bm25mi_search_batch(bm25mi_session *s, const int64_t *query_offse
     return SearchCommon(s, q, out_scores, out_ids, nullptr, false, false);
 }
 
+extern "C" int bm25mi_search_learned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                                     int32_t queries, const int32_t *k_per_query, int32_t timed, float *out_scores,
+                                     int32_t *out_ids, int32_t *fallback_out)
+{
+    Query q{query_offsets, query_terms, queries, k_per_query, nullptr};
+    q.learned = true;
+    return SearchCommon(s, q, out_scores, out_ids, fallback_out, true, timed != 0);
+}
+
+extern "C" int bm25mi_learned_thresholds(bm25mi_session *s, const int64_t *query_offsets,
+                                         const int32_t *query_terms, int32_t queries, const int32_t *k_per_query,
+                                         float *out_thresholds)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            if (!s->impl || s->impl->closed || s->impl->failed)
+                throw InvalidState("BM25 session closed/failed");
+            if (s->impl->serving)
+                throw InvalidState("BM25 session is serving: submit through its server");
+            Query q{query_offsets, query_terms, queries, k_per_query, nullptr};
+            q.learned = true;
+            s->impl->PredictThresholds(q, out_thresholds);
+        },
+        &s->error);
+}
+
+extern "C" int bm25mi_set_threshold_model(bm25mi_session *s, const float *model, int64_t count, float y_mean,
+                                          float y_scale, float safety)
+{
+    if (!s)
+        return Invoke([] { throw InvalidState("BM25 session is not initialized"); });
+    std::lock_guard<std::mutex> guard(s->mutex);
+    return Invoke(
+        [&] {
+            Require(s->impl && !s->impl->closed && !s->impl->failed, "BM25 session closed/failed");
+            Require(!s->impl->serving, "Stop the session's server before changing its model");
+            constexpr int64_t kParams = int64_t(kHidden) * kFeatures + kHidden + kHidden * kHidden + kHidden + kHidden + 1;
+            Require(model && count == kParams + 2 * kFeatures, "Model must hold the MLP parameters, mean and scale");
+            for (int64_t i = 0; i < count; ++i)
+                Require(std::isfinite(model[i]), "Model values must be finite");
+            for (int i = 0; i < kFeatures; ++i)
+                Require(model[kParams + kFeatures + i] > 0.0F, "Feature scales must be positive");
+            Require(std::isfinite(y_mean) && std::isfinite(y_scale) && y_scale > 0.0F && std::isfinite(safety) &&
+                        safety > 0.0F,
+                    "Invalid model output transform");
+            Session &ss = *s->impl;
+            try {
+                BM25MI_HIP(hipSetDevice(ss.device), "set device");
+                if (!ss.dModel)
+                    BM25MI_HIP(hipMalloc(&ss.dModel, static_cast<size_t>(count) * sizeof(float)), "alloc model");
+                BM25MI_HIP(hipMemcpy(ss.dModel, model, static_cast<size_t>(count) * sizeof(float),
+                                     hipMemcpyHostToDevice),
+                           "upload model");
+            } catch (const DeviceError &) {
+                ss.failed = true;
+                throw;
+            }
+            ss.modelYMean = y_mean;
+            ss.modelYScale = y_scale;
+            ss.modelSafety = safety;
+            if (ss.cfg.use_graph) {
+                try {
+                    for (auto &slot : ss.slots) {
+                        if (slot.exec[3])
+                            BM25MI_HIP(hipGraphExecDestroy(slot.exec[3]), "destroy learned graph exec");
+                        if (slot.graph[3])
+                            BM25MI_HIP(hipGraphDestroy(slot.graph[3]), "destroy learned graph");
+                        slot.exec[3] = nullptr;
+                        slot.graph[3] = nullptr;
+                        ss.Capture(slot);
+                    }
+                } catch (const DeviceError &) {
+                    ss.failed = true;
+                    throw;
+                }
+            }
+        },
+        &s->error);
+}
+
 extern "C" int bm25mi_search_batch_noalloc(bm25mi_session *s, const int64_t *query_offsets,
                                            const int32_t *query_terms, int32_t queries, const int32_t *k_per_query,
                                            float *out_scores, int32_t *out_ids)
@@ -931,8 +1221,8 @@ static double LowerBound(const Session &ss, const int32_t *terms, int64_t n, int
     return f;
 }
 
-// Per-query features of the learned K-th-score model. Every feature depends only on
-// index statistics of the query's terms (IDs >= vocab ignored, repeats counted).
+// Per-query features of the learned K-th-score model (Session::Features); they depend only on
+// index statistics of the query's terms.
 extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                                      int32_t queries, const int32_t *k_per_query, float *out_features)
 {
@@ -945,42 +1235,9 @@ extern "C" int bm25mi_query_features(bm25mi_session *s, const int64_t *query_off
             Require(out_features != nullptr, "Null feature output");
             CheckQueries(query_offsets, query_terms, queries, k_per_query);
             const Session &ss = *s->impl;
-            const double n = static_cast<double>(ss.ndocs);
-            for (int32_t q = 0; q < queries; ++q) {
-                std::vector<double> m;
-                double sumMean = 0, sumIdf = 0, maxIdf = 0, sumDf = 0, maxDf = 0, minDf = -1, density = 0,
-                       logAll = 0;
-                for (int64_t w = query_offsets[q]; w < query_offsets[q + 1]; ++w) {
-                    const int32_t t = query_terms[w];
-                    if (t >= ss.vocab)
-                        continue;
-                    const double df = static_cast<double>(ss.toffHost[t + 1] - ss.toffHost[t]);
-                    m.push_back(ss.tmaxHost[t]);
-                    sumMean += ss.tmeanHost[t];
-                    sumIdf += ss.idfHost[t];
-                    maxIdf = std::max(maxIdf, static_cast<double>(ss.idfHost[t]));
-                    sumDf += df;
-                    maxDf = std::max(maxDf, df);
-                    minDf = minDf < 0 ? df : std::min(minDf, df);
-                    density += ss.tmaxHost[t] * df / n;
-                    logAll += std::log(std::max(df, 0.5) / n);
-                }
-                std::sort(m.begin(), m.end(), std::greater<double>());
-                double ub = 0, rest = 0;
-                for (size_t i = 0; i < m.size(); ++i) {
-                    ub += m[i];
-                    if (i >= 3)
-                        rest += m[i];
-                }
-                float *f = out_features + static_cast<int64_t>(q) * BM25MI_FEATURES;
-                const double vals[BM25MI_FEATURES] = {
-                    std::log(static_cast<double>(k_per_query[q])), static_cast<double>(m.size()), ub,
-                    m.size() > 0 ? m[0] : 0.0, m.size() > 1 ? m[1] : 0.0, m.size() > 2 ? m[2] : 0.0, rest, sumMean,
-                    sumIdf, maxIdf, std::log1p(sumDf), std::log1p(maxDf), std::log1p(std::max(minDf, 0.0)), density,
-                    std::log(n), std::max(-40.0, logAll)};
-                for (int i = 0; i < BM25MI_FEATURES; ++i)
-                    f[i] = static_cast<float>(vals[i]);
-            }
+            for (int32_t q = 0; q < queries; ++q)
+                ss.Features(query_terms + query_offsets[q], query_offsets[q + 1] - query_offsets[q], k_per_query[q],
+                            out_features + static_cast<int64_t>(q) * BM25MI_FEATURES, nullptr);
         },
         &s->error);
 }

@@ -72,6 +72,8 @@ typedef struct {
                                 0 or 1 = one */
     int64_t pool_entries;    /* candidate-pool entries per slot; 0 = max_queries queries of K = max_k
                                 (a batch whose queries need more is rejected / split by the server) */
+    int32_t live_max_k;      /* largest K a threshold query is scored with on the live path (larger
+                                ones take the plain path); 0 = 64 */
 } bm25mi_config;
 
 /* Per-Search counters, filled by bm25mi_last_stats. */
@@ -154,6 +156,34 @@ int bm25mi_search_pruned(bm25mi_session *s, const int64_t *query_offsets, const 
                          int32_t queries, const int32_t *k_per_query, const float *thresholds,
                          float *out_scores, int32_t *out_ids, int32_t *fallback_out);
 
+/* The resident K-th-score model of learned Searches (no reference counterpart). model holds
+ * the MLP's parameters (bm25mi_mlp_param_count(BM25MI_FEATURES) floats in the layout of the
+ * bm25mi_mlp_* kernels) followed by the features' mean[BM25MI_FEATURES] and
+ * scale[BM25MI_FEATURES]; count must be their total. For a query with features f
+ * (bm25mi_query_features) and score upper bound u (bm25mi_query_bounds), the model's output z
+ * on (f - mean) / scale gives the threshold clamp(safety * u * exp(z * y_scale + y_mean), lower,
+ * u). The model is copied to device memory (allocated by the first call on a session). */
+int bm25mi_set_threshold_model(bm25mi_session *s, const float *model, int64_t count, float y_mean, float y_scale,
+                               float safety);
+
+/* The thresholds a learned Search of this batch would use, predicted on the device by the
+ * session's model (no reference counterpart): out_thresholds[q] = 0 for a query routed to the
+ * plain path. */
+int bm25mi_learned_thresholds(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                              int32_t queries, const int32_t *k_per_query, float *out_thresholds);
+
+/* Learned Search (no reference counterpart): bm25mi_search_pruned whose thresholds the
+ * session's resident model predicts on the device, inside the Search (one kernel before the
+ * scoring kernels). A query is routed to the live path when its threshold is at least 0.3 of its
+ * score upper bound and its K at most cfg.live_max_k, else to the plain path; a query that keeps
+ * fewer than K docs is rerun on the plain path (fallback_out[q] = 1), so results always equal
+ * bm25mi_search_batch. timed != 0: direct launches with HIP events, as bm25mi_search_timed
+ * (score_kernel_ms then spans threshold prediction through plain and live scoring). Requires
+ * bm25mi_set_threshold_model. */
+int bm25mi_search_learned(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
+                          int32_t queries, const int32_t *k_per_query, int32_t timed, float *out_scores,
+                          int32_t *out_ids, int32_t *fallback_out);
+
 /* Same as bm25mi_search_batch (thresholds == NULL) or bm25mi_search_pruned, but
  * launches the kernels directly instead of replaying the graph, with HIP events
  * around the scoring kernel (bm25mi_stats.score_kernel_ms). For measurement. */
@@ -161,8 +191,12 @@ int bm25mi_search_timed(bm25mi_session *s, const int64_t *query_offsets, const i
                         int32_t queries, const int32_t *k_per_query, const float *thresholds, float *out_scores,
                         int32_t *out_ids, int32_t *fallback_out);
 
-/* Query features for the learned threshold (no reference counterpart):
- * writes float[queries * BM25MI_FEATURES]. */
+/* Query features for the learned threshold (no reference counterpart): writes
+ * float[queries * BM25MI_FEATURES], per query: log K, the number of known terms, log of the score
+ * upper bound, log of the largest and second-largest K-th term impacts (the largest is the
+ * provable lower bound of the K-th score), the three largest term maxima, the sum of the K-th
+ * term impacts, of the term mean impacts and of the idf, log1p of the sum, maximum and minimum
+ * df, the density sum of term maximum * df / docs, and the sum of log(df / docs) (at least -40). */
 #define BM25MI_FEATURES 16
 int bm25mi_query_features(bm25mi_session *s, const int64_t *query_offsets, const int32_t *query_terms,
                           int32_t queries, const int32_t *k_per_query, float *out_features);
@@ -229,16 +263,21 @@ int bm25mi_server_submit(bm25mi_server *srv, const int32_t *terms, int32_t n, in
  * the same. A request with K above 64 is queued as a plain one. */
 int bm25mi_server_submit_pruned(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, float threshold,
                                 uint64_t *ticket);
+/* Same as a learned Search of one query: its threshold is predicted on the device by the
+ * session's model (bm25mi_set_threshold_model) in the batch it is admitted to; a request that
+ * keeps fewer than K docs is rerun as a plain one. Batches hold learned and plain requests, or
+ * requests with thresholds of their own and plain ones. */
+int bm25mi_server_submit_learned(bm25mi_server *srv, const int32_t *terms, int32_t n, int32_t k, uint64_t *ticket);
 /* Blocks until the query is done; writes its K scores and doc IDs (either may be NULL) and
  * the submit-to-completion latency in microseconds (may be NULL). */
 int bm25mi_server_wait(bm25mi_server *srv, uint64_t ticket, float *scores, int32_t *ids, double *latency_us);
 /* Open-loop load: query i (terms offsets[i]..offsets[i+1], K ks[i], threshold thresholds[i]
- * when thresholds is not NULL) is submitted at the i-th arrival of a seeded Poisson process
- * of rate qps; latency_us[i] gets its latency, scores / ids (optional) its rows at offset
- * sum_{j<i} ks[j], *elapsed_s the wall time of the run. */
+ * when thresholds is not NULL, learned when learned != 0) is submitted at the i-th arrival of a
+ * seeded Poisson process of rate qps; latency_us[i] gets its latency, scores / ids (optional) its
+ * rows at offset sum_{j<i} ks[j], *elapsed_s the wall time of the run. */
 int bm25mi_server_run_poisson(bm25mi_server *srv, const int64_t *offsets, const int32_t *terms, int32_t n,
-                              const int32_t *ks, const float *thresholds, double qps, uint64_t seed,
-                              double *latency_us, float *scores, int32_t *ids, double *elapsed_s);
+                              const int32_t *ks, const float *thresholds, int32_t learned, double qps,
+                              uint64_t seed, double *latency_us, float *scores, int32_t *ids, double *elapsed_s);
 /* Batches launched, queries served (reruns included), the largest number of batches in flight
  * and the pruned queries rerun as plain ones, so far. Any output may be NULL. */
 int bm25mi_server_counters(bm25mi_server *srv, int64_t *batches, int64_t *queries, int32_t *max_inflight,

@@ -72,6 +72,7 @@ def trained(corpus10m):
                                                lr=2e-3, seed=0)
     vq, vk = learned.make_queries(offsets, 1024, seed=200)
     model.calibrate(learned.collect(s, vq, vk, batch=64), fallback_rate=0.01)
+    model.upload(s)
     yield s, model
     s.close()
 
@@ -79,21 +80,40 @@ def trained(corpus10m):
 @pytest.mark.parametrize('nq', [8, 64])
 @pytest.mark.parametrize('k', [10, 100])
 def test_learned_thresholds_synthetic10m(corpus10m, trained, nq, k):
+    """Held-out batches of 2-12-term queries: the learned Search (thresholds predicted on the
+    GPU inside the Search), the threshold path with the same model's host-side predictions, and
+    the plain path all equal the oracle."""
     index, offsets = corpus10m
     s, model = trained
     for seed in (0, 1):
         queries = learned.make_queries(offsets, nq, seed=900 + 10 * nq + k + seed, k_choices=(k,))[0]
         want_s, want_i = orc.search(index, *bm25mi.pack(queries), k, shards=16, threads=16, chunk=16)
+        got_s, got_i, fb = s.search(queries, k, learned=True)
+        assert_same(got_s, got_i, want_s, want_i, f'learned (device) Q{nq} K{k} seed {seed}')
         lower, upper = s.bounds(queries, k)
         thr = model.thresholds(s.features(queries, k), lower, upper)
         got_s, got_i, fb = s.search(queries, k, thresholds=thr)
-        assert_same(got_s, got_i, want_s, want_i, f'learned Q{nq} K{k} seed {seed}')
+        assert_same(got_s, got_i, want_s, want_i, f'learned (host) Q{nq} K{k} seed {seed}')
         plain_s, plain_i, _ = s.search(queries, k)
         assert_same(plain_s, plain_i, want_s, want_i, f'plain Q{nq} K{k} seed {seed}')
 
 
+def test_learned_headline_batches_synthetic10m(corpus10m, trained):
+    """The headline shape (Q8 x 3 terms, K10) through the learned Search: results equal the
+    oracle and far fewer postings are streamed than on the plain path."""
+    index, offsets = corpus10m
+    s, _ = trained
+    for seed in (1, 2, 3):
+        queries = bm25mi.draw_queries(offsets, 8, 3, seed=seed)
+        want_s, want_i = orc.search(index, *bm25mi.pack(queries), 10, shards=16, threads=16, chunk=8)
+        got_s, got_i, _ = s.search(queries, 10, learned=True, timed=True)
+        assert_same(got_s, got_i, want_s, want_i, f'learned headline seed {seed}')
+        st = s.stats()
+        assert st['postings_scanned'] < 0.5 * st['postings_total']
+
+
 @pytest.mark.parametrize('qps', [10000.0, 40000.0])
-def test_serving_synthetic10m_every_request(corpus10m, qps, need_gpu):
+def test_serving_synthetic10m_every_request(corpus10m, trained, qps, need_gpu):
     index, offsets = corpus10m
     rng = np.random.default_rng(int(qps))
     n = 600
@@ -106,11 +126,13 @@ def test_serving_synthetic10m_every_request(corpus10m, qps, need_gpu):
     # Thresholds around the true K-th score: every fifth above it (those rerun on the plain
     # path), the rest below it.
     thr = (kth * np.where(np.arange(n) % 5 == 0, 1.02, 0.9)).astype(np.float32)
+    _, model = trained
     with bm25mi.Session(index, max_queries=256, max_words=256 * 32, max_k=1000, slots=4,
                         pool_entries=1 << 26) as s:
+        model.upload(s)
         with bm25mi.Server(s, max_batch=256) as srv:
-            for mode, t in (('plain', None), ('learned', thr)):
-                r = srv.run_poisson(queries, ks, qps=qps, seed=int(qps) + 7, thresholds=t)
+            for mode, t, lrn in (('plain', None, False), ('thresholds', thr, False), ('learned', None, True)):
+                r = srv.run_poisson(queries, ks, qps=qps, seed=int(qps) + 7, thresholds=t, learned=lrn)
                 assert_same(r['scores'], r['ids'], want_s, want_i, f'serving {mode} at {qps} QPS')
             c = srv.counters()
-            assert c['queries'] >= 2 * n and c['batches'] < c['queries']
+            assert c['queries'] >= 3 * n and c['batches'] < c['queries']

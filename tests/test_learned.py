@@ -8,6 +8,7 @@ import pytest
 import bm25mi
 from bm25mi import learned
 from oracle import mlp_oracle as mo
+from oracle import oracle as orc
 
 
 def random_case(seed, n=300, n_in=16):
@@ -114,3 +115,85 @@ def test_learned_threshold_prunes_and_keeps_results(small_corpus, need_gpu):
         # The trained threshold streams fewer postings than the plain path and than the
         # provable lower bound used as the threshold without the network.
         assert scanned['learned'] < min(scanned['lower'], scanned['plain']), scanned
+
+
+@pytest.fixture(scope='module')
+def trained_small(small_corpus):
+    """A model trained on exact K-th scores of the 300K-doc corpus, calibrated on held-out
+    queries and made resident on a session."""
+    corpus, index = small_corpus
+    s = bm25mi.Session(index, max_queries=64, max_words=64 * 16, max_k=1000)
+    qs, ks = learned.make_queries(corpus.offsets, 2048, seed=100, k_choices=(10, 50, 100))
+    model = learned.ThresholdModel(seed=0).fit(learned.collect(s, qs, ks, batch=64), epochs=40, batch=256, lr=2e-3,
+                                               seed=0)
+    vq, vk = learned.make_queries(corpus.offsets, 512, seed=200, k_choices=(10, 50, 100))
+    model.calibrate(learned.collect(s, vq, vk, batch=64), fallback_rate=0.02)
+    model.upload(s)
+    yield corpus, index, s, model
+    s.close()
+
+
+@pytest.mark.gpu
+def test_device_thresholds_match_float64_oracle(trained_small, need_gpu):
+    """The threshold kernel's MLP (inside every learned Search) against the float64 oracle of the
+    same model: threshold = clamp(safety * upper * exp(z * y_scale + y_mean), lower, upper), and 0
+    for queries routed to the plain path (threshold below 0.3 of the upper bound, or K above 64)."""
+    corpus, index, s, model = trained_small
+    queries, ks = learned.make_queries(corpus.offsets, 64, seed=300, terms=(1, 12), k_choices=(1, 10, 64, 65, 100))
+    queries[0], queries[1] = [], [index.vocab + 5]
+    got = s.learned_thresholds(queries, ks)
+    feats = s.features(queries, ks)
+    lower, upper = s.bounds(queries, ks)
+    x = (feats.astype(np.float64) - model.mean) / model.scale
+    z = mo.forward(model.params, x)
+    want = np.clip(model.safety * upper * np.exp(z * model.y_scale + model.y_mean), lower, upper)
+    live = (upper > 0) & (ks <= 64) & (want >= 0.3 * upper)
+    margin = np.abs(want - 0.3 * upper) > 1e-3 * upper  # routing decisions far from the boundary
+    assert not got[:2].any()
+    assert (got[margin & ~live] == 0).all()
+    assert np.allclose(got[margin & live], want[margin & live], rtol=1e-4, atol=0)
+    assert (live & margin).sum() >= 8 and (~live & margin).sum() >= 8
+
+
+@pytest.mark.gpu
+def test_learned_search_equals_plain_and_prunes(trained_small, need_gpu):
+    """Learned Searches (thresholds predicted on the GPU inside the Search) return the plain
+    path's results bit for bit, on batches mixing live and plain routing, empty and unknown-term
+    queries and K from 1 to 1000, timed and untimed; they stream fewer postings than the plain
+    path, and a query rerun on the plain path is flagged."""
+    corpus, index, s, model = trained_small
+    scanned = {'plain': 0, 'learned': 0}
+    for seed in range(4):
+        queries, ks = learned.make_queries(corpus.offsets, 48, seed=400 + seed, terms=(1, 12),
+                                           k_choices=(1, 10, 50, 100, 1000))
+        queries[3] = []
+        queries[5] = [index.vocab + 1, queries[5][0]]
+        want_s, want_i = orc.search(index, *bm25mi.pack(queries), ks, threads=8)
+        for timed in (False, True):
+            got_s, got_i, fb = s.search(queries, ks, learned=True, timed=timed)
+            assert np.array_equal(got_i, want_i), f'seed {seed} timed={timed}'
+            assert np.array_equal(got_s.view(np.uint32), want_s.view(np.uint32)), f'seed {seed} timed={timed}'
+            assert set(np.unique(fb)) <= {0, 1}
+        scanned['learned'] += s.stats()['postings_scanned']
+        s.search(queries, ks, timed=True)
+        scanned['plain'] += s.stats()['postings_scanned']
+    assert scanned['learned'] < scanned['plain'], scanned
+    # Short queries at K10 (the headline shape): every query takes the live path.
+    queries = learned.make_queries(corpus.offsets, 32, seed=500, terms=(3, 3), k_choices=(10,))[0]
+    want_s, want_i = orc.search(index, *bm25mi.pack(queries), 10, threads=8)
+    got_s, got_i, fb = s.search(queries, 10, learned=True)
+    assert np.array_equal(got_i, want_i) and np.array_equal(got_s.view(np.uint32), want_s.view(np.uint32))
+    assert (s.learned_thresholds(queries, 10) > 0).mean() > 0.8
+
+
+@pytest.mark.gpu
+def test_learned_search_needs_a_model(small_corpus, need_gpu):
+    _, index = small_corpus
+    with bm25mi.Session(index, max_queries=2, max_words=8, max_k=10) as s:
+        with pytest.raises(bm25mi.Bm25Error) as err:
+            s.search([[1, 2]], 10, learned=True)
+        assert err.value.code == bm25mi._lib.ILLEGAL_OPERATION
+        with pytest.raises(bm25mi.Bm25Error) as err:
+            s.set_threshold_model(np.zeros(10, np.float32), np.zeros(16), np.ones(16), 0.0, 1.0, 1.0)
+        assert err.value.code == bm25mi._lib.INVALID_PARAM
+        s.search([[1, 2]], 10)  # the session stays usable

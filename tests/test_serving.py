@@ -127,3 +127,27 @@ def test_pruned_requests_fall_back_and_match_plain(corpus, need_gpu):
             _, upper = s.bounds(queries, ks)
             live = (thr >= np.float32(0.3) * upper) & (np.asarray(ks) <= 64)
             assert c['fallbacks'] >= int(((factor > 1) & (kth > 0) & live).sum()) > 0
+
+
+def test_learned_requests_match_plain(corpus, need_gpu):
+    """Learned requests (thresholds predicted on the GPU in each admitted batch) mixed with plain
+    ones under Poisson load: every result equals the same query searched alone."""
+    from bm25mi import learned
+    c, index = corpus
+    queries, ks = workload(corpus, 160, seed=41)
+    with bm25mi.Session(index, max_queries=32, max_words=32 * 32, max_k=1000, slots=3) as s:
+        tq, tk = learned.make_queries(c.offsets, 1024, seed=5, k_choices=(10, 100))
+        model = learned.ThresholdModel(seed=0).fit(learned.collect(s, tq, tk, batch=32), epochs=20, batch=128)
+        model.calibrate(learned.collect(s, tq[:256], tk[:256], batch=32), fallback_rate=0.05)
+        model.upload(s)
+        want_s, want_i = alone(s, queries, ks)
+        with bm25mi.Server(s, max_batch=32) as srv:
+            r = srv.run_poisson(queries, ks, qps=1e5, seed=3, learned=True)
+            assert np.array_equal(r['ids'], want_i)
+            assert np.array_equal(r['scores'].view(np.uint32), want_s.view(np.uint32))
+            tickets = [srv.submit(q, int(k), learned=bool(i % 2)) for i, (q, k) in enumerate(zip(queries[:40], ks[:40]))]
+            rows = np.concatenate([[0], np.cumsum(ks[:40])])
+            for i, t in enumerate(tickets):
+                sc, ids, _ = srv.wait(t, int(ks[i]))
+                assert np.array_equal(ids, want_i[rows[i]:rows[i + 1]])
+                assert np.array_equal(sc.view(np.uint32), want_s[rows[i]:rows[i + 1]].view(np.uint32))
