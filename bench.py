@@ -3,8 +3,11 @@
 Headline workload: the 10,000,000-doc synthetic corpus (vocabulary 1,000,000, Zipf 1.07,
 lognormal lengths of mean 56, k1 1.5, b 0.75, seed 0), Q=8 queries of 3 distinct df-weighted
 terms, K=10. One step is one full Search of one Q8 batch: host term IDs in, host top-K (doc IDs +
-FP32 scores) out, index resident in HBM. Steps alternate between two fixed batches A and B, like
-the reference's A/B protocol (FR/test/ascendhost/bm25/benchmark_common.h:51-107).
+FP32 scores) out, index resident in HBM. The timed steps cycle forward and backward over a
+rotation of 8 batches (seeds 1..8, about 880 MB of distinct postings, well above the 256 MB
+Infinity Cache); the reference's A/B protocol (batches A and B in AB, BA cycles,
+FR/test/ascendhost/bm25/benchmark_common.h:51-107) is timed beside it. Every timed call writes
+a poisoned output buffer whose bits are checked against the batch's warm-up result.
 
 Prints one JSON line: QPS (value), ms per Search, the scoring kernel's HBM roofline fraction,
 the reference's own CPU Search timed on this host's cores, parity against the CPU oracle, and
@@ -99,24 +102,84 @@ def ab_protocol(call, batches, warmup=5, cycles=30):
     return ms, out
 
 
+POISON = 0x7FC0DEAD  # a NaN as FP32 score bits, an invalid doc ID as int32
+
+
+def cycle_order(n, steps):
+    """Batch index per step: forward then backward cycles over n batches (A B, B A, ...)."""
+    order = []
+    c = 0
+    while len(order) < steps:
+        order.extend(range(n) if c % 2 == 0 else range(n - 1, -1, -1))
+        c += 1
+    return order[:steps]
+
+
+def headline_protocol(session, batches, args):
+    """args.warmup untimed Searches, then args.steps timed end-to-end Searches in forward/backward
+    cycles over `batches`. Each timed call writes its own output buffer, poisoned beforehand;
+    after the timed region every call's doc IDs and FP32 score bits are compared with its
+    batch's warm-up result."""
+    nq, k = args.queries, args.k
+    width = 2 * nq * k + nq
+    ref = {}
+    for step, b in enumerate(cycle_order(len(batches), max(args.warmup, len(batches)))):
+        s, i, _ = session.search(batches[b], k)
+        ref.setdefault(b, (s.copy(), i.copy()))
+    order = cycle_order(len(batches), args.steps)
+    bufs = np.full((args.steps, width), POISON, np.int32)
+    samples = []
+    t_start = time.perf_counter()
+    for step, b in enumerate(order):
+        t = time.perf_counter()
+        session.search(batches[b], k, out=bufs[step])
+        samples.append((time.perf_counter() - t) * 1e3)
+    total_s = time.perf_counter() - t_start
+    rows = nq * k
+    mismatches = 0
+    for step, b in enumerate(order):
+        got = (bufs[step, :rows].view(np.float32), bufs[step, rows:2 * rows])
+        mismatches += same(got, (ref[b][0].ravel(), ref[b][1].ravel()))
+    out = summarize(samples, nq)
+    out.update({'batches': len(batches), 'bit_checks': len(order), 'mismatches': mismatches, 'total_s': total_s,
+                'results': ref})
+    return out
+
+
+def posting_sets(offsets, batches):
+    """Posting bytes (8 B per posting of each distinct term) of the batches: their union, the mean
+    per batch, A and B's shared bytes and the mean shared between consecutive batches."""
+    df = np.diff(offsets)
+    sets = [set(np.unique(terms).tolist()) for _, terms in batches]
+    mb = lambda ts: 8 * int(df[sorted(ts)].sum()) / 1e6  # noqa: E731
+    union = set().union(*sets)
+    return {'union_mb': mb(union), 'mean_batch_mb': statistics.mean(mb(x) for x in sets),
+            'a_and_b_mb': mb(sets[0] & sets[1]),
+            'consecutive_shared_mb': statistics.mean(mb(sets[i] & sets[(i + 1) % len(sets)]) for i in range(len(sets))),
+            'infinity_cache_mb': 256}
+
+
 def same(a, b):
     """Mismatching entries between two (scores, ids) results: doc IDs or FP32 score bits."""
     return int((np.asarray(a[1]) != np.asarray(b[1])).sum()) + int(
         (np.asarray(a[0], np.float32).view(np.uint32) != np.asarray(b[0], np.float32).view(np.uint32)).sum())
 
 
-def cpu_legs(args, index, packed, gpu_results):
+def cpu_legs(args, index, packed, gpu_results, rotation):
     from oracle import oracle
     cores = host_cores()
     threads = min(32, cores)
     out = {'host': {'cores': cores, 'model': cpu_model()}}
-    # The oracle (restated reference CPU Search at this index's k1): parity and its timing.
-    port_ms, port_res = ab_protocol(
+    # The oracle (restated reference CPU Search at this index's k1): its timing on A/B, and parity
+    # of every batch of the rotation.
+    port_ms, _ = ab_protocol(
         lambda b: oracle.search(index, b[0], b[1], args.k, shards=threads, threads=threads, chunk=args.queries),
         packed, warmup=2, cycles=args.cpu_cycles)
     out['port'] = summarize(port_ms, args.queries)
-    mismatches = sum(same(gpu_results[i], port_res[i]) for i in (0, 1))
-    out['parity'] = {'mismatches_vs_oracle': mismatches, 'checked_batches': 2,
+    mismatches = sum(same(gpu_results[i], oracle.search(index, b[0], b[1], args.k, shards=threads, threads=threads,
+                                                        chunk=args.queries))
+                     for i, b in enumerate(rotation))
+    out['parity'] = {'mismatches_vs_oracle': mismatches, 'checked_batches': len(rotation),
                      'rule': 'doc IDs and FP32 score bits equal'}
     # The reference's own CPU Search (compiled from its sources into oracle/_ref): both variants.
     if oracle.ref_available() and args.k <= 32:
@@ -153,6 +216,118 @@ def cpu_legs(args, index, packed, gpu_results):
     cpu_baseline = {'value': base['qps'], 'unit': 'queries/s', 'cores': threads, 'kind': kind,
                     'ms_median': base['ms_median'], 'sample': sample}
     return cpu_baseline, out
+
+
+ROLES = ('gpu', 'cpu_reference', 'cpu_reference_single', 'bm25s_restated')
+
+
+def role_main(args):
+    """One role of the round protocol, in a process of its own: the A and B headline batches in
+    AB/BA cycles (args.warmup warm-up cycles, args.cycles timed), each case's median and nearest-rank
+    P95 of the per-call time, and the result bits of both cases, written as JSON to args.role_out."""
+    corpus = bm25mi.generate(args.corpus)
+    index = corpus.index()
+    packed = [bm25mi.pack(bm25mi.draw_queries(corpus.offsets, args.queries, args.terms, seed=s)) for s in (1, 2)]
+    del corpus
+    cores = host_cores()
+    closer = None
+    comparable = True  # same arithmetic as this library (k1 of the index)
+    if args.role == 'gpu':
+        sess = bm25mi.Session(index, max_queries=args.queries, max_words=args.queries * args.terms, max_k=args.k,
+                              block_docs=args.block_docs)
+        call, closer, threads = (lambda b: sess.search(b, args.k)[:2]), sess.close, 0
+    else:
+        from oracle import oracle
+        if args.role == 'bm25s_restated':
+            impacts = index.impacts()
+            threads = min(8, cores)
+            call = lambda b: oracle.bm25s_search(index, impacts, b[0], b[1], args.k, threads=threads)  # noqa: E731
+        else:
+            threads = 1 if args.role == 'cpu_reference_single' else min(32, cores)
+            if oracle.ref_available():
+                ref = oracle.RefSession(index, args.k, threads, variant='plain', max_queries=args.queries,
+                                        max_words=args.queries * args.terms)
+                call, closer = (lambda b: ref.search(b[0], b[1])), ref.close
+                comparable = False  # the reference's own constant k1 = 1.2
+            else:
+                call = lambda b: oracle.search(index, b[0], b[1], args.k, shards=threads, threads=threads,  # noqa: E731
+                                               chunk=args.queries)
+    ms, res = [], {}
+    per = {0: [], 1: []}
+    for c in range(args.warmup + args.cycles):
+        for i in ((0, 1) if c % 2 == 0 else (1, 0)):
+            t = time.perf_counter()
+            res[i] = call(packed[i])
+            if c >= args.warmup:
+                per[i].append((time.perf_counter() - t) * 1e3)
+    if closer:
+        closer()
+    summary = {case: {'median_batch_ms': statistics.median(per[i]), 'p95_batch_ms': nearest_rank(per[i], 0.95),
+                      'samples': len(per[i])} for i, case in enumerate('AB')}
+    bits = {case: [np.asarray(res[i][0], np.float32).view(np.uint32).ravel().tolist(),
+                   np.asarray(res[i][1]).ravel().tolist()] for i, case in enumerate('AB')}
+    Path(args.role_out).write_text(json.dumps({'role': args.role, 'threads': threads, 'summary': summary,
+                                               'results': bits if comparable else None}))
+
+
+def gate(rows, candidate, roles):
+    """The reference's decision rule (standalone/final_support.py:336-358): per case, the
+    candidate's median beats the best of the baseline roles' medians in every round, and the
+    median over rounds of its P95 is below theirs."""
+    cases = {}
+    for case in 'AB':
+        base = [min(row[r]['summary'][case]['median_batch_ms'] for r in roles) for row in rows]
+        cand = [row[candidate]['summary'][case]['median_batch_ms'] for row in rows]
+        bp95 = [min(row[r]['summary'][case]['p95_batch_ms'] for r in roles) for row in rows]
+        cp95 = [row[candidate]['summary'][case]['p95_batch_ms'] for row in rows]
+        wins = sum(a < b for a, b in zip(cand, base))
+        cases[case] = {'baseline_medians_ms': base, 'candidate_medians_ms': cand, 'baseline_p95_ms': bp95,
+                       'candidate_p95_ms': cp95, 'wins': wins,
+                       'baseline_over_candidate': statistics.median(base) / statistics.median(cand),
+                       'passed': wins == len(rows) and statistics.median(cp95) < statistics.median(bp95)}
+    return {'target_met': all(v['passed'] for v in cases.values()), 'baseline_roles': list(roles), 'cases': cases}
+
+
+def rounds_leg(args):
+    """The reference's round protocol (standalone/benchmark_final.py:196-285): args.rounds rounds,
+    each running every role in a process of its own, in order on even rounds and in reverse order
+    on odd ones; then the CPU gate over the rounds and the result bits of every role checked equal
+    to the GPU's."""
+    import subprocess
+    import tempfile
+    rows, mismatches = [], 0
+    with tempfile.TemporaryDirectory() as tmp:
+        for r in range(args.rounds):
+            order = list(ROLES) if r % 2 == 0 else list(reversed(ROLES))
+            row = {'round': r + 1, 'process_order': order}
+            for role in order:
+                out = Path(tmp) / f'{r}_{role}.json'
+                cmd = [sys.executable, str(ROOT / 'bench.py'), '--role', role, '--role-out', str(out),
+                       '--corpus', args.corpus, '--queries', str(args.queries), '--terms', str(args.terms),
+                       '--k', str(args.k), '--block-docs', str(args.block_docs), '--warmup', '5',
+                       '--cycles', str(args.round_cycles)]
+                t0 = time.time()
+                subprocess.run(cmd, check=True, timeout=900)
+                rec = json.loads(out.read_text())
+                row[role] = {'summary': rec['summary'], 'threads': rec['threads'], 'wall_s': round(time.time() - t0, 1),
+                             'results': rec.pop('results')}
+                log(f'round {r + 1} {role}: ' + json.dumps(rec['summary']))
+            rows.append(row)
+    compared = set()
+    for row in rows:
+        for role in ROLES:
+            if role != 'gpu' and row[role]['results'] is not None:
+                compared.add(role)
+                mismatches += sum(a != b for case in 'AB' for part in (0, 1)
+                                  for a, b in zip(row[role]['results'][case][part], row['gpu']['results'][case][part]))
+    for row in rows:
+        for role in ROLES:
+            row[role].pop('results')
+    return {'rounds': rows, 'cpu_decision': gate(rows, 'gpu', ('cpu_reference', 'cpu_reference_single')),
+            'bm25s_decision': gate(rows, 'gpu', ('bm25s_restated',)),
+            'results_vs_gpu_mismatches': mismatches, 'roles_compared_with_gpu': sorted(compared),
+            'note': 'the reference CPU Search (oracle/_ref) scores with its own k1 = 1.2, so its results are '
+                    'left out of the bit comparison; roles with the index\'s k1 are compared'}
 
 
 def learned_leg(args, index, offsets):
@@ -314,6 +489,7 @@ def main():
     ap.add_argument('--queries', type=int, default=8)
     ap.add_argument('--terms', type=int, default=3)
     ap.add_argument('--k', type=int, default=10)
+    ap.add_argument('--rotation', type=int, default=8, help='headline batches timed in rotation (seeds 1..N)')
     ap.add_argument('--block-docs', type=int, default=0, help='docs per scoring task (0 = library default)')
     ap.add_argument('--grid', type=int, default=0, help='persistent scoring waves (0 = fill the GPU)')
     ap.add_argument('--task-cost', type=int, default=0, help='per-task cost of the wave schedule (0 = default)')
@@ -334,7 +510,15 @@ def main():
     ap.add_argument('--serving-seconds', type=float, default=1.0)
     ap.add_argument('--pool-entries', type=int, default=1 << 26, help='candidate-pool entries per serving slot')
     ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: headline leg only')
+    ap.add_argument('--rounds', type=int, default=0,
+                    help='add the round protocol: this many rounds of the GPU and CPU roles, one process each')
+    ap.add_argument('--round-cycles', type=int, default=30, help='timed A/B cycles per role and round')
+    ap.add_argument('--role', choices=ROLES, help=argparse.SUPPRESS)
+    ap.add_argument('--role-out', help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.role:
+        role_main(args)
+        return
     if args.gpus != 1:
         print(json.dumps({'error': 'the library runs on one GPU; sharding a Search across GPUs is not implemented',
                           'n_gpus': args.gpus}))
@@ -345,9 +529,12 @@ def main():
     index = corpus.index()
     offsets = corpus.offsets
     log(f'corpus {args.corpus}: N={index.num_docs} V={index.vocab} M={index.postings} ({time.time() - t0:.1f}s)')
-    qa = bm25mi.draw_queries(offsets, args.queries, args.terms, seed=1)
-    qb = bm25mi.draw_queries(offsets, args.queries, args.terms, seed=2)
-    packed = [bm25mi.pack(qa), bm25mi.pack(qb)]
+    # Headline batches: A and B (seeds 1, 2) and the rotation of args.rotation batches (seeds 1..R,
+    # A and B among them) whose union of postings is well above the 256 MB Infinity Cache.
+    rotation = [bm25mi.pack(bm25mi.draw_queries(offsets, args.queries, args.terms, seed=s))
+                for s in range(1, max(2, args.rotation) + 1)]
+    packed = rotation[:2]
+    data = posting_sets(offsets, rotation)
     del corpus
 
     t0 = time.time()
@@ -355,33 +542,42 @@ def main():
                              block_docs=args.block_docs, grid_blocks=args.grid, task_cost=args.task_cost)
     log(f'session create {time.time() - t0:.1f}s')
 
-    results = {}
-    for step in range(args.warmup):
-        session.search(packed[step % 2], args.k)
-    # Timed region: K end-to-end Searches (host IDs -> host top-K), the session's default launches.
-    samples = []
-    t_start = time.perf_counter()
-    for step in range(args.steps):
-        t = time.perf_counter()
-        s, i, _ = session.search(packed[step % 2], args.k)
-        samples.append((time.perf_counter() - t) * 1e3)
-        results[step % 2] = (s, i)
-    total_s = time.perf_counter() - t_start
-    log(f'timed {args.steps} Searches: median {statistics.median(samples):.4f} ms')
+    # Reference protocol on A/B, then the rotation (whose figures are the headline's): warm-up,
+    # then K timed end-to-end Searches (host IDs -> host top-K) in forward/backward cycles, every
+    # call into its own poisoned output buffer, all bit-checked after the timed region.
+    ab = headline_protocol(session, packed, args)
+    log(f'A/B: {args.steps} Searches, median {ab["ms_median"]:.4f} ms, bit checks {ab["bit_checks"]}, '
+        f'mismatches {ab["mismatches"]}')
+    rot = headline_protocol(session, rotation, args)
+    log(f'rotation of {len(rotation)}: {args.steps} Searches, median {rot["ms_median"]:.4f} ms, '
+        f'bit checks {rot["bit_checks"]}, mismatches {rot["mismatches"]}')
+    ab.pop('results')
+    results = rot.pop('results')
+    total_s = rot.pop('total_s')
+    ab.pop('total_s')
 
-    # Scoring-kernel time with HIP events on the launch stream (direct launches).
-    kernel_ms, bytes_per, final_ms, device_ms = [], [], [], []
-    for step in range(max(20, min(args.steps, 100))):
-        offs, terms = packed[step % 2]
-        session.search(packed[step % 2], args.k, timed=True)
-        st = session.stats()
-        kernel_ms.append(st['score_kernel_ms'])
-        final_ms.append(st['final_ms'])
-        device_ms.append(st['device_ms'])
-        bytes_per.append(algorithmic_bytes(offs, terms, args.k, st))
+    # Scoring-kernel time with HIP events on the launch stream (direct launches), per batch of the
+    # rotation (and of A/B for comparison).
+    def kernel_times(batches, calls):
+        kernel_ms, bytes_per, final_ms, device_ms = [], [], [], []
+        for step in range(calls):
+            offs, terms = batches[step % len(batches)]
+            session.search(batches[step % len(batches)], args.k, timed=True)
+            st = session.stats()
+            kernel_ms.append(st['score_kernel_ms'])
+            final_ms.append(st['final_ms'])
+            device_ms.append(st['device_ms'])
+            bytes_per.append(algorithmic_bytes(offs, terms, args.k, st))
+        return kernel_ms, bytes_per, final_ms, device_ms
+
+    calls = max(4 * len(rotation), min(args.steps, 100))
+    kernel_ms, bytes_per, final_ms, device_ms = kernel_times(rotation, calls)
     k_ms = statistics.mean(kernel_ms)
     b_avg = statistics.mean(bytes_per)
     achieved = b_avg / (k_ms * 1e-3) / 1e9
+    ab_kernel, ab_bytes, _, _ = kernel_times(packed, max(20, min(args.steps, 100)))
+    ab_achieved = statistics.mean(ab_bytes) / (statistics.mean(ab_kernel) * 1e-3) / 1e9
+    session.search(packed[0], args.k, timed=True)
     st = session.stats()
     call_stats = []
     for step in range(20):
@@ -394,14 +590,14 @@ def main():
     traffic, traffic_note = measured_traffic(f'KScore<{st["block_docs"]}>', args)
     record = {
         'metric': 'Q8/K10 full-Search ms @10M docs; QPS; % HBM roofline; learned vs plain',
-        'value': args.queries * args.steps / total_s,
+        'value': args.queries * args.steps / total_s,  # the rotation's timed Searches
         'unit': 'queries/s',
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
         'ms_per_step': 1e3 * total_s / args.steps,
-        'ms_median': statistics.median(samples),
-        'ms_p95': nearest_rank(samples, 0.95),
+        'ms_median': rot['ms_median'],
+        'ms_p95': rot['ms_p95'],
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
@@ -410,11 +606,18 @@ def main():
         'config': {'workload': f'{args.corpus} full Search, Q{args.queries}x{args.terms} terms, K{args.k}',
                    'docs': index.num_docs, 'vocab': index.vocab, 'postings': index.postings,
                    'queries': args.queries, 'terms_per_query': args.terms, 'k': args.k,
-                   'block_docs': st['block_docs'], 'k1': index.k1, 'b': index.b},
+                   'block_docs': st['block_docs'], 'k1': index.k1, 'b': index.b,
+                   'batches': f'rotation of {len(rotation)} Q{args.queries} batches (seeds 1..{len(rotation)})'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'traffic_source': traffic_note,
                      'kernel': f'KScore<{st["block_docs"]}>', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg,
-                     'launch_waves': st['score_grid']},
+                     'launch_waves': st['score_grid'], 'batches': 'rotation',
+                     'ab_frac': ab_achieved / HBM_PEAK_GBS, 'ab_kernel_ms': statistics.mean(ab_kernel)},
+        'protocol': {'rotation': rot, 'ab': ab, 'postings': data,
+                     'rule': 'forward/backward cycles over the batches (AB, BA, ... for A/B); every timed '
+                             'call writes its own output buffer, pre-filled with a NaN/invalid-ID pattern, '
+                             'and its doc IDs and FP32 score bits are checked equal to that batch\'s '
+                             'warm-up result after the timed region'},
         'postings_per_search': st['postings_total'],
         'search_breakdown': {
             'device_ms': statistics.mean(device_ms),  # timed (direct) launches
@@ -433,7 +636,7 @@ def main():
 
     if not args.profile and not args.no_cpu:
         t0 = time.time()
-        record['cpu_baseline'], cpu = cpu_legs(args, index, packed, results)
+        record['cpu_baseline'], cpu = cpu_legs(args, index, packed, results, rotation)
         record['parity'] = cpu.pop('parity')
         record['cpu_baselines'] = cpu
         log(f'CPU legs {time.time() - t0:.1f}s')
@@ -446,6 +649,10 @@ def main():
         t0 = time.time()
         record['serving'] = serving_leg(args, index, offsets, model)
         log(f'serving leg {time.time() - t0:.1f}s')
+    if args.rounds and not args.profile:
+        t0 = time.time()
+        record['rounds'] = rounds_leg(args)
+        log(f'round protocol {time.time() - t0:.1f}s')
     if args.batched and not args.profile:
         del index
         t0 = time.time()

@@ -94,14 +94,16 @@ class Session:
             raise ValueError('one K per query required')
         return offsets, terms, nq, ks
 
-    def search(self, queries, k, thresholds=None, timed=False, learned=False):
+    def search(self, queries, k, thresholds=None, timed=False, learned=False, out=None):
         """Exact top-K per query. queries: list of term-ID lists or (offsets, terms).
 
         k: int or per-query array. thresholds: optional per-query predicted K-th
         scores (threshold path; results are still exact). learned: thresholds predicted on
         the GPU by the session's resident model (set_threshold_model) inside the Search.
-        Returns (scores, ids, fallback) with rows concatenated per query (shape (Q, K) when
-        k is an int).
+        out: optional int32 buffer of 2 * (result rows) + Q entries that receives the scores
+        (as FP32 bits), IDs and fallback flags, e.g. pre-filled with a poison pattern so that
+        a caller sees every entry written. Returns (scores, ids, fallback) with rows
+        concatenated per query (shape (Q, K) when k is an int), views of `out` when given.
         """
         offsets, terms = self._pack(queries)
         nq = len(offsets) - 1
@@ -122,7 +124,13 @@ class Session:
             pk = _lib.ptr(ks)
             rows = int(ks.sum())
         # Scores, IDs and fallback flags share one buffer: one address lookup per call.
-        out = np.zeros(2 * rows + nq, np.int32)
+        if out is None:
+            out = np.zeros(2 * rows + nq, np.int32)
+        elif not (isinstance(out, np.ndarray) and out.dtype == np.int32 and out.shape == (2 * rows + nq,)
+                  and out.flags.c_contiguous and out.flags.writeable):
+            raise ValueError(f'out must be a writeable contiguous int32 array of {2 * rows + nq} entries')
+        else:
+            out[2 * rows:] = 0  # fallback flags: the plain entry point does not write them
         base = _lib.ptr(out)
         scores, ids, fallback = out[:rows].view(np.float32), out[rows:2 * rows], out[2 * rows:]
         pScores, pIds, pFallback = base, base + 4 * rows, base + 8 * rows

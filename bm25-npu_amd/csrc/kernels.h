@@ -32,8 +32,15 @@ constexpr int32_t kLiveMaxK = 64;
 // 64-posting columns a scoring wave keeps in flight per round (14: 82.6 us against 84.3 us for 12
 // on the 10M-doc Q8 x 3-term K10 Search), and the live kernel's (its registers fit 12).
 constexpr int kColumns = BM25MI_COLUMNS;
-constexpr int kLiveColumns = 12;
+#ifndef BM25MI_LIVE_COLUMNS
+#define BM25MI_LIVE_COLUMNS 12
+#endif
+constexpr int kLiveColumns = BM25MI_LIVE_COLUMNS;
 constexpr int kStats = 16;
+// Item waves of the planning kernel of threshold queries: a multiple of 256 up to this many (the
+// live kernel reads their live counts, 64 per lane at most, in one round of 16-byte loads).
+constexpr int kMaxPruneWaves = 4096;
+constexpr int kPruneItems = 4; // items (64 granules of one query) a planning wave bounds together
 // Score histogram of kept candidates, per query: bucket = (score bits >> 19) - kHistBase,
 // i.e. 16 buckets per octave over [2^-4, 2^8) (lower scores fall in bucket 0, higher ones
 // in the last bucket, which only makes the bound it yields smaller).
@@ -119,8 +126,10 @@ constexpr int kFeatures = 16;  // model inputs per query
 constexpr int kHidden = 64;    // width of the model's two hidden layers
 
 // The resident K-th-score model of learned batches and the routing rules of threshold queries.
-// model: W1[64][16] b1[64] W2[64][64] b2[64] W3[64] b3[1], then the features' mean[16] and
-// scale[16]; nullptr when the session has none.
+// model: W1 b1[64] W2 b2[64] W3[64] b3[1], then the features' mean[16] and scale[16]; nullptr
+// when the session has none. The weight matrices W1[64][16] and W2[64][64] (hidden unit j's
+// inputs in row j) are regrouped on the device: float4 v * 64 + j holds row j's inputs
+// 4v .. 4v+3, so that a wave's lanes (one per hidden unit) load them coalesced.
 struct ModelView {
     const float *model;
     float yMean, yScale, safety; // threshold = safety * upper * exp(z * yScale + yMean), clamped
@@ -137,13 +146,11 @@ struct WorkView {
     unsigned long long *stats;// [grid][kStats]: one counter row per scoring wave
     // Written by the planning kernel for batches with a threshold query: per item (q * items per
     // query + c, the query's granules 64c .. 64c+63) the mask of granules whose upper bound
-    // reaches the query's threshold (0 for plain queries), and per wave the dead blocks it found.
+    // reaches the query's threshold (0 for plain queries), and per planning wave the dead blocks
+    // and the live granules it found (the live kernel's waves cut their shares from the latter).
     unsigned long long *itemMask; // [maxQ * ceil(granules / 64)]
-    uint32_t *dead;           // [2 * grid] (the planning kernel runs with twice the scoring grid)
-    // Live plan of the live kernel's waves, from the item masks (KPlan): wave w takes `count`
-    // live granules in item order, starting at the `skip`-th live granule of item `item`:
-    // {item, skip, count, 0}. The waves' shares differ by at most one granule.
-    uint4 *plan;              // [grid]
+    uint32_t *dead;           // [kMaxPruneWaves]
+    uint32_t *live;           // [kMaxPruneWaves]
     // Also per (query word w of the batch, block) for the short words of threshold queries: the
     // index of the word's first posting with doc at or after the block's first doc.
     uint32_t *cursor;         // [maxWords * blocks]

@@ -850,59 +850,80 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 // upper is the FP32 sum of its terms' largest impacts; the threshold is
 // clamp(safety * upper * exp(z * yScale + yMean), lower, upper), lower = tau0 (the K-th largest
 // impact of any of its terms). A threshold below minRatio * upper, or K above liveMaxK, routes
-// the query to the plain path (0). Lane j computes hidden unit j of both layers; the second
-// layer's weights are read in four rows of 16 to keep few registers live. The computation is the
-// same instruction sequence on the same inputs wherever it runs.
-__device__ __forceinline__ float PredictThreshold(const ModelView &mv, const BatchView &bt, int q)
-{
-    const int lane = static_cast<int>(__lane_id());
-    const float *W1 = mv.model, *B1 = W1 + kHidden * kFeatures, *W2 = B1 + kHidden, *B2 = W2 + kHidden * kHidden;
-    const float *W3 = B2 + kHidden, *B3 = W3 + kHidden, *mean = B3 + 1, *scale = mean + kFeatures;
-    const float ub = bt.upper[q];
-    const float lower = __uint_as_float(bt.tau0[q]);
-    const int32_t k = bt.k[q];
-    const float4 *w1 = reinterpret_cast<const float4 *>(W1 + lane * kFeatures);
-    float z1 = B1[lane];
+// the query to the plain path (0). Lane j computes hidden unit j of both layers. Load() issues
+// every load (the query's inputs and the lane's rows of both layers, one round trip) and
+// Compute() waits for them, so that a caller can overlap them with its own loads.
+struct ThresholdPredictor {
+    float ub, lower, b1, b2, w3, b3;
+    int32_t k;
+    float4 f[kFeatures / 4], a[kFeatures / 4], w[kHidden / 4];
+    float mu[kFeatures], sd[kFeatures]; // mean and scale follow b3 and are only 4-byte aligned
+
+    __device__ __forceinline__ void Load(const ModelView &mv, const BatchView &bt, int q)
+    {
+        const int lane = static_cast<int>(__lane_id());
+        const float *W1 = mv.model, *B1 = W1 + kHidden * kFeatures, *W2 = B1 + kHidden, *B2 = W2 + kHidden * kHidden;
+        const float *W3 = B2 + kHidden, *B3 = W3 + kHidden, *mean = B3 + 1, *scale = mean + kFeatures;
+        ub = bt.upper[q];
+        lower = __uint_as_float(bt.tau0[q]);
+        k = bt.k[q];
+        const float4 *f4 = reinterpret_cast<const float4 *>(bt.feat + static_cast<int64_t>(q) * kFeatures);
+        const float4 *w1 = reinterpret_cast<const float4 *>(W1) + lane; // device layout: see ModelView
+        const float4 *w2 = reinterpret_cast<const float4 *>(W2) + lane;
 #pragma unroll
-    for (int v = 0; v < kFeatures / 4; ++v) {
-        const float4 w = w1[v];
-        const float *f = bt.feat + static_cast<int64_t>(q) * kFeatures + 4 * v;
-        z1 = fmaf(w.x, (f[0] - mean[4 * v]) / scale[4 * v], z1);
-        z1 = fmaf(w.y, (f[1] - mean[4 * v + 1]) / scale[4 * v + 1], z1);
-        z1 = fmaf(w.z, (f[2] - mean[4 * v + 2]) / scale[4 * v + 2], z1);
-        z1 = fmaf(w.w, (f[3] - mean[4 * v + 3]) / scale[4 * v + 3], z1);
-    }
-    const float h1 = fmaxf(z1, 0.0F);
-    float z2 = B2[lane];
-    const float4 *w2 = reinterpret_cast<const float4 *>(W2 + lane * kHidden);
-#pragma unroll 1
-    for (int c = 0; c < kHidden / 16; ++c) {
-        float4 w[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-            w[v] = w2[c * 4 + v];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = c * 16 + 4 * v;
-            z2 = fmaf(w[v].x, __shfl(h1, i), z2);
-            z2 = fmaf(w[v].y, __shfl(h1, i + 1), z2);
-            z2 = fmaf(w[v].z, __shfl(h1, i + 2), z2);
-            z2 = fmaf(w[v].w, __shfl(h1, i + 3), z2);
+        for (int v = 0; v < kFeatures / 4; ++v) {
+            f[v] = f4[v];
+            a[v] = w1[v * kHidden];
         }
-    }
-    float y = W3[lane] * fmaxf(z2, 0.0F);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        y += __shfl_xor(y, o);
-    y += B3[0];
-    float thr = 0.0F;
-    if (ub > 0.0F && k <= mv.liveMaxK) {
-        thr = fminf(fmaxf(mv.safety * ub * expf(y * mv.yScale + mv.yMean), lower), ub);
-        if (!(thr >= mv.minRatio * ub))
-            thr = 0.0F;
+        for (int i = 0; i < kFeatures; ++i) {
+            mu[i] = mean[i];
+            sd[i] = scale[i];
+        }
+#pragma unroll
+        for (int v = 0; v < kHidden / 4; ++v)
+            w[v] = w2[v * kHidden];
+        b1 = B1[lane];
+        b2 = B2[lane];
+        w3 = W3[lane];
+        b3 = B3[0];
     }
-    return thr;
-}
+
+    __device__ __forceinline__ float Compute(const ModelView &mv) const
+    {
+        float z1 = b1;
+#pragma unroll
+        for (int v = 0; v < kFeatures / 4; ++v) {
+            z1 = fmaf(a[v].x, (f[v].x - mu[4 * v]) / sd[4 * v], z1);
+            z1 = fmaf(a[v].y, (f[v].y - mu[4 * v + 1]) / sd[4 * v + 1], z1);
+            z1 = fmaf(a[v].z, (f[v].z - mu[4 * v + 2]) / sd[4 * v + 2], z1);
+            z1 = fmaf(a[v].w, (f[v].w - mu[4 * v + 3]) / sd[4 * v + 3], z1);
+        }
+        // Hidden unit i of the first layer, broadcast from lane i (a scalar read per unit).
+        const uint32_t h1 = __float_as_uint(fmaxf(z1, 0.0F));
+        const auto H = [&](int i) { return __uint_as_float(ReadLane(h1, i)); };
+        float z2 = b2;
+#pragma unroll
+        for (int v = 0; v < kHidden / 4; ++v) {
+            z2 = fmaf(w[v].x, H(4 * v), z2);
+            z2 = fmaf(w[v].y, H(4 * v + 1), z2);
+            z2 = fmaf(w[v].z, H(4 * v + 2), z2);
+            z2 = fmaf(w[v].w, H(4 * v + 3), z2);
+        }
+        float y = w3 * fmaxf(z2, 0.0F);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            y += __shfl_xor(y, o);
+        y += b3;
+        float thr = 0.0F;
+        if (ub > 0.0F && k <= mv.liveMaxK) {
+            thr = fminf(fmaxf(mv.safety * ub * expf(y * mv.yScale + mv.yMean), lower), ub);
+            if (!(thr >= mv.minRatio * ub))
+                thr = 0.0F;
+        }
+        return thr;
+    }
+};
 
 __device__ __forceinline__ unsigned long long WaveOr64(unsigned long long v)
 {
@@ -914,41 +935,79 @@ __device__ __forceinline__ unsigned long long WaveOr64(unsigned long long v)
 
 constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
 
-// Planning kernel of threshold queries (the learned path; no reference counterpart). Every
-// wave takes a contiguous range of (query, 64-granule chunk) items in query-major order; lane
-// g sums, over the query's words, an upper bound of each word's impact in granule g: the
-// granule maximum of a long list, the term maximum of a short list when one of its postings
-// lies in the granule (found through the posting samples, then the postings themselves). FP32
-// addition of non-negative values is monotone, so the FP32 score of every doc in the granule,
-// summed in query order, is at most the same sum of these bounds; the sum here may be taken
-// in another order, which the factor (1 - 2^-16) on the threshold covers for up to 64 words.
-// A block with no granule reaching the threshold cannot hold a doc scoring at least it. The
-// kernel writes every item's mask of live granules (0 for the items of plain queries) and, per
-// wave, the number of dead blocks. It runs with twice the scoring grid
-// (it needs 46 VGPRs and no LDS, so all its waves are resident): at Q8 every wave then takes at
-// most one item instead of two in a row.
+// Planning kernel of threshold queries (the learned path; no reference counterpart). Wave b of
+// the first `waves` takes the items [items * b / waves, items * (b + 1) / waves) of the
+// query-major item order (item q * ceil(granules / 64) + c: granules 64c .. 64c + 63 of query q),
+// kPruneItems of one query at a time; lane g sums, over the query's words, an upper bound of
+// each word's impact in granule g of each item: the granule maximum of a long list (all of the
+// items' loads of up to eight long words in flight together), the term maximum of a short list
+// when one of its postings lies in the granule (found through the posting samples, then the
+// postings themselves). FP32 addition of non-negative values is monotone, so the FP32 score of
+// every doc in the granule, summed in query order, is at most the same sum of these bounds; the
+// sum here may be taken in another order, which the factor (1 - 2^-16) on the threshold covers
+// for up to 64 words. A block with no granule reaching the threshold cannot hold a doc scoring at
+// least it. The kernel writes every item's mask of live granules (0 for the items of plain
+// queries) and, per wave, the live granules and dead blocks it found; the live kernel cuts its
+// waves' shares from the live counts. For learned batches (header[3] = 1), whose routing the
+// staging kernel decided, the extra wave `waves` writes the plain queries' cost prefix and the
+// header counts. nqKnown: the batch's query count when the launch knows it (-1: read it from
+// the packet, as a captured graph does), which saves a dependent load.
 template <int B>
-__global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, WorkView wk, int32_t blocks)
+__global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t waves,
+                                                int32_t nqKnown)
 {
     constexpr int kGpb = B / kGranule;
     constexpr int kBpc = kPruneSpan / kGpb; // blocks per item
+    constexpr int C = kPruneItems;
     constexpr int kSampleRounds = (kLongMin >> kSampleShift) / kWave;
     static_assert(kPruneSpan % kGpb == 0 && kGpb <= kPruneSpan, "block size");
+    static_assert(C * kBpc <= kWave, "one lane per block of a chunk");
     const uint32_t lane = __lane_id();
-    const int32_t nq = bt.header[0];
+    const int32_t nq = nqKnown >= 0 ? nqKnown : bt.header[0];
+    if (static_cast<int32_t>(blockIdx.x) == waves) {
+        if (bt.header[3] == 0)
+            return;
+        // Learned batch: the wave plan of the plain kernel over the queries routed to the plain path.
+        auto *cost = const_cast<int64_t *>(bt.costPrefix);
+        auto *header = const_cast<int32_t *>(bt.header);
+        int64_t acc = 0;
+        int32_t live = 0;
+        for (int base = 0; base < nq; base += kWave) {
+            const int q = base + static_cast<int>(lane);
+            const bool in = q < nq;
+            const bool isLive = in && bt.thr[q] > 0.0F;
+            int64_t c = in && !isLive ? bt.planCost[q] : 0;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int64_t y = __shfl_up(c, o);
+                if (lane >= static_cast<uint32_t>(o))
+                    c += y;
+            }
+            if (in)
+                cost[q + 1] = acc + c;
+            acc += __shfl(c, kWave - 1);
+            live += __popcll(__ballot(isLive));
+        }
+        if (lane == 0) {
+            cost[0] = 0;
+            header[1] = live;
+            header[2] = nq - live;
+        }
+        return;
+    }
     const int64_t granules = ix.granules;
     const uint32_t cpq = static_cast<uint32_t>((granules + kPruneSpan - 1) / kPruneSpan);
     const uint32_t items = static_cast<uint32_t>(nq) * cpq;
-    const uint32_t per = (items + gridDim.x - 1) / gridDim.x;
-    const uint32_t i0 = min(items, blockIdx.x * per), i1 = min(items, i0 + per);
+    const uint32_t i0 = static_cast<uint32_t>(static_cast<uint64_t>(items) * blockIdx.x / waves);
+    const uint32_t i1 = static_cast<uint32_t>(static_cast<uint64_t>(items) * (blockIdx.x + 1) / waves);
     int qCur = -1, n = 0;
     int64_t wq0 = 0; // the query's first word in the batch
     float thrLive = 0.0F;
     uint4 info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-    uint32_t dead = 0;
-    for (uint32_t it = i0; it < i1; ++it) {
+    uint32_t dead = 0, live = 0;
+    for (uint32_t it = i0; it < i1;) {
         const int q = static_cast<int>(it / cpq);
-        const int64_t gBase = static_cast<int64_t>(it - static_cast<uint32_t>(q) * cpq) * kPruneSpan;
+        const uint32_t qEnd = min(i1, static_cast<uint32_t>(q + 1) * cpq); // this range's items of query q
         if (q != qCur) {
             qCur = q;
             // Independent loads, issued together.
@@ -964,53 +1023,64 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
                     info = bt.words[w0 + lane];
             }
         }
-        const int64_t blk = gBase / kGpb + lane; // lane b < kBpc: block gBase / kGpb + b of this chunk
-        const bool inRange = static_cast<int>(lane) < kBpc && blk < blocks;
         if (!(thrLive > 0.0F)) {
-            if (lane == 0)
-                wk.itemMask[it] = 0ULL; // a plain query
+            for (uint32_t j = it + lane; j < qEnd; j += kWave)
+                wk.itemMask[j] = 0ULL; // a plain query
+            it = qEnd;
             continue;
         }
-        const int64_t g = gBase + lane;
-        const bool valid = g < granules;
-        float ub = 0.0F;
-        // Long words: eight granule-maximum rows in flight at a time; the first eight stay in
-        // flight through the short words' searches and are added after them.
-        unsigned long long longs = __ballot(static_cast<int>(lane) < n && info.y > 0 && static_cast<int32_t>(info.z) >= 0);
-        float vFirst[8];
+        const uint32_t nIt = min(static_cast<uint32_t>(C), qEnd - it);
+        const int64_t gBase = static_cast<int64_t>(it - static_cast<uint32_t>(q) * cpq) * kPruneSpan;
+        bool valid[C];
+        float ub[C];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            vFirst[u] = 0.0F;
+        for (int u = 0; u < C; ++u) {
+            valid[u] = static_cast<uint32_t>(u) < nIt && gBase + u * kPruneSpan + lane < granules;
+            ub[u] = 0.0F;
+        }
+        // Long words: the items' granule-maximum rows of eight words in flight at a time; the
+        // first eight stay in flight through the short words' searches and are added after them.
+        unsigned long long longs = __ballot(static_cast<int>(lane) < n && info.y > 0 && static_cast<int32_t>(info.z) >= 0);
+        float vFirst[8][C];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            int64_t row = -1;
             if (longs) {
                 const int j = __ffsll(static_cast<long long>(longs)) - 1;
                 longs &= longs - 1;
-                const int64_t slotRow = static_cast<int64_t>(static_cast<int32_t>(ReadLane(info.z, j))) * granules;
-                if (valid)
-                    vFirst[u] = ix.bmax[slotRow + g];
+                row = static_cast<int64_t>(static_cast<int32_t>(ReadLane(info.z, j))) * granules + gBase + lane;
             }
+#pragma unroll
+            for (int u = 0; u < C; ++u)
+                vFirst[t][u] = row >= 0 && valid[u] ? ix.bmax[row + u * kPruneSpan] : 0.0F;
         }
         while (longs) {
-            float v[8];
+            float v[8][C];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                v[u] = 0.0F;
+            for (int t = 0; t < 8; ++t) {
+                int64_t row = -1;
                 if (longs) {
                     const int j = __ffsll(static_cast<long long>(longs)) - 1;
                     longs &= longs - 1;
-                    const int64_t slotRow = static_cast<int64_t>(static_cast<int32_t>(ReadLane(info.z, j))) * granules;
-                    if (valid)
-                        v[u] = ix.bmax[slotRow + g];
+                    row = static_cast<int64_t>(static_cast<int32_t>(ReadLane(info.z, j))) * granules + gBase + lane;
                 }
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    v[t][u] = row >= 0 && valid[u] ? ix.bmax[row + u * kPruneSpan] : 0.0F;
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                ub += v[u];
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    ub[u] += v[t][u];
         }
         // Short words: the postings between the samples that bracket the chunk's docs.
         unsigned long long shorts = __ballot(static_cast<int>(lane) < n && info.y > 0 && static_cast<int32_t>(info.z) < 0);
         if (shorts) {
             const int32_t dlo = static_cast<int32_t>(gBase << kGranuleShift);
-            const int32_t dhi = static_cast<int32_t>(min((gBase + kPruneSpan) << kGranuleShift, static_cast<int64_t>(ix.ndocs)));
+            const int32_t dhi = static_cast<int32_t>(
+                min((gBase + static_cast<int64_t>(nIt) * kPruneSpan) << kGranuleShift, static_cast<int64_t>(ix.ndocs)));
+            const uint32_t chunkBlocks = nIt * kBpc;
             while (shorts) {
                 const int j = __ffsll(static_cast<long long>(shorts)) - 1;
                 shorts &= shorts - 1;
@@ -1031,48 +1101,65 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
                 cb = WaveSum(cb);
                 const uint32_t ws = ca > 0 ? ((s0 + ca - 1) << kSampleShift) + 1 : first;
                 const uint32_t we = cb < s1 - s0 ? (s0 + cb) << kSampleShift : first + cnt;
-                unsigned long long m = 0;
-                uint32_t below = 0; // lane b < kBpc: postings of [ws, we) before block b of the chunk
+                unsigned long long m[C];
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    m[u] = 0;
+                uint32_t below = 0; // lane b < chunkBlocks: postings of [ws, we) before block b of the chunk
                 for (uint32_t p0 = ws; p0 < we; p0 += kWave) {
                     const bool v = p0 + lane < we;
                     const uint32_t d = v ? ix.post[p0 + lane].x : 0xFFFFFFFFu;
                     const uint32_t rel = (d >> kGranuleShift) - static_cast<uint32_t>(gBase);
-                    if (v && rel < static_cast<uint32_t>(kPruneSpan))
-                        m |= 1ULL << rel;
 #pragma unroll
-                    for (int b = 0; b < kBpc; ++b) {
+                    for (int u = 0; u < C; ++u)
+                        if (v && rel - static_cast<uint32_t>(u * kPruneSpan) < static_cast<uint32_t>(kPruneSpan))
+                            m[u] |= 1ULL << (rel - static_cast<uint32_t>(u * kPruneSpan));
+#pragma unroll
+                    for (int b = 0; b < C * kBpc; ++b) {
                         const uint32_t c = static_cast<uint32_t>(
                             __popcll(__ballot(d < static_cast<uint32_t>(dlo) + static_cast<uint32_t>(b * B))));
                         below += static_cast<int>(lane) == b ? c : 0u;
                     }
                 }
-                m = WaveOr64(m);
-                if ((m >> lane) & 1ULL)
-                    ub += __uint_as_float(ReadLane(info.w, j));
+                const float mx = __uint_as_float(ReadLane(info.w, j));
+#pragma unroll
+                for (int u = 0; u < C; ++u)
+                    if ((WaveOr64(m[u]) >> lane) & 1ULL)
+                        ub[u] += mx;
                 // Postings before ws lie below the chunk's first doc and those from we on at or
                 // above its end, so ws + below is the word's first posting in block b: the live
                 // kernel starts its search of the block's granules there.
-                if (static_cast<int>(lane) < kBpc && gBase / kGpb + lane < blocks)
+                if (lane < chunkBlocks && gBase / kGpb + lane < blocks)
                     wk.cursor[(wq0 + j) * blocks + gBase / kGpb + lane] = ws + below;
             }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            ub += vFirst[u];
-        const unsigned long long liveG = __ballot(valid && ub >= thrLive);
-        unsigned long long bm = 0; // lane b < kBpc: the live granules of block b of the item
-        if (static_cast<int>(lane) < kBpc) {
-            if constexpr (kGpb == kPruneSpan)
-                bm = liveG;
-            else
-                bm = (liveG >> (lane * kGpb)) & ((1ULL << kGpb) - 1ULL);
+        for (int u = 0; u < C; ++u) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                ub[u] += vFirst[t][u];
+            if (static_cast<uint32_t>(u) < nIt) {
+                const unsigned long long liveG = __ballot(valid[u] && ub[u] >= thrLive);
+                unsigned long long bm = 0; // lane b < kBpc: the live granules of block b of the item
+                if (static_cast<int>(lane) < kBpc) {
+                    if constexpr (kGpb == kPruneSpan)
+                        bm = liveG;
+                    else
+                        bm = (liveG >> (lane * kGpb)) & ((1ULL << kGpb) - 1ULL);
+                }
+                const bool inRange = static_cast<int>(lane) < kBpc && gBase / kGpb + u * kBpc + lane < blocks;
+                if (lane == 0)
+                    wk.itemMask[it + u] = liveG;
+                live += static_cast<uint32_t>(__popcll(liveG));
+                dead += static_cast<uint32_t>(__popcll(__ballot(inRange && bm == 0)));
+            }
         }
-        if (lane == 0)
-            wk.itemMask[it] = liveG;
-        dead += static_cast<uint32_t>(__popcll(__ballot(inRange && bm == 0)));
+        it += nIt;
     }
-    if (lane == 0)
+    if (lane == 0) {
         wk.dead[blockIdx.x] = dead;
+        wk.live[blockIdx.x] = live;
+    }
 }
 
 #ifndef BM25MI_LIVE_SLOTS
@@ -1081,145 +1168,6 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
 // Granules a live batch scores together (16 x 128 FP32 = 8 KB of LDS per wave).
 constexpr int kLiveSlots = BM25MI_LIVE_SLOTS;
 static_assert(kLiveSlots >= 2 && kLiveSlots <= 32 && (kLiveSlots & (kLiveSlots - 1)) == 0, "live batch size");
-constexpr int kPlanThreads = 1024;
-#ifndef BM25MI_PLAN_GROUPS
-#define BM25MI_PLAN_GROUPS 16
-#endif
-#ifndef BM25MI_PLAN_LOADS
-#define BM25MI_PLAN_LOADS 8
-#endif
-constexpr int kPlanGroups = BM25MI_PLAN_GROUPS; // workgroups of the live plan, each assigning a portion
-constexpr int kPlanLoads = BM25MI_PLAN_LOADS;   // item masks in flight per thread
-
-// ceil(a / b) and floor(a / b) for a < 2^53 and 1 <= b < 2^32, from the reciprocal r = 1 / b (a
-// double): the estimate is within one of the true value and an integer check corrects it.
-__device__ __forceinline__ uint64_t CeilDiv(uint64_t a, uint64_t b, double r)
-{
-    uint64_t q = static_cast<uint64_t>(ceil(static_cast<double>(a) * r));
-    if (q > 0 && (q - 1) * b >= a)
-        --q;
-    else if (q * b < a)
-        ++q;
-    return q;
-}
-__device__ __forceinline__ uint64_t FloorDiv(uint64_t a, uint64_t b, double r)
-{
-    uint64_t q = static_cast<uint64_t>(floor(static_cast<double>(a) * r));
-    if (q * b > a)
-        --q;
-    else if ((q + 1) * b <= a)
-        ++q;
-    return q;
-}
-
-// Live plan (the learned path; no reference counterpart). kPlanGroups workgroups turn the
-// planning kernel's item masks into equal shares of the live granules for the `waves` waves of
-// the live kernel. A live granule's rank is its position in item order (items query-major,
-// granules ascending); wave w takes ranks [T * w / waves, T * (w + 1) / waves) of the T live
-// granules, so the shares differ by at most one granule. Every workgroup counts all items (kPlanLoads
-// masks in flight per thread) for T and for the live granules before its portion of the items,
-// then scans its portion (a contiguous run per thread) for every item's rank prefix P_i, and each
-// item writes the plan of every wave whose first rank falls in it: the item, the live granules of
-// it before that rank, and the wave's share. The intervals [ceil(P_i * waves / T),
-// ceil(P_{i+1} * waves / T)) of the items partition [0, waves), so every wave gets exactly one
-// plan. For learned batches (header[3] = 1), whose routing the threshold kernel decided, wave 0
-// of workgroup 0 then writes the plain queries' cost prefix and the header counts.
-__global__ __launch_bounds__(kPlanThreads) void KPlan(BatchView bt, WorkView wk, uint32_t itemsPerQuery, int32_t waves)
-{
-    __shared__ uint32_t red[kPlanThreads / 64];
-    const int32_t nq = bt.header[0];
-    const uint32_t items = static_cast<uint32_t>(nq) * itemsPerQuery;
-    const uint32_t p0 = static_cast<uint32_t>(static_cast<uint64_t>(items) * blockIdx.x / gridDim.x);
-    const uint32_t p1 = static_cast<uint32_t>(static_cast<uint64_t>(items) * (blockIdx.x + 1) / gridDim.x);
-    uint32_t all = 0, beforeP = 0;
-    for (uint32_t b = threadIdx.x; b < items; b += kPlanThreads * kPlanLoads) {
-        uint32_t c[kPlanLoads];
-#pragma unroll
-        for (int k = 0; k < kPlanLoads; ++k) {
-            const uint32_t i = b + k * kPlanThreads;
-            c[k] = i < items ? static_cast<uint32_t>(__popcll(wk.itemMask[i])) : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < kPlanLoads; ++k) {
-            all += c[k];
-            beforeP += b + k * kPlanThreads < p0 ? c[k] : 0u;
-        }
-    }
-    const uint32_t total = BlockSum<kPlanThreads>(all, red);
-    const uint32_t before = BlockSum<kPlanThreads>(beforeP, red);
-    // This thread's run of the portion: items [r0, r1), up to per of them.
-    const uint32_t per = (p1 - p0 + kPlanThreads - 1) / kPlanThreads;
-    const uint32_t r0 = min(p0 + threadIdx.x * per, p1), r1 = min(r0 + per, p1);
-    uint32_t run = 0;
-    for (uint32_t j = r0; j < r1; j += 16) {
-        uint32_t c[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            c[k] = j + k < r1 ? static_cast<uint32_t>(__popcll(wk.itemMask[j + k])) : 0u;
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            run += c[k];
-    }
-    uint32_t portion = 0;
-    uint64_t P = static_cast<uint64_t>(before) + BlockExclusive<kPlanThreads>(run, red, &portion);
-    if (total == 0) {
-        if (blockIdx.x == 0)
-            for (int w = static_cast<int>(threadIdx.x); w < waves; w += kPlanThreads)
-                wk.plan[w] = make_uint4(0u, 0u, 0u, 0u);
-    } else {
-        const uint64_t G = static_cast<uint64_t>(waves), T = total;
-        const double rT = 1.0 / static_cast<double>(T), rG = 1.0 / static_cast<double>(G);
-        for (uint32_t j = r0; j < r1; j += 16) {
-            uint32_t cnt[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k)
-                cnt[k] = j + k < r1 ? static_cast<uint32_t>(__popcll(wk.itemMask[j + k])) : 0u;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                if (cnt[k]) {
-                    const uint64_t wLo = CeilDiv(P * G, T, rT);
-                    const uint64_t wHi = min(CeilDiv((P + cnt[k]) * G, T, rT), G);
-                    uint64_t s0 = FloorDiv(T * wLo, G, rG);
-                    for (uint64_t w = wLo; w < wHi; ++w) {
-                        const uint64_t s1 = FloorDiv(T * (w + 1), G, rG);
-                        wk.plan[w] = make_uint4(j + k, static_cast<uint32_t>(s0 - P), static_cast<uint32_t>(s1 - s0), 0u);
-                        s0 = s1;
-                    }
-                }
-                P += cnt[k];
-            }
-        }
-    }
-    if (blockIdx.x != 0 || threadIdx.x >= kWave || bt.header[3] == 0)
-        return;
-    // Learned batch: the wave plan of the plain kernel over the queries routed to the plain path.
-    const int lane = static_cast<int>(threadIdx.x);
-    auto *cost = const_cast<int64_t *>(bt.costPrefix);
-    auto *header = const_cast<int32_t *>(bt.header);
-    int64_t acc = 0;
-    int32_t live = 0;
-    for (int base = 0; base < nq; base += kWave) {
-        const int q = base + lane;
-        const bool in = q < nq;
-        const bool isLive = in && bt.thr[q] > 0.0F;
-        int64_t c = in && !isLive ? bt.planCost[q] : 0;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int64_t y = __shfl_up(c, o);
-            if (lane >= o)
-                c += y;
-        }
-        if (in)
-            cost[q + 1] = acc + c;
-        acc += __shfl(c, kWave - 1);
-        live += __popcll(__ballot(isLive));
-    }
-    if (lane == 0) {
-        cost[0] = 0;
-        header[1] = live;
-        header[2] = nq - live;
-    }
-}
 
 // Index (0..63) of the n-th set bit (n counted from 0) of m, which has more than n set bits.
 __device__ __forceinline__ uint32_t NthSetBit(unsigned long long m, uint32_t n)
@@ -1238,8 +1186,8 @@ __device__ __forceinline__ uint32_t NthSetBit(unsigned long long m, uint32_t n)
 }
 
 // Live kernel of threshold queries (the learned path; no reference counterpart). Every wave
-// scores its share of the live granules (KPlan: a contiguous run in item order, so the waves'
-// work differs by at most one granule), kLiveSlots at a time, from any query. It locates up to
+// scores its share of the live granules (a contiguous run in item order; the waves' shares differ
+// by at most one granule), kLiveSlots at a time, from any query. It locates up to
 // 64 of its granules at once from a window of 64 item masks (one load per lane), then takes them
 // into slots. A batch resolves all its (granule, word) pairs together: one gather of the query
 // words, one load of two directory entries per long word (or a window of the short list's
@@ -1249,9 +1197,10 @@ __device__ __forceinline__ uint32_t NthSetBit(unsigned long long m, uint32_t n)
 // query-word order as on the plain path. Docs scoring at least the query's threshold (and its
 // start bound) are appended to the query's pool; a pool that overflows makes the final kernel
 // rerun the query on the plain path. Batching turns the per-granule chain of dependent loads
-// into one chain per kLiveSlots granules. Counter rows go to row rowBase + wave.
+// into one chain per kLiveSlots granules. Counter rows go to row rowBase + wave. pruneWaves: the
+// planning kernel's item waves, a multiple of 256 up to kMaxPruneWaves.
 template <int B>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KLive(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t rowBase, int32_t pruneGrid)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KLive(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t rowBase, int32_t pruneWaves)
 {
     constexpr int kGpb = B / kGranule;
     constexpr int U = kLiveColumns;
@@ -1263,15 +1212,64 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t lane = __lane_id();
     const uint32_t itemsPerQuery = static_cast<uint32_t>((ix.granules + kWave - 1) / kWave);
     const uint32_t items = static_cast<uint32_t>(bt.header[0]) * itemsPerQuery;
-    // This wave's share of the live granules.
-    const uint4 share = wk.plan[blockIdx.x];
-    uint32_t winItem = share.x; // first item of the window of item masks (lane l: item winItem + l)
-    uint32_t skip = share.y;    // live granules of the window before the next one to take
-    uint32_t left = share.z;    // granules still to take
+    // This wave's share of the live granules: ranks [T * w / W, T * (w + 1) / W) of the T live
+    // granules in item order (W waves), located from the planning waves' live counts: lane l
+    // sums those of planning waves [l * per, l * per + per) (per = pruneWaves / 64), the lane
+    // holding rank T * w / W then finds its planning wave b among its counts, and the window of
+    // item masks starts at b's first item.
+    uint32_t winItem = 0; // first item of the window of item masks (lane l: item winItem + l)
+    uint32_t skip = 0;    // live granules of the window before the next one to take
+    uint32_t left = 0;    // granules still to take
+    {
+        const uint32_t per = static_cast<uint32_t>(pruneWaves) / kWave; // a multiple of 4, at most 64
+        const uint4 *own = reinterpret_cast<const uint4 *>(wk.live + lane * per);
+        uint4 c4[kMaxPruneWaves / (4 * kWave)];
+#pragma unroll
+        for (int j = 0; j < kMaxPruneWaves / (4 * kWave); ++j)
+            c4[j] = static_cast<uint32_t>(4 * j) < per ? own[j] : make_uint4(0u, 0u, 0u, 0u);
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kMaxPruneWaves / (4 * kWave); ++j)
+            sum += c4[j].x + c4[j].y + c4[j].z + c4[j].w;
+        uint32_t incl = sum;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o));
+            if (lane >= static_cast<uint32_t>(o))
+                incl += y;
+        }
+        const uint32_t total = ReadLane(incl, kWave - 1);
+        const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(total) * blockIdx.x / gridDim.x);
+        const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(total) * (blockIdx.x + 1) / gridDim.x);
+        left = r1 - r0;
+        if (left > 0) {
+            const int l = __popcll(__ballot(incl <= r0)); // the lane whose planning waves hold rank r0
+            // Walk lane l's counts (read from its registers) to the planning wave b holding r0.
+            uint32_t before = ReadLane(incl - sum, l), b = 0;
+            bool found = false;
+#pragma unroll
+            for (int j = 0; j < kMaxPruneWaves / kWave; ++j) {
+                const uint4 &v4 = c4[j / 4];
+                const uint32_t v = ReadLane(j % 4 == 0 ? v4.x : j % 4 == 1 ? v4.y : j % 4 == 2 ? v4.z : v4.w, l);
+                if (!found && static_cast<uint32_t>(j) < per) {
+                    if (before + v > r0) {
+                        found = true;
+                        b = static_cast<uint32_t>(l) * per + static_cast<uint32_t>(j);
+                    } else {
+                        before += v;
+                    }
+                }
+            }
+            skip = r0 - before;
+            winItem = static_cast<uint32_t>(static_cast<uint64_t>(items) * b / static_cast<uint32_t>(pruneWaves));
+        }
+    }
     // Located granules not yet in a slot: lane i < listN holds granule i of the list as
-    // item << 6 | bit (items < 2^26: the session bounds granules per Search by 2^32) and its
-    // query's word range as first word << 7 | words (first word <= 2^24, words <= 64).
-    uint32_t listIB = 0, listWN = 0, listN = 0, listAt = 0;
+    // item << 6 | bit (items < 2^26: the session bounds granules per Search by 2^32), its
+    // query's word range as first word << 7 | words (first word <= 2^24, words <= 64), K and
+    // floor (score bits: the threshold, at least the start bound and tau; tau only grows, so a
+    // stale read is still a lower bound).
+    uint32_t listIB = 0, listWN = 0, listK = 1, listFloor = 1, listN = 0, listAt = 0;
     unsigned long long scanned = 0, scored = 0, emitted = 0, emitting = 0, useful = 0;
 
     // Per segment of a batch in position-major order (lane r: segment r): first posting, count,
@@ -1296,9 +1294,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     do {                                                                                                               \
     } while (0)
 #endif
-    int ns = 0; // slots held (lane s < ns: slot s's query, granule and word range), some carried over
+    int ns = 0; // slots held (lane s < ns: slot s's query, granule, word range, K and floor), some carried over
     // Word offsets fit 32 bits (a batch holds at most 2^24 words).
-    uint32_t slotQ = 0, slotG = 0, slotW0 = 0, slotNw = 0;
+    uint32_t slotQ = 0, slotG = 0, slotW0 = 0, slotNw = 0, slotK = 1, slotFloor = 1;
     for (;;) {
         BM25MI_LPHASE(0);
         // 1. Up to kLiveSlots granules of the share.
@@ -1306,18 +1304,27 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (listAt == listN) {
                 if (left == 0)
                     break;
-                // Locate the next min(left, 64) granules: load the window of item masks, moving
-                // it on while it holds fewer than skip + 1 live granules; then lane i finds the
-                // granule of rank skip + i in it.
+                // Locate the next min(left, 64) granules: load the window of item masks (with
+                // each item's query state, in the same round trip), moving it on while it holds
+                // fewer than skip + 1 live granules; then lane i finds the granule of rank
+                // skip + i in it.
                 unsigned long long winMask = 0;
                 uint32_t winIncl = 0, winTotal = 0; // inclusive live-granule counts over the lanes
+                uint32_t winWN = 0, winK = 1, winFloor = 1;
                 for (;;) {
                     if (winItem >= items) {
                         left = 0; // past the last item (the plan counts only granules of the masks)
                         break;
                     }
                     const uint32_t it = winItem + lane;
-                    winMask = it < items ? wk.itemMask[it] : 0ULL;
+                    const bool inWin = it < items;
+                    const uint32_t qw = inWin ? it / itemsPerQuery : 0u;
+                    winMask = inWin ? wk.itemMask[it] : 0ULL;
+                    const uint32_t w0 = static_cast<uint32_t>(bt.qoff[qw]), w1 = static_cast<uint32_t>(bt.qoff[qw + 1]);
+                    winK = static_cast<uint32_t>(bt.k[qw]);
+                    const uint32_t tauW = __hip_atomic_load(wk.tau + qw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    winFloor = max(max(max(__float_as_uint(bt.thr[qw]), bt.tau0[qw]), tauW), 1u);
+                    winWN = w0 << 7 | (w1 - w0);
                     winIncl = static_cast<uint32_t>(__popcll(winMask));
 #pragma unroll
                     for (int o = 1; o < kWave; o <<= 1) {
@@ -1346,11 +1353,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 const uint32_t incl = static_cast<uint32_t>(__shfl(static_cast<int>(winIncl), static_cast<int>(idx)));
                 const uint32_t bit = NthSetBit(m, r - (incl - static_cast<uint32_t>(__popcll(m))));
                 const uint32_t item = winItem + idx;
-                const uint32_t q = item / itemsPerQuery;
                 listIB = item << 6 | bit;
-                // The query's word range, loaded with the granule so that slots gather at once.
-                const uint32_t w0 = static_cast<uint32_t>(bt.qoff[q]);
-                listWN = w0 << 7 | (static_cast<uint32_t>(bt.qoff[q + 1]) - w0);
+                listWN = static_cast<uint32_t>(__shfl(static_cast<int>(winWN), static_cast<int>(idx)));
+                listK = static_cast<uint32_t>(__shfl(static_cast<int>(winK), static_cast<int>(idx)));
+                listFloor = static_cast<uint32_t>(__shfl(static_cast<int>(winFloor), static_cast<int>(idx)));
                 // A live block counts once, at its first live granule.
                 const uint32_t b0 = bit & ~static_cast<uint32_t>(kGpb - 1);
                 scored += static_cast<unsigned long long>(
@@ -1362,12 +1368,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             }
             const int i = static_cast<int>(listAt);
             const uint32_t ib = ReadLane(listIB, i), wn = ReadLane(listWN, i);
+            const uint32_t kl = ReadLane(listK, i), fl = ReadLane(listFloor, i);
             const uint32_t qi = (ib >> 6) / itemsPerQuery;
             if (static_cast<int>(lane) == ns) {
                 slotQ = qi;
                 slotG = ((ib >> 6) - qi * itemsPerQuery) * kWave + (ib & 63u);
                 slotW0 = wn >> 7;
                 slotNw = wn & 127u;
+                slotK = kl;
+                slotFloor = fl;
             }
             ++ns;
             ++listAt;
@@ -1376,17 +1385,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             break;
 
         BM25MI_LPHASE(1);
-        // 2. The slots' query words and floors (the threshold, at least the start bound and tau).
+        // 2. The slots' query words (K and floor came with the granules).
         const bool isSlot = static_cast<int>(lane) < ns;
         const int64_t w0 = isSlot ? static_cast<int64_t>(slotW0) : 0;
         const uint32_t nw = isSlot ? slotNw : 0u;
-        uint32_t floorS = 1, kS = 1;
-        if (isSlot) {
-            kS = static_cast<uint32_t>(bt.k[slotQ]);
-            // tau only grows, so a stale (workgroup-scope) read is still a lower bound.
-            const uint32_t tauS = __hip_atomic_load(wk.tau + slotQ, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            floorS = max(max(max(__float_as_uint(bt.thr[slotQ]), bt.tau0[slotQ]), tauS), 1u);
-        }
+        const uint32_t floorS = isSlot ? slotFloor : 1u, kS = isSlot ? slotK : 1u;
         uint32_t pairEnd = nw; // inclusive prefix of the slots' word counts
 #pragma unroll
         for (int o = 1; o < kLiveSlots; o <<= 1) {
@@ -1741,6 +1744,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         slotG = static_cast<uint32_t>(__shfl(static_cast<int>(slotG), static_cast<int>(min(lane + m, 63u))));
         slotW0 = static_cast<uint32_t>(__shfl(static_cast<int>(slotW0), static_cast<int>(min(lane + m, 63u))));
         slotNw = static_cast<uint32_t>(__shfl(static_cast<int>(slotNw), static_cast<int>(min(lane + m, 63u))));
+        slotK = static_cast<uint32_t>(__shfl(static_cast<int>(slotK), static_cast<int>(min(lane + m, 63u))));
+        slotFloor = static_cast<uint32_t>(__shfl(static_cast<int>(slotFloor), static_cast<int>(min(lane + m, 63u))));
         ns -= m;
     }
     // This wave's counter row: rowBase + blockIdx.x (rowBase = gridDim.x when the plain kernel's
@@ -1753,10 +1758,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             row[i] = 0;
         row[kStatPostings] = scanned;
         row[kStatScored] = scored;
-        // The planning kernel ran with twice this grid: waves b and b + grid of it.
         // The planning kernel's dead-block counts (its grid differs from this one).
         unsigned long long deadBlocks = 0;
-        for (int32_t b = static_cast<int32_t>(blockIdx.x); b < pruneGrid; b += static_cast<int32_t>(gridDim.x))
+        for (int32_t b = static_cast<int32_t>(blockIdx.x); b < pruneWaves; b += static_cast<int32_t>(gridDim.x))
             deadBlocks += wk.dead[b];
         row[kStatSkipped] = deadBlocks;
         row[kStatCandidates] = emitted;
@@ -1783,13 +1787,22 @@ constexpr int kStageBatch = 8; // 16-byte loads in flight per thread
 // copy, in stream order before the kernels that read it. Unit u of the ranges' concatenation goes
 // to thread u % (threads of the grid); each thread issues kStageBatch loads before its stores, so
 // a packet of up to threads * kStageBatch units costs one host-memory round trip. For learned
-// batches (predict != 0; the host packet's thresholds are not staged) wave w of workgroup b then
-// predicts the threshold of query 4b + w (PredictThreshold, reading the query's inputs from the
-// host packet) and writes it into the device packet.
+// batches (predict = the queries to predict, at least the batch's; the host packet's thresholds
+// are not staged) wave w of workgroup b also predicts the threshold of query 4b + w
+// (ThresholdPredictor, reading the query's inputs from the host packet, its loads in flight
+// with the copy's) and writes it into the device packet when the query is in the batch.
 __global__ __launch_bounds__(kStageThreads) void KStage(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                         StageRanges r, BatchView host, float *thrOut, ModelView mv,
                                                         int32_t predict)
 {
+    const int q = static_cast<int>(blockIdx.x * (kStageThreads / kWave) + (threadIdx.x >> 6));
+    const bool predicting = q < predict;
+    ThresholdPredictor tp;
+    int32_t nq = 0;
+    if (predicting) {
+        nq = host.header[0];
+        tp.Load(mv, host, q);
+    }
     uint32_t total = 0;
     for (int k = 0; k < r.n; ++k)
         total += r.len[k];
@@ -1817,13 +1830,10 @@ __global__ __launch_bounds__(kStageThreads) void KStage(const uint4 *__restrict_
             if (at[j] != 0xFFFFFFFFu)
                 dst[at[j]] = v[j];
     }
-    if (!predict)
+    if (!predicting)
         return;
-    const int q = static_cast<int>(blockIdx.x * (kStageThreads / kWave) + (threadIdx.x >> 6));
-    if (q >= host.header[0])
-        return;
-    const float thr = PredictThreshold(mv, host, q);
-    if ((threadIdx.x & 63) == 0)
+    const float thr = tp.Compute(mv);
+    if ((threadIdx.x & 63) == 0 && q < nq)
         thrOut[q] = thr;
 }
 
@@ -1832,7 +1842,7 @@ void LaunchStage(const uint4 *src, uint4 *dst, const StageRanges &ranges, const 
 {
     const int grid = predictQueries > 0 ? (predictQueries + kStageThreads / kWave - 1) / (kStageThreads / kWave) : 1;
     hipLaunchKernelGGL(KStage, dim3(grid), dim3(kStageThreads), 0, stream, src, dst, ranges, host, thrOut, mv,
-                       predictQueries > 0 ? 1 : 0);
+                       predictQueries);
 }
 
 // Initial state of the per-query counters; run once at session creation. After that
@@ -1877,32 +1887,21 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     __shared__ uint32_t sRed[kFinalThreads / 64];
     __shared__ uint32_t sN;
     const int q = blockIdx.x;
-    const int32_t nq = bt.header[0];
     const int tid = threadIdx.x;
-    // The query's state, loaded before the counter rows so that all these loads share one
-    // memory round trip (wave 0 also loads the score histogram).
-    const bool active = q < nq;
-    uint32_t k = 1, countQ = 0, tauQ = 1, tau0Q = 1;
-    int64_t poolLo = 0, poolHi = 0;
-    float thrQ = 0.0F;
+    // The query's state, loaded with the query count and before the counter rows so that all
+    // these loads share one memory round trip (wave 0 also loads the score histogram). The
+    // grid holds at most the session's query capacity, so the loads stay within the packet and
+    // the per-query arrays also for workgroups beyond the batch's queries, which ignore them.
+    const int32_t nq = bt.header[0];
     uint32_t hregs[kHistPerLane];
+    uint32_t k = static_cast<uint32_t>(bt.k[q]);
+    const int64_t poolLo = bt.poolOff[q], poolHi = bt.poolOff[q + 1];
+    const uint32_t countQ = wk.count[q], tauQ = wk.tau[q], tau0Q = bt.tau0[q];
+    const float thrQ = bt.thr[q];
 #pragma unroll
     for (int i = 0; i < kHistPerLane; ++i)
-        hregs[i] = 0;
-    if (active) {
-        k = static_cast<uint32_t>(bt.k[q]);
-        poolLo = bt.poolOff[q];
-        poolHi = bt.poolOff[q + 1];
-        countQ = wk.count[q];
-        tauQ = wk.tau[q];
-        tau0Q = bt.tau0[q];
-        thrQ = bt.thr[q];
-        if (tid < kWave) {
-#pragma unroll
-            for (int i = 0; i < kHistPerLane; ++i)
-                hregs[i] = wk.hist[static_cast<int64_t>(q) * kHistBins + tid * kHistPerLane + i];
-        }
-    }
+        hregs[i] = tid < kWave ? wk.hist[static_cast<int64_t>(q) * kHistBins + tid * kHistPerLane + i] : 0u;
+    const bool active = q < nq;
     {
         // The scoring kernel has finished: this workgroup sums its share of the per-wave
         // counter rows (64 threads per counter; counters 12..15 take the maximum) into its
@@ -2075,17 +2074,18 @@ bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const 
 }
 
 bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream)
+                 int32_t waves, int32_t nqKnown, hipStream_t stream)
 {
+    // waves item waves and the learned batches' header wave.
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KPrune<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KPrune<2048>, dim3(waves + 1), dim3(kWave), 0, stream, ix, bt, wk, blocks, waves, nqKnown);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KPrune<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KPrune<4096>, dim3(waves + 1), dim3(kWave), 0, stream, ix, bt, wk, blocks, waves, nqKnown);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KPrune<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks);
+        hipLaunchKernelGGL(KPrune<8192>, dim3(waves + 1), dim3(kWave), 0, stream, ix, bt, wk, blocks, waves, nqKnown);
         return true;
     default:
         return false;
@@ -2093,27 +2093,28 @@ bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const 
 }
 
 bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                int32_t grid, int32_t rowBase, int32_t pruneGrid, hipStream_t stream)
+                int32_t grid, int32_t rowBase, int32_t pruneWaves, hipStream_t stream)
 {
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KLive<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneGrid);
+        hipLaunchKernelGGL(KLive<2048>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneWaves);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KLive<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneGrid);
+        hipLaunchKernelGGL(KLive<4096>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneWaves);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KLive<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneGrid);
+        hipLaunchKernelGGL(KLive<8192>, dim3(grid), dim3(kWave), 0, stream, ix, bt, wk, blocks, rowBase, pruneWaves);
         return true;
     default:
         return false;
     }
 }
 
-void LaunchPlan(const BatchView &bt, const WorkView &wk, int64_t granules, int32_t liveGrid, hipStream_t stream)
+int32_t PruneWaves(int64_t items)
 {
-    const uint32_t itemsPerQuery = static_cast<uint32_t>((granules + kWave - 1) / kWave);
-    hipLaunchKernelGGL(KPlan, dim3(kPlanGroups), dim3(kPlanThreads), 0, stream, bt, wk, itemsPerQuery, liveGrid);
+    // About kPruneItems items per wave, in whole 256-wave steps.
+    const int64_t w = (items + kPruneItems - 1) / kPruneItems;
+    return static_cast<int32_t>(std::min<int64_t>(kMaxPruneWaves, std::max<int64_t>(256, (w + 255) / 256 * 256)));
 }
 
 // Resident planning-kernel waves per CU, or 0.

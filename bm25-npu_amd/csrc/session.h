@@ -51,7 +51,7 @@ struct Query {
 // counters, result block, stream and captured graph.
 struct Slot {
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
-    void *dHist = nullptr, *dItems = nullptr, *dDead = nullptr, *dCursor = nullptr, *dPlan = nullptr;
+    void *dHist = nullptr, *dItems = nullptr, *dDead = nullptr, *dCursor = nullptr, *dLive = nullptr;
     void *hPacket = nullptr, *hPacketDev = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     StageRanges ranges{}; // the packet ranges the staging kernel copies for the packed batch
     BatchView btHost{};   // the packet's arrays in its mapped host memory (staging kernel)
@@ -59,7 +59,7 @@ struct Slot {
     WorkView wk{};
     hipStream_t stream = nullptr;
     // Captured Searches by batch mode: [0] plain queries only; [2] threshold queries only
-    // (planning, live plan, live scoring); [3] learned (captured once the session has a model).
+    // (planning, live scoring); [3] learned (captured once the session has a model).
     // Mode 1 (both kinds) is launched directly, [1] stays empty.
     hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
     hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -68,6 +68,10 @@ struct Slot {
     int32_t nq = 0;
     int mode = 0; // 0 plain, 1 mixed, 2 threshold queries only, 3 learned (thresholds predicted on the
                   // device); modes 0, 2 and 3 may replay a captured graph (exec[mode])
+    // A learned batch none of whose queries the device can route to the plain path: each one's
+    // start bound is at least kLiveMinRatio of its score upper bound (the threshold's clamp keeps
+    // it there) and its K at most liveMaxK. Its direct launch leaves out the plain kernel.
+    bool allLive = false;
     std::vector<int64_t> out; // [maxQ+1]
     int64_t total = 0;
     int32_t finalGrid = 0; // final-kernel workgroups of the launched Search (its counter rows)
@@ -88,7 +92,6 @@ public:
     int32_t blockDocs = 2048, granule = kGranule, blocks = 0, grid = 0, cus = 0;
     int32_t gridCapacity = 0; // the packet and counter rows hold this many scoring waves
     int32_t liveGrid = 0;     // live-kernel waves (as many as are resident at once)
-    int32_t pruneCapacity = 0; // planning-kernel waves resident at once (its largest grid)
     int64_t poolEntries = 0; // candidate-pool entries of each slot
     int64_t taskCost = kTaskCost;
     std::vector<int64_t> toffHost;
@@ -195,16 +198,19 @@ void LaunchStage(const uint4 *src, uint4 *dst, const StageRanges &ranges, const 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
                  int32_t addRows = 0);
-// Live granules of threshold queries, after LaunchPrune (with pruneGrid waves) and LaunchPlan
-// (for this grid); counter rows from rowBase on.
+// Live granules of threshold queries, after LaunchPrune with pruneWaves item waves; counter rows
+// from rowBase on.
 bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                int32_t grid, int32_t rowBase, int32_t pruneGrid, hipStream_t stream);
+                int32_t grid, int32_t rowBase, int32_t pruneWaves, hipStream_t stream);
 int LiveBlocksPerCu(int blockDocs);
 int PruneBlocksPerCu(int blockDocs);
-// The live kernel's per-wave shares of the live granules, after LaunchPrune.
-void LaunchPlan(const BatchView &bt, const WorkView &wk, int64_t granules, int32_t liveGrid, hipStream_t stream);
+// Planning of threshold queries: `waves` item waves (a multiple of 256 up to kMaxPruneWaves) and
+// the learned batches' header wave; nqKnown = the batch's query count, or -1 to read it from the
+// packet (captured graphs).
 bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream);
+                 int32_t waves, int32_t nqKnown, hipStream_t stream);
+// Item waves of the planning kernel for a batch of `items` items (64 granules of one query each).
+int32_t PruneWaves(int64_t items);
 int ScoreBlocksPerCu(int blockDocs);
 // One workgroup per query slot (workgroups beyond the batch's queries only sum counter rows).
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t workgroups, unsigned long long *hostStats,

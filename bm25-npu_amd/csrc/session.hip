@@ -238,13 +238,15 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(perCu >= 1, "The scoring kernel does not fit this device");
     grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : cus * perCu;
     Require(grid >= 1 && grid <= (1 << 20), "grid_blocks out of range");
-    const int prunePerCu = PruneBlocksPerCu(blockDocs);
-    Require(prunePerCu >= 1, "The planning kernel does not fit this device");
-    pruneCapacity = cus * prunePerCu;
+    Require(PruneBlocksPerCu(blockDocs) >= 1, "The planning kernel does not fit this device");
     const int livePerCu = LiveBlocksPerCu(blockDocs);
     Require(livePerCu >= 1, "The live kernel does not fit this device");
+#ifdef BM25MI_LIVE_PER_CU
+    liveGrid = cus * std::min(livePerCu, BM25MI_LIVE_PER_CU);
+#else
     liveGrid = cus * livePerCu;
-    // Counter rows, live plans and dead counts: 2 x the largest grid.
+#endif
+    // Counter rows: 2 x the largest grid.
     gridCapacity = std::max(std::max(grid, liveGrid), 2 * cus * perCu);
     Require(static_cast<int64_t>(cfg.max_queries) * blocks < (int64_t(1) << 32), "Too many scoring tasks per Search");
     itemsPerQuery = (granules + kWave - 1) / kWave;
@@ -370,8 +372,8 @@ void Session::InitSlot(Slot &slot)
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
     slot.dStats = Alloc<unsigned long long>(2 * static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
     slot.dItems = Alloc<unsigned long long>(q * static_cast<size_t>(itemsPerQuery), "alloc live item masks");
-    slot.dPlan = Alloc<uint4>(static_cast<size_t>(gridCapacity), "alloc live plan");
-    slot.dDead = Alloc<uint32_t>(static_cast<size_t>(pruneCapacity), "alloc dead counts");
+    slot.dDead = Alloc<uint32_t>(kMaxPruneWaves, "alloc dead counts");
+    slot.dLive = Alloc<uint32_t>(kMaxPruneWaves, "alloc live counts");
     slot.dCursor = Alloc<uint32_t>(std::max<size_t>(1, static_cast<size_t>(cfg.max_words)) * static_cast<size_t>(blocks),
                                    "alloc short-list cursors");
     slot.bt = PacketView(static_cast<unsigned char *>(slot.dPacket));
@@ -384,7 +386,7 @@ void Session::InitSlot(Slot &slot)
                        static_cast<unsigned long long *>(slot.dStats),
                        static_cast<unsigned long long *>(slot.dItems),
                        static_cast<uint32_t *>(slot.dDead),
-                       static_cast<uint4 *>(slot.dPlan),
+                       static_cast<uint32_t *>(slot.dLive),
                        static_cast<uint32_t *>(slot.dCursor),
                        reinterpret_cast<int32_t *>(rs + resStatus),
                        reinterpret_cast<float *>(rs + resScores),
@@ -445,6 +447,9 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
 {
     // Learned batches: the staging kernel also predicts the thresholds (one wave per query).
     const int32_t predict = mode == 3 ? (capture ? cfg.max_queries : slot.nq) : 0;
+    // Mixed and learned batches also run the plain kernel, unless no query of a learned batch can
+    // be routed to the plain path (a captured graph keeps it).
+    const bool plainToo = mode == 1 || (mode == 3 && (capture || !slot.allLive));
     LaunchStage(static_cast<const uint4 *>(slot.hPacketDev), static_cast<uint4 *>(slot.dPacket),
                 capture ? FullRanges(mode == 3) : slot.ranges, slot.btHost,
                 reinterpret_cast<float *>(static_cast<unsigned char *>(slot.dPacket) + offThr), Model(), predict,
@@ -458,20 +463,16 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
                             timed ? slot.ev[2] : nullptr),
                 "Unsupported block size");
     } else {
-        // One stream: threshold prediction (learned batches), planning, the live plan (with the
-        // plain queries' wave plan of learned batches), live scoring, then plain scoring of the
-        // queries routed to the plain path (mixed and learned batches), which adds its counters to
-        // the live kernel's rows.
-        // One planning wave per item of the batch, up to the waves resident at once.
-        const int32_t pruneGrid = capture ? pruneCapacity
-                                          : static_cast<int32_t>(std::min<int64_t>(
-                                                std::max<int64_t>(1, static_cast<int64_t>(slot.nq) * itemsPerQuery),
-                                                pruneCapacity));
-        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, pruneGrid, slot.stream), "Unsupported block size");
-        LaunchPlan(slot.bt, slot.wk, granules, liveGrid, slot.stream);
-        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, liveGrid, 0, pruneGrid, slot.stream),
+        // One stream: planning (with the plain queries' wave plan of learned batches), live
+        // scoring, then plain scoring of the queries routed to the plain path (mixed and learned
+        // batches), which adds its counters to the live kernel's rows.
+        const int32_t pruneWaves =
+            PruneWaves(static_cast<int64_t>(capture ? cfg.max_queries : slot.nq) * itemsPerQuery);
+        Require(LaunchPrune(blockDocs, ix, slot.bt, slot.wk, blocks, pruneWaves, capture ? -1 : slot.nq, slot.stream),
                 "Unsupported block size");
-        if (mode != 2)
+        Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, liveGrid, 0, pruneWaves, slot.stream),
+                "Unsupported block size");
+        if (plainToo)
             Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, nullptr, nullptr, liveGrid),
                     "Unsupported block size");
     }
@@ -479,7 +480,7 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
         BM25MI_HIP(hipEventRecord(slot.ev[2], slot.stream), "record score end");
     // Counter rows: [0, grid) of the plain kernel, [0, liveGrid) of the live kernel (the plain
     // kernel adds into the live kernel's rows where both wrote).
-    const int32_t rows = mode == 0 ? grid : mode == 2 ? liveGrid : std::max(grid, liveGrid);
+    const int32_t rows = mode == 0 ? grid : plainToo ? std::max(grid, liveGrid) : liveGrid;
     // One final workgroup per query of the batch, and at least 8 (up to the session's capacity)
     // to share the counter rows; a captured graph has one per query of the session's capacity.
     // Workgroups beyond the batch's queries only sum counter rows.
@@ -540,7 +541,7 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
     // A threshold below kLiveMinRatio of the query's score upper bound (the FP32 sum of its
     // terms' largest impacts) leaves most granules live, and above K = liveMaxK the live path's
     // per-granule cut costs more than it saves: such a query is scored on the plain path, with
-    // the same results. Learned batches are routed on the device (KThreshold) by the same rules;
+    // the same results. Learned batches are routed on the device (staging kernel) by the same rules;
     // their queries with K <= liveMaxK get pools that fit either path.
     // Per-query host state lives in the slot's buffers, sized at create time: packing allocates
     // nothing.
@@ -605,10 +606,15 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
         listed[q] = thr[q] > 0.0F ? 1 : 0;
         withThr += listed[q] ? 1 : 0;
     }
+    // A learned batch none of whose queries may take the live path (every K above liveMaxK, or
+    // all excluded by the caller) is launched as a plain batch: no prediction or planning.
+    bool learnedLaunch = false;
+    for (int32_t q = 0; q < nq && query.learned && !learnedLaunch; ++q)
+        learnedLaunch = Predicted(query, q);
     reinterpret_cast<int32_t *>(pk)[1] = withThr;
     reinterpret_cast<int32_t *>(pk)[2] = nq - withThr;
-    reinterpret_cast<int32_t *>(pk)[3] = query.learned ? 1 : 0;
-    slot.mode = query.learned ? 3 : withThr == 0 ? 0 : withThr < nq ? 1 : 2;
+    reinterpret_cast<int32_t *>(pk)[3] = learnedLaunch ? 1 : 0;
+    slot.mode = learnedLaunch ? 3 : withThr == 0 ? 0 : withThr < nq ? 1 : 2;
     // Learned batches: the plan with every query plain, the most the device-side routing can
     // keep (checked here), and what the threshold kernel needs to route and re-plan.
     PlanWaves(perQuery, listed, nq, reinterpret_cast<int64_t *>(pk + offCost));
@@ -626,30 +632,39 @@ void Session::Pack(const Query &query, Slot &slot, int64_t poolBudget)
         add(0, 16);
         add(offQoff, (n + 1) * 8);
         add(offK, n * 4);
-        if (!query.learned)
+        if (!learnedLaunch)
             add(offThr, n * 4);
         add(offTau0, n * 4);
         add(offPool, (n + 1) * 8);
         add(offOut, (n + 1) * 8);
         add(offTerms, static_cast<size_t>(words) * 16);
         add(offCost, (n + 1) * 8);
-        if (query.learned) {
+        if (learnedLaunch) {
             add(offFeat, n * kFeatures * 4);
             add(offUpper, n * 4);
             add(offPlanCost, n * 8);
         }
     }
-    if (query.learned) {
+    slot.allLive = false;
+    if (learnedLaunch) {
         auto *feat = reinterpret_cast<float *>(pk + offFeat);
         auto *upper = reinterpret_cast<float *>(pk + offUpper);
         auto *planCost = reinterpret_cast<int64_t *>(pk + offPlanCost);
+        bool allLive = true;
         for (int32_t q = 0; q < nq; ++q) {
             Features(query.terms + query.offsets[q], query.offsets[q + 1] - query.offsets[q], query.k[q],
                      feat + static_cast<int64_t>(q) * kFeatures, upper + q);
             if (!Predicted(query, q))
-                upper[q] = 0.0F; // routed to the plain path by the threshold kernel
+                upper[q] = 0.0F; // routed to the plain path by the staging kernel
+            // The predicted threshold is clamped to at least the start bound, so a start bound
+            // of at least kLiveMinRatio of the upper bound (the same FP32 product as on the
+            // device) keeps the query on the live path whatever the model predicts.
+            float lower;
+            std::memcpy(&lower, &tau0[q], 4);
+            allLive = allLive && upper[q] > 0.0F && query.k[q] <= liveMaxK && lower >= kLiveMinRatio * upper[q];
             planCost[q] = (taskCost + (perQuery[q] + blocks - 1) / blocks) * blocks;
         }
+        slot.allLive = allLive;
     }
     slot.nq = nq;
     slot.total = total;
@@ -813,7 +828,7 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
     wk.stats = reinterpret_cast<unsigned long long *>(base + offRows);
     wk.itemMask = nullptr; // threshold planning only
     wk.dead = wk.cursor = nullptr;
-    wk.plan = nullptr;
+    wk.live = nullptr;
     try {
         slot.timed = false;
         LaunchStage(static_cast<const uint4 *>(slot.hPacketDev), reinterpret_cast<uint4 *>(base), slot.ranges,
@@ -839,6 +854,10 @@ void Session::PredictThresholds(const Query &query, float *out)
     Require(dModel != nullptr, "Learned thresholds need a threshold model (bm25mi_set_threshold_model)");
     Slot &slot = slots[0];
     Pack(query, slot);
+    if (slot.mode != 3) {
+        std::fill_n(out, query.count, 0.0F); // no query may take the live path
+        return;
+    }
     try {
         LaunchStage(static_cast<const uint4 *>(slot.hPacketDev), static_cast<uint4 *>(slot.dPacket), slot.ranges,
                     slot.btHost, reinterpret_cast<float *>(static_cast<unsigned char *>(slot.dPacket) + offThr),
@@ -920,7 +939,7 @@ void Session::Release() noexcept
             slot.stream = nullptr;
         }
         for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dItems,
-                         &slot.dDead, &slot.dCursor, &slot.dPlan})
+                         &slot.dDead, &slot.dCursor, &slot.dLive})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
@@ -1119,11 +1138,21 @@ extern "C" int bm25mi_set_threshold_model(bm25mi_session *s, const float *model,
                         safety > 0.0F,
                     "Invalid model output transform");
             Session &ss = *s->impl;
+            // Device layout: both weight matrices regrouped so that hidden unit j's four inputs
+            // 4v .. 4v+3 are float4 v * 64 + j (one coalesced 16-byte load per lane and v).
+            std::vector<float> dev(model, model + count);
+            const auto regroup = [&](int64_t at, int inputs) {
+                for (int j = 0; j < kHidden; ++j)
+                    for (int i = 0; i < inputs; ++i)
+                        dev[at + (static_cast<int64_t>(i / 4) * kHidden + j) * 4 + i % 4] = model[at + j * inputs + i];
+            };
+            regroup(0, kFeatures);
+            regroup(int64_t(kHidden) * kFeatures + kHidden, kHidden);
             try {
                 BM25MI_HIP(hipSetDevice(ss.device), "set device");
                 if (!ss.dModel)
                     BM25MI_HIP(hipMalloc(&ss.dModel, static_cast<size_t>(count) * sizeof(float)), "alloc model");
-                BM25MI_HIP(hipMemcpy(ss.dModel, model, static_cast<size_t>(count) * sizeof(float),
+                BM25MI_HIP(hipMemcpy(ss.dModel, dev.data(), static_cast<size_t>(count) * sizeof(float),
                                      hipMemcpyHostToDevice),
                            "upload model");
             } catch (const DeviceError &) {
@@ -1294,16 +1323,17 @@ extern "C" int bm25mi_set_launch(bm25mi_session *s, int32_t grid_waves, int32_t 
 }
 
 #ifdef BM25MI_PHASES
-// Measurement builds only: the per-wave counter rows of slot 0's last Search ([grid][16]).
+// Measurement builds only: the per-wave counter rows of slot 0's last Search
+// ([max(scoring grid, live grid)][16]).
 extern "C" int bm25mi_debug_wave_rows(bm25mi_session *s, unsigned long long *out, int64_t rows)
 {
     return Invoke([&] {
         Require(s && s->impl && out, "Null session or output");
         std::lock_guard<std::mutex> guard(s->mutex);
         Session &ss = *s->impl;
-        Require(rows >= ss.grid, "Output holds fewer rows than the scoring grid");
-        BM25MI_HIP(hipMemcpy(out, ss.slots[0].dStats, sizeof(unsigned long long) * kStats * ss.grid,
-                             hipMemcpyDeviceToHost),
+        const int64_t n = std::max(ss.grid, ss.liveGrid);
+        Require(rows >= n, "Output holds fewer rows than the scoring and live grids");
+        BM25MI_HIP(hipMemcpy(out, ss.slots[0].dStats, sizeof(unsigned long long) * kStats * n, hipMemcpyDeviceToHost),
                    "copy wave rows");
     });
 }

@@ -59,6 +59,26 @@ def test_synthetic8k_q8x3_k10(synth, synth_session):
         assert_same(got_s, got_i, want_s, want_i, f'seed {seed}')
 
 
+def test_poisoned_output_buffer_is_fully_written(synth, synth_session):
+    """A caller-supplied output buffer pre-filled with a NaN / invalid-ID pattern (the
+    reference's output poisoning, FR/test/ascendhost/bm25/search_diagnostic.cpp:151-161) holds
+    exactly the oracle's rows afterwards; a buffer of the wrong size is rejected."""
+    corpus, index = synth
+    queries = bm25mi.draw_queries(corpus.offsets, 8, 3, seed=11)
+    want_s, want_i = orc.search(index, *bm25mi.pack(queries), 10)
+    for learned_path in (False, True):
+        buf = np.full(2 * 80 + 8, 0x7FC0DEAD, np.int32)
+        thr = None
+        if learned_path:
+            lower, _ = synth_session.bounds(queries, 10)
+            thr = lower  # a provable lower bound of each K-th score: the threshold path, same rows
+        got_s, got_i, fb = synth_session.search(queries, 10, thresholds=thr, out=buf)
+        assert_same(got_s, got_i, want_s, want_i, f'poisoned buffer, thresholds={learned_path}')
+        assert np.shares_memory(got_s, buf) and not fb.any()
+    with pytest.raises(ValueError):
+        synth_session.search(queries, 10, out=np.zeros(10, np.int32))
+
+
 @pytest.mark.parametrize('k', [1, 2, 10, 32, 100, 1000, 1024])
 def test_synthetic8k_all_k(synth, synth_session, k):
     corpus, index = synth
