@@ -422,6 +422,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // The query's pool (offset, capacity) and its score-histogram bound as last read.
     int64_t poolBase = 0, poolCap = 0;
     uint32_t histBound = 1, histAge = 0;
+    // This wave's own appends to the query's pool, per histogram bucket (lane l: buckets
+    // l*kHistPerLane ...): k of them at or above a bucket's edge bound the K-th score from below
+    // as well. The bound is taken each time the count has doubled (a few times per query) and
+    // published through tau, so that every wave of the query starts its selections from it.
+    uint32_t ownHist[kHistPerLane];
+#pragma unroll
+    for (int i = 0; i < kHistPerLane; ++i)
+        ownHist[i] = 0;
+    uint32_t ownCount = 0, ownAt = 0;
     for (uint32_t t = tBegin; t < tEnd; ++t) {
         BM25MI_PHASE(0);
         const int q = static_cast<int>(t / static_cast<uint32_t>(blocks));
@@ -443,6 +452,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             poolCap = bt.poolOff[q + 1] - poolBase;
             histBound = 1;
             histAge = kHistEvery;
+#pragma unroll
+            for (int i = 0; i < kHistPerLane; ++i)
+                ownHist[i] = 0;
+            ownCount = ownAt = 0;
         }
         if (thr > 0.0F) {
             // A threshold query's live blocks are scored by KLive.
@@ -450,15 +463,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             continue;
         }
 
-        // Lower bound of the query's K-th score, needed only by the selection: tau is loaded
-        // here, every task, and the score histogram at the query's first task here and then
-        // every kHistEvery-th task; both are used in the selection, so the loads stay in flight
-        // through the streaming. These bounds only grow, so a value read early is still a lower
-        // bound. The histogram is not read every task: the appends' atomics keep its lines out
-        // of the caches, and a load's completion is counted in order with the posting loads
-        // issued after it, so every read stalls the task's first column round (measured on the
-        // 10M-doc Q8 x 3-term K10 Search: 97 us reading every task, 94 us every 8th task and
-        // after each append, 84 us every 16th task, with 3.4K instead of 1.1K candidates).
+        // Lower bound of the query's K-th score, needed only by the selection: the score
+        // histogram, read at the query's first task here and then every kHistEvery-th task, and
+        // tau (below). These bounds only grow, so a value read early is still a lower bound. The
+        // histogram is not read every task: the appends' atomics keep its lines out of the
+        // caches, and a load's completion is counted in order with the posting loads issued
+        // after it, so every read stalls the task's first column round (measured on the 10M-doc
+        // Q8 x 3-term K10 Search: 97 us reading every task, 94 us every 8th task and after each
+        // append, 84 us every 16th task; reading it after the first round of posting loads at
+        // each wave's 4th task halved the candidates but doubled the streaming time).
         const bool readHist = histAge >= (k > kHistLargeK ? kHistEveryLargeK : kHistEvery);
         uint32_t hregs[kHistPerLane];
 #pragma unroll
@@ -472,7 +485,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         ++histAge;
-        const uint32_t tauLoaded = __hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // The query's tau: every task, loaded (device scope: the raises the other waves
+        // published, not a cached copy) after the task's first round of posting loads, so that a
+        // slow read of this contended line delays only the selection, not the postings.
+        uint32_t tauLoaded = 1;
 
         const int64_t g0 = static_cast<int64_t>(blk) * kGpb;
         const int gpb = static_cast<int>(min(static_cast<int64_t>(kGpb), ix.granules - g0));
@@ -626,6 +642,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                     }
                 }
                 if (r0 == 0) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    tauLoaded = __hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (r0 == 0) {
 #pragma unroll
                     for (int r = 0; r < kVec; ++r)
                         acc4[r * kWave + lane] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
@@ -747,6 +768,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             kept = cr.kept;
             mine = cr.mine;
             tauNew = cr.tauNew;
+            histBound = max(histBound, tauNew); // the block holds k docs at or above it
             ++cuts;
         }
         uint32_t slot0 = 0;
@@ -793,10 +815,19 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         for (int r = 0; r < kHistPerLane; ++r) {
             const uint32_t bin = lane * kHistPerLane + r;
             const uint32_t c = sHist[bin];
+            ownHist[r] += c;
             if (c)
                 atomicAdd(histw + bin, c);
         }
-        if (lane == 0 && tauNew)
+        ownCount += kept;
+        if (ownCount >= k && ownCount >= 2 * ownAt) {
+            ownAt = ownCount;
+            const uint32_t ob = WaveHistBoundRegs(ownHist, k);
+            if (ob > histBound && ob > tauLoaded)
+                tauNew = max(tauNew, ob);
+            histBound = max(histBound, ob);
+        }
+        if (lane == 0 && tauNew > tauLoaded)
             atomicMax(wk.tau + q, tauNew);
         emitted += kept;
         ++emitting;
