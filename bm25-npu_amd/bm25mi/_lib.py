@@ -31,6 +31,7 @@ class Config(C.Structure):
         ('slots', C.c_int32),
         ('pool_entries', C.c_int64),
         ('live_max_k', C.c_int32),
+        ('keep_cap', C.c_int32),
     ]
 
 

@@ -30,7 +30,7 @@ class Session:
     """A resident index on one GPU. All searches go through libbm25mi."""
 
     def __init__(self, index, max_queries=8, max_words=24, max_k=10, block_docs=0, k1=None, device=0,
-                 use_graph=False, grid_blocks=0, task_cost=0, slots=1, pool_entries=0, live_max_k=0):
+                 use_graph=False, grid_blocks=0, task_cost=0, slots=1, pool_entries=0, live_max_k=0, keep_cap=0):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.bm25mi_config_default(C.byref(cfg))
@@ -46,6 +46,7 @@ class Session:
         cfg.slots = slots
         cfg.pool_entries = pool_entries
         cfg.live_max_k = live_max_k
+        cfg.keep_cap = keep_cap
         self.config = cfg
         self._kcache = {}
         self.num_docs, self.vocab = index.num_docs, index.vocab

@@ -55,9 +55,10 @@ constexpr uint32_t kHistBase = (127u - 4u) << 4;
 
 // Candidates a block may append: all of them when at most this many score at or above the
 // query's floor, else a cut keeping between K and this many. It sizes the candidate pools.
-__host__ __device__ constexpr int32_t BlockKeepCap(int32_t k, int32_t blockDocs)
+// keepCap > 0 (the session's keep_cap) replaces the default max(2K, 64) with max(K, keepCap).
+__host__ __device__ constexpr int32_t BlockKeepCap(int32_t k, int32_t blockDocs, int32_t keepCap = 0)
 {
-    const int32_t c = 2 * k > 64 ? 2 * k : 64;
+    const int32_t c = keepCap > 0 ? (keepCap > k ? keepCap : k) : (2 * k > 64 ? 2 * k : 64);
     return c < blockDocs ? c : blockDocs;
 }
 

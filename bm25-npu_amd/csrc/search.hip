@@ -362,10 +362,10 @@ __device__ __forceinline__ void WaveRange(const BatchView &bt, int32_t blocks, u
 }
 
 template <int B>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t addRows)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t addRows, int32_t keepCap)
 {
     // addRows: the live kernel ran before this one in the same Search and wrote counter rows
-    // [0, addRows); a wave below that adds its counters to its row.
+    // [0, addRows); a wave below that adds its counters to its row. keepCap: BlockKeepCap's.
     constexpr int kVec = B / (4 * kWave);   // float4 accumulators per lane
     constexpr int kGpb = B / kGranule;      // granules per block
     constexpr int U = kColumns;
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // one per non-empty bucket counts them, and the cut raises tau.
         LdsU32 *bitsL = (LdsU32 *)bits4; // acc is in LDS
         uint32_t tb = floor, te = 0, tauNew = 0;
-        const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B));
+        const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B, keepCap));
         if (kept > cap) {
             const CutResult cr = SelectCut<B>(bitsL, floor, k, cap, mx);
             tb = cr.tb;
@@ -2085,19 +2085,22 @@ void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 }
 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, int32_t addRows)
+                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, int32_t addRows, int32_t keepCap)
 {
     // With events, hipExtLaunchKernelGGL stamps them at the kernel's own start and end (the
     // interval the profiler reports), not around the launch.
     switch (blockDocs) {
     case 2048:
-        hipExtLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows);
+        hipExtLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows,
+                              keepCap);
         return true;
     case 4096:
-        hipExtLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows);
+        hipExtLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows,
+                              keepCap);
         return true;
     case 8192:
-        hipExtLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows);
+        hipExtLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows,
+                              keepCap);
         return true;
     default:
         return false;

@@ -91,7 +91,8 @@ static void FailIfInjected(const char *stage)
 
 int64_t Session::PoolCapacity(int32_t k, bool threshold) const
 {
-    const int64_t plain = std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs), ndocs);
+    const int64_t plain =
+        std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs, cfg.keep_cap), ndocs);
     // A threshold query may be scored on the plain path (Pack), so its pool covers both.
     return threshold ? std::max(plain, std::min<int64_t>(granules * std::min<int64_t>(k, kGranule), ndocs)) : plain;
 }
@@ -198,6 +199,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(cfg.slots >= 0 && cfg.slots <= 16, "slots must be in [0, 16]");
     Require(cfg.pool_entries >= 0, "pool_entries must be >= 0");
     Require(cfg.live_max_k >= 0 && cfg.live_max_k <= BM25MI_MAX_K, "live_max_k must be in [0, 1024]");
+    Require(cfg.keep_cap >= 0 && cfg.keep_cap <= 8192, "keep_cap must be in [0, 8192]");
     if (cfg.task_cost > 0)
         taskCost = cfg.task_cost;
     blockDocs = cfg.block_docs ? cfg.block_docs : 2048;
@@ -460,7 +462,7 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
     if (mode == 0) {
         Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, timed ? slot.ev[1] : nullptr,
-                            timed ? slot.ev[2] : nullptr),
+                            timed ? slot.ev[2] : nullptr, 0, cfg.keep_cap),
                 "Unsupported block size");
     } else {
         // One stream: planning (with the plain queries' wave plan of learned batches), live
@@ -473,7 +475,8 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
         Require(LaunchLive(blockDocs, ix, slot.bt, slot.wk, blocks, liveGrid, 0, pruneWaves, slot.stream),
                 "Unsupported block size");
         if (plainToo)
-            Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, nullptr, nullptr, liveGrid),
+            Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, nullptr, nullptr, liveGrid,
+                                cfg.keep_cap),
                     "Unsupported block size");
     }
     if (timed && mode != 0)
@@ -836,7 +839,8 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
         // The scratch keeps the last call's contents, possibly another session's: the counters
         // are set by a fill kernel before scoring.
         LaunchInit(wk, cfg.max_queries, slot.stream);
-        Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream), "Unsupported block size");
+        Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream, nullptr, nullptr, 0, cfg.keep_cap),
+                "Unsupported block size");
         slot.finalGrid = FinalGrid(slot.nq);
         LaunchFinal(bt, wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), grid, slot.stream);
         BM25MI_HIP(hipGetLastError(), "launch Search kernels (scratch)");
@@ -978,6 +982,7 @@ extern "C" void bm25mi_config_default(bm25mi_config *cfg)
     cfg->slots = 1;
     cfg->pool_entries = 0;
     cfg->live_max_k = 0;
+    cfg->keep_cap = 0;
 }
 
 extern "C" int bm25mi_create(const int64_t *term_offsets, int64_t vocab, const int32_t *doc_ids, const float *tf,

@@ -74,6 +74,11 @@ typedef struct {
                                 (a batch whose queries need more is rejected / split by the server) */
     int32_t live_max_k;      /* largest K a threshold query is scored with on the live path (larger
                                 ones take the plain path); 0 = 64 */
+    int32_t keep_cap;        /* candidates a scoring task may append before it cuts to its top
+                                max(K, keep_cap) (at most block_docs; sizes the candidate pools);
+                                0 = max(2K, 64). A launch setting like block_docs: results are
+                                the same for every value (the reference's candidate capacity
+                                variants, tools/capacity_sources.py) */
 } bm25mi_config;
 
 /* Per-Search counters, filled by bm25mi_last_stats. */

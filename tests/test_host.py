@@ -229,6 +229,7 @@ def test_config_defaults():
     _lib.load().bm25mi_config_default(C.byref(cfg))
     assert cfg.use_graph == 0 and cfg.block_docs == 2048
     assert cfg.grid_blocks == 0 and cfg.task_cost == 0 and cfg.slots == 1 and cfg.pool_entries == 0
+    assert cfg.live_max_k == 0 and cfg.keep_cap == 0
     assert cfg.max_queries == 8 and cfg.max_words == 24 and cfg.max_k == 10 and cfg.k1 == 1.2
 
 
