@@ -81,7 +81,7 @@ def measured_traffic(kernel, args):
     if not path.exists() or (args.corpus, args.queries, args.terms, args.k) != ('synthetic10m', 8, 3, 10):
         return None, None
     rec = json.loads(path.read_text())
-    if rec.get('kernel') != kernel:
+    if rec.get('kernel') != kernel or rec.get('rotation') != args.rotation:
         return None, None
     return rec['traffic_bytes_per_launch'], rec['source']
 
@@ -430,6 +430,10 @@ def serving_leg(args, index, offsets, model):
         lengths = rng.integers(1, 33, size=n)
         ks = rng.choice(np.array([10, 100, 1000], np.int32), size=n, p=[0.7, 0.25, 0.05]).astype(np.int32)
         queries = bm25mi.draw_queries(offsets, n, lengths, seed=int(qps))
+        # The plain path streams every posting of a request's terms (8 B each): the HBM traffic
+        # the offered rate asks for, against the 8 TB/s peak.
+        df = np.diff(offsets)
+        post = float(np.mean([df[q].sum() for q in queries]))
         plain = None
         for mode in (['plain', 'learned'] if model is not None else ['plain']):
             with bm25mi.Server(s, max_batch=256) as srv:
@@ -439,7 +443,10 @@ def serving_leg(args, index, offsets, model):
             rec = {'mode': mode, 'offered_qps': qps, 'requests': n, 'sustained_qps': r['sustained_qps'],
                    'p50_ms': r['p50_ms'], 'p95_ms': r['p95_ms'], 'p99_ms': r['p99_ms'], 'batches': c['batches'],
                    'mean_batch': c['queries'] / max(1, c['batches']), 'max_inflight': c['max_inflight'],
-                   'fallbacks': c['fallbacks']}
+                   'fallbacks': c['fallbacks'], 'postings_per_request': post,
+                   'plain_gbs_at_offered': post * 8 * qps / 1e9,
+                   'plain_gbs_at_sustained': post * 8 * r['sustained_qps'] / 1e9,
+                   'hbm_peak_gbs': HBM_PEAK_GBS}
             if mode == 'plain':
                 plain = (r['scores'], r['ids'])
             else:
@@ -509,7 +516,7 @@ def main():
     ap.add_argument('--serving-requests', type=int, default=6000)
     ap.add_argument('--serving-seconds', type=float, default=1.0)
     ap.add_argument('--pool-entries', type=int, default=1 << 26, help='candidate-pool entries per serving slot')
-    ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: headline leg only')
+    ap.add_argument('--profile', action='store_true', help='short run for rocprofv3: the rotation only')
     ap.add_argument('--rounds', type=int, default=0,
                     help='add the round protocol: this many rounds of the GPU and CPU roles, one process each')
     ap.add_argument('--round-cycles', type=int, default=30, help='timed A/B cycles per role and round')
@@ -545,16 +552,19 @@ def main():
     # Reference protocol on A/B, then the rotation (whose figures are the headline's): warm-up,
     # then K timed end-to-end Searches (host IDs -> host top-K) in forward/backward cycles, every
     # call into its own poisoned output buffer, all bit-checked after the timed region.
-    ab = headline_protocol(session, packed, args)
-    log(f'A/B: {args.steps} Searches, median {ab["ms_median"]:.4f} ms, bit checks {ab["bit_checks"]}, '
-        f'mismatches {ab["mismatches"]}')
+    # A profiling run (--profile) times the rotation only, so that a trace's dispatches are its.
+    ab = None
+    if not args.profile:
+        ab = headline_protocol(session, packed, args)
+        log(f'A/B: {args.steps} Searches, median {ab["ms_median"]:.4f} ms, bit checks {ab["bit_checks"]}, '
+            f'mismatches {ab["mismatches"]}')
+        ab.pop('results')
+        ab.pop('total_s')
     rot = headline_protocol(session, rotation, args)
     log(f'rotation of {len(rotation)}: {args.steps} Searches, median {rot["ms_median"]:.4f} ms, '
         f'bit checks {rot["bit_checks"]}, mismatches {rot["mismatches"]}')
-    ab.pop('results')
     results = rot.pop('results')
     total_s = rot.pop('total_s')
-    ab.pop('total_s')
 
     # Scoring-kernel time with HIP events on the launch stream (direct launches), per batch of the
     # rotation (and of A/B for comparison).
@@ -575,8 +585,11 @@ def main():
     k_ms = statistics.mean(kernel_ms)
     b_avg = statistics.mean(bytes_per)
     achieved = b_avg / (k_ms * 1e-3) / 1e9
-    ab_kernel, ab_bytes, _, _ = kernel_times(packed, max(20, min(args.steps, 100)))
-    ab_achieved = statistics.mean(ab_bytes) / (statistics.mean(ab_kernel) * 1e-3) / 1e9
+    ab_frac = ab_kernel_ms = None
+    if not args.profile:
+        ab_kernel, ab_bytes, _, _ = kernel_times(packed, max(20, min(args.steps, 100)))
+        ab_kernel_ms = statistics.mean(ab_kernel)
+        ab_frac = statistics.mean(ab_bytes) / (ab_kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     session.search(packed[0], args.k, timed=True)
     st = session.stats()
     call_stats = []
@@ -612,7 +625,7 @@ def main():
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'traffic_source': traffic_note,
                      'kernel': f'KScore<{st["block_docs"]}>', 'kernel_ms': k_ms, 'algorithmic_bytes': b_avg,
                      'launch_waves': st['score_grid'], 'batches': 'rotation',
-                     'ab_frac': ab_achieved / HBM_PEAK_GBS, 'ab_kernel_ms': statistics.mean(ab_kernel)},
+                     'ab_frac': ab_frac, 'ab_kernel_ms': ab_kernel_ms},
         'protocol': {'rotation': rot, 'ab': ab, 'postings': data,
                      'rule': 'forward/backward cycles over the batches (AB, BA, ... for A/B); every timed '
                              'call writes its own output buffer, pre-filled with a NaN/invalid-ID pattern, '
