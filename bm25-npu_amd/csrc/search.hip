@@ -30,20 +30,40 @@
 
 namespace bm25mi {
 
+__device__ __forceinline__ uint32_t ReadLane(uint32_t v, int lane)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+}
+
+// Wave-wide reductions (every lane active, every lane gets the result): the device library's
+// DPP row operations and two lane reads, instead of six dependent LDS permutes.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_or_u64(unsigned long long);
+
 __device__ __forceinline__ uint32_t WaveSum(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o);
-    return v;
+    return __ockl_wfred_add_u32(v);
 }
 
 __device__ __forceinline__ uint32_t WaveMax(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = max(v, static_cast<uint32_t>(__shfl_xor(v, o)));
-    return v;
+    return __ockl_wfred_max_u32(v);
+}
+
+// Inclusive prefix sum over the 64 lanes (every lane active): row_shr 1, 2, 4, 8 within each
+// 16-lane row (lanes without a source add 0), then row 0's total into row 1 and row 2's into
+// row 3 (row_bcast:15), then rows 0-1's total into rows 2-3 (row_bcast:31). Six DPP adds instead
+// of six dependent LDS permutes.
+__device__ __forceinline__ uint32_t WaveInclusiveSum(uint32_t v)
+{
+    int x = static_cast<int>(v);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true); // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true); // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true); // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true); // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15, rows 1 and 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31, rows 2 and 3
+    return static_cast<uint32_t>(x);
 }
 
 // Block-wide sum/max over NT threads; red must hold NT/64 words. All threads call.
@@ -79,13 +99,7 @@ template <int NT> __device__ __forceinline__ uint32_t BlockMax(uint32_t v, uint3
 template <int NT> __device__ __forceinline__ uint32_t BlockExclusive(uint32_t v, uint32_t *red, uint32_t *total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o)
-            x += y;
-    }
+    const uint32_t x = WaveInclusiveSum(v);
     __syncthreads();
     if (lane == 63)
         red[wave] = x;
@@ -178,14 +192,8 @@ __device__ __forceinline__ uint32_t WaveHistBoundRegs(const uint32_t (&h)[kHistP
     for (int i = 0; i < kHistPerLane; ++i)
         sum += h[i];
     // above = counts in buckets of higher lanes (suffix over lanes).
-    uint32_t x = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_down(x, o);
-        if (lane + o < 64)
-            x += y;
-    }
-    uint32_t above = x - sum;
+    const uint32_t incl = WaveInclusiveSum(sum);
+    uint32_t above = ReadLane(incl, kWave - 1) - incl;
     uint32_t best = 0; // bucket + 1, or 0
 #pragma unroll
     for (int i = kHistPerLane - 1; i >= 0; --i) {
@@ -210,17 +218,9 @@ __device__ __forceinline__ uint32_t WaveHistBound(const uint32_t *hist, uint32_t
 
 __device__ __forceinline__ unsigned long long WaveSum64(unsigned long long v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o);
-    return v;
+    return __ockl_wfred_add_u64(v);
 }
 
-
-__device__ __forceinline__ uint32_t ReadLane(uint32_t v, int lane)
-{
-    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
-}
 
 // k-th largest (1 <= k <= 64) of one value per lane: a bitonic sort across the 64 lanes
 // through shuffles, descending, then lane k-1.
@@ -609,12 +609,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         auto streamAdd = [&](uint32_t segLo, uint32_t segLen, int nseg) {
             const uint32_t myCols = static_cast<int>(lane) < nseg ? (segLen + kWave - 1) / kWave : 0u;
             uint32_t colEnd = myCols;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(colEnd), o));
-                if (lane >= static_cast<uint32_t>(o))
-                    colEnd += y;
-            }
+            colEnd = WaveInclusiveSum(colEnd);
             const uint32_t columns = ReadLane(colEnd, kWave - 1);
             const uint32_t colStart = colEnd - myCols;
             // Lanes past the last segment hold ~0: column c belongs to segment
@@ -775,12 +770,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         if (lane == 0)
             slot0 = atomicAdd(wk.count + q, kept);
         uint32_t before = mine;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
-            if (lane >= static_cast<uint32_t>(o))
-                before += y;
-        }
+        before = WaveInclusiveSum(before);
         before -= mine;
 #pragma unroll
         for (int r = 0; r < kHistPerLane; ++r)
@@ -958,10 +948,7 @@ struct ThresholdPredictor {
 
 __device__ __forceinline__ unsigned long long WaveOr64(unsigned long long v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v |= __shfl_xor(v, o);
-    return v;
+    return __ockl_wfred_or_u64(v);
 }
 
 constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
@@ -1263,12 +1250,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         for (int j = 0; j < kMaxPruneWaves / (4 * kWave); ++j)
             sum += c4[j].x + c4[j].y + c4[j].z + c4[j].w;
         uint32_t incl = sum;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o));
-            if (lane >= static_cast<uint32_t>(o))
-                incl += y;
-        }
+        incl = WaveInclusiveSum(incl);
         const uint32_t total = ReadLane(incl, kWave - 1);
         const uint32_t r0 = static_cast<uint32_t>(static_cast<uint64_t>(total) * blockIdx.x / gridDim.x);
         const uint32_t r1 = static_cast<uint32_t>(static_cast<uint64_t>(total) * (blockIdx.x + 1) / gridDim.x);
@@ -1357,12 +1339,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                     winFloor = max(max(max(__float_as_uint(bt.thr[qw]), bt.tau0[qw]), tauW), 1u);
                     winWN = w0 << 7 | (w1 - w0);
                     winIncl = static_cast<uint32_t>(__popcll(winMask));
-#pragma unroll
-                    for (int o = 1; o < kWave; o <<= 1) {
-                        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(winIncl), o));
-                        if (lane >= static_cast<uint32_t>(o))
-                            winIncl += y;
-                    }
+                    winIncl = WaveInclusiveSum(winIncl);
                     winTotal = ReadLane(winIncl, kWave - 1);
                     if (skip < winTotal)
                         break;
@@ -1422,12 +1399,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         const uint32_t nw = isSlot ? slotNw : 0u;
         const uint32_t floorS = isSlot ? slotFloor : 1u, kS = isSlot ? slotK : 1u;
         uint32_t pairEnd = nw; // inclusive prefix of the slots' word counts
-#pragma unroll
-        for (int o = 1; o < kLiveSlots; o <<= 1) {
-            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(pairEnd), o));
-            if (lane >= static_cast<uint32_t>(o))
-                pairEnd += y;
-        }
+        pairEnd = WaveInclusiveSum(pairEnd);
         // The leading m slots whose (granule, word) pairs fit the 64 lanes (m >= 1: a query has
         // at most 64 words); the others carry over to the next batch.
         const int m = __popcll(__ballot(isSlot && pairEnd <= static_cast<uint32_t>(kWave)));
@@ -1600,12 +1572,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         }
         const uint32_t gCols = lane < groups ? (gTotal + kWave - 1) / kWave : 0u;
         uint32_t gColEnd = gCols;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(gColEnd), o));
-            if (lane >= static_cast<uint32_t>(o))
-                gColEnd += y;
-        }
+        gColEnd = WaveInclusiveSum(gColEnd);
         const uint32_t columns = ReadLane(gColEnd, kWave - 1);
         const uint32_t gColStart = gColEnd - gCols;
         if (lane >= groups)
@@ -1736,12 +1703,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             if (lane == 0)
                 slot0 = atomicAdd(wk.count + q, kept);
             uint32_t before = mine;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(before), o));
-                if (lane >= static_cast<uint32_t>(o))
-                    before += y;
-            }
+            before = WaveInclusiveSum(before);
             uint32_t slot = ReadLane(slot0, 0) + before - mine;
             unsigned long long *pool = wk.pool + bt.poolOff[q];
             const int64_t capq = bt.poolOff[q + 1] - bt.poolOff[q];
