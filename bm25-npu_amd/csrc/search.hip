@@ -148,22 +148,6 @@ template <int NT, class T> __device__ void BitonicDesc(T *s, int n)
         }
 }
 
-// The same network over one wave's 64 lanes (one key per lane), in registers: the lower lane of
-// each compare-exchange pair keeps the larger key in a descending run, the smaller otherwise.
-__device__ __forceinline__ unsigned long long WaveBitonicDesc(unsigned long long v)
-{
-    const int lane = static_cast<int>(__lane_id());
-#pragma unroll
-    for (int size = 2; size <= kWave; size <<= 1)
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const unsigned long long o = __shfl_xor(v, stride);
-            const bool takeMax = ((lane & stride) == 0) == ((lane & size) == 0);
-            v = takeMax ? (v > o ? v : o) : (v < o ? v : o);
-        }
-    return v;
-}
-
 __device__ __forceinline__ uint32_t HistBucket(uint32_t bits)
 {
     const uint32_t b = bits >> 19;
@@ -1181,9 +1165,11 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
 }
 
 #ifndef BM25MI_LIVE_SLOTS
-#define BM25MI_LIVE_SLOTS 16
+#define BM25MI_LIVE_SLOTS 8
 #endif
-// Granules a live batch scores together (16 x 128 FP32 = 8 KB of LDS per wave).
+// Granules a live batch scores together (8 x 128 FP32 = 4 KB of LDS per wave; on the 10M-doc
+// Q8 x 3-term K10 batches 8 measured 0.077 / 0.058 ms of device time per learned Search against
+// 0.079 / 0.063 for 16 and 0.113 / 0.076 for 32).
 constexpr int kLiveSlots = BM25MI_LIVE_SLOTS;
 static_assert(kLiveSlots >= 2 && kLiveSlots <= 32 && (kLiveSlots & (kLiveSlots - 1)) == 0, "live batch size");
 
@@ -1967,10 +1953,21 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     }
     const uint32_t m = c < k ? c : k;
     if (c <= static_cast<uint32_t>(kWave)) {
-        // Up to 64 survivors (the usual case at small K): one wave sorts them without barriers.
+        // Up to 64 survivors (the usual case at small K): one wave ranks them without barriers.
+        // Keys are unique, so a key's position is the number of keys above it, counted against
+        // every lane's key read as two scalars (its reads come before its writes in the wave's
+        // program order).
         if (tid < kWave) {
             const unsigned long long v = tid < static_cast<int>(c) ? sKeys[tid] : 0ULL;
-            sKeys[tid] = WaveBitonicDesc(v);
+            const uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+            uint32_t rank = 0;
+            for (int j = 0; j < static_cast<int>(c); ++j) {
+                const unsigned long long o =
+                    (static_cast<unsigned long long>(ReadLane(hi, j)) << 32) | ReadLane(lo, j);
+                rank += o > v ? 1u : 0u;
+            }
+            if (tid < static_cast<int>(c))
+                sKeys[rank] = v;
         }
         __syncthreads();
     } else if (c <= static_cast<uint32_t>(kSortCap)) {
