@@ -39,6 +39,8 @@ __device__ __forceinline__ uint32_t ReadLane(uint32_t v, int lane)
 // DPP row operations and two lane reads, instead of six dependent LDS permutes.
 extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(unsigned long long);
 extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_or_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_add_f32(float);
 
 __device__ __forceinline__ uint32_t WaveSum(uint32_t v)
 {
@@ -907,19 +909,17 @@ struct ThresholdPredictor {
         // Hidden unit i of the first layer, broadcast from lane i (a scalar read per unit).
         const uint32_t h1 = __float_as_uint(fmaxf(z1, 0.0F));
         const auto H = [&](int i) { return __uint_as_float(ReadLane(h1, i)); };
-        float z2 = b2;
+        // Four independent partial sums (a shorter dependent chain), then the wave sum.
+        float z2a = b2, z2b = 0.0F, z2c = 0.0F, z2d = 0.0F;
 #pragma unroll
         for (int v = 0; v < kHidden / 4; ++v) {
-            z2 = fmaf(w[v].x, H(4 * v), z2);
-            z2 = fmaf(w[v].y, H(4 * v + 1), z2);
-            z2 = fmaf(w[v].z, H(4 * v + 2), z2);
-            z2 = fmaf(w[v].w, H(4 * v + 3), z2);
+            z2a = fmaf(w[v].x, H(4 * v), z2a);
+            z2b = fmaf(w[v].y, H(4 * v + 1), z2b);
+            z2c = fmaf(w[v].z, H(4 * v + 2), z2c);
+            z2d = fmaf(w[v].w, H(4 * v + 3), z2d);
         }
-        float y = w3 * fmaxf(z2, 0.0F);
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1)
-            y += __shfl_xor(y, o);
-        y += b3;
+        const float z2 = (z2a + z2b) + (z2c + z2d);
+        const float y = __ockl_wfred_add_f32(w3 * fmaxf(z2, 0.0F)) + b3;
         float thr = 0.0F;
         if (ub > 0.0F && k <= mv.liveMaxK) {
             thr = fminf(fmaxf(mv.safety * ub * expf(y * mv.yScale + mv.yMean), lower), ub);
@@ -936,6 +936,7 @@ __device__ __forceinline__ unsigned long long WaveOr64(unsigned long long v)
 }
 
 constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
+constexpr int kPruneGroup = 4; // planning waves per workgroup
 
 // Planning kernel of threshold queries (the learned path; no reference counterpart). Wave b of
 // the first `waves` takes the items [items * b / waves, items * (b + 1) / waves) of the
@@ -955,9 +956,13 @@ constexpr int kPruneSpan = 64; // granules per planning item (one per lane)
 // header counts. nqKnown: the batch's query count when the launch knows it (-1: read it from
 // the packet, as a captured graph does), which saves a dependent load.
 template <int B>
-__global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t waves,
-                                                int32_t nqKnown)
+__global__ __launch_bounds__(kPruneGroup * kWave) void KPrune(IndexView ix, BatchView bt, WorkView wk, int32_t blocks,
+                                                              int32_t waves, int32_t nqKnown)
 {
+    // kPruneGroup independent waves per workgroup (fewer workgroups to dispatch); wave w.
+    const int32_t w = static_cast<int32_t>(blockIdx.x) * kPruneGroup + static_cast<int32_t>(threadIdx.x >> 6);
+    if (w > waves)
+        return;
     constexpr int kGpb = B / kGranule;
     constexpr int kBpc = kPruneSpan / kGpb; // blocks per item
     constexpr int C = kPruneItems;
@@ -966,7 +971,7 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
     static_assert(C * kBpc <= kWave, "one lane per block of a chunk");
     const uint32_t lane = __lane_id();
     const int32_t nq = nqKnown >= 0 ? nqKnown : bt.header[0];
-    if (static_cast<int32_t>(blockIdx.x) == waves) {
+    if (w == waves) {
         if (bt.header[3] == 0)
             return;
         // Learned batch: the wave plan of the plain kernel over the queries routed to the plain path.
@@ -1000,8 +1005,8 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
     const int64_t granules = ix.granules;
     const uint32_t cpq = static_cast<uint32_t>((granules + kPruneSpan - 1) / kPruneSpan);
     const uint32_t items = static_cast<uint32_t>(nq) * cpq;
-    const uint32_t i0 = static_cast<uint32_t>(static_cast<uint64_t>(items) * blockIdx.x / waves);
-    const uint32_t i1 = static_cast<uint32_t>(static_cast<uint64_t>(items) * (blockIdx.x + 1) / waves);
+    const uint32_t i0 = static_cast<uint32_t>(static_cast<uint64_t>(items) * static_cast<uint32_t>(w) / waves);
+    const uint32_t i1 = static_cast<uint32_t>(static_cast<uint64_t>(items) * static_cast<uint32_t>(w + 1) / waves);
     int qCur = -1, n = 0;
     int64_t wq0 = 0; // the query's first word in the batch
     float thrLive = 0.0F;
@@ -1159,8 +1164,8 @@ __global__ __launch_bounds__(kWave) void KPrune(IndexView ix, BatchView bt, Work
         it += nIt;
     }
     if (lane == 0) {
-        wk.dead[blockIdx.x] = dead;
-        wk.live[blockIdx.x] = live;
+        wk.dead[w] = dead;
+        wk.live[w] = live;
     }
 }
 
@@ -1902,11 +1907,7 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
             for (int u = 0; u < 8; ++u)
                 acc = stat >= kStatMax ? max(acc, v[u]) : acc + v[u];
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long y = __shfl_xor(acc, o);
-            acc = stat >= kStatMax ? max(acc, y) : acc + y;
-        }
+        acc = stat >= kStatMax ? __ockl_wfred_max_u64(acc) : __ockl_wfred_add_u64(acc); // stat: one per wave
         if ((tid & 63) == 0)
             hostStats[static_cast<int64_t>(q) * kStats + stat] = acc;
     }
@@ -2069,16 +2070,17 @@ bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const 
 bool LaunchPrune(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t waves, int32_t nqKnown, hipStream_t stream)
 {
-    // waves item waves and the learned batches' header wave.
+    // waves item waves and the learned batches' header wave, kPruneGroup waves per workgroup.
+    const dim3 groups((waves + 1 + kPruneGroup - 1) / kPruneGroup), threads(kPruneGroup * kWave);
     switch (blockDocs) {
     case 2048:
-        hipLaunchKernelGGL(KPrune<2048>, dim3(waves + 1), dim3(kWave), 0, stream, ix, bt, wk, blocks, waves, nqKnown);
+        hipLaunchKernelGGL(KPrune<2048>, groups, threads, 0, stream, ix, bt, wk, blocks, waves, nqKnown);
         return true;
     case 4096:
-        hipLaunchKernelGGL(KPrune<4096>, dim3(waves + 1), dim3(kWave), 0, stream, ix, bt, wk, blocks, waves, nqKnown);
+        hipLaunchKernelGGL(KPrune<4096>, groups, threads, 0, stream, ix, bt, wk, blocks, waves, nqKnown);
         return true;
     case 8192:
-        hipLaunchKernelGGL(KPrune<8192>, dim3(waves + 1), dim3(kWave), 0, stream, ix, bt, wk, blocks, waves, nqKnown);
+        hipLaunchKernelGGL(KPrune<8192>, groups, threads, 0, stream, ix, bt, wk, blocks, waves, nqKnown);
         return true;
     default:
         return false;
@@ -2110,20 +2112,20 @@ int32_t PruneWaves(int64_t items)
     return static_cast<int32_t>(std::min<int64_t>(kMaxPruneWaves, std::max<int64_t>(256, (w + 255) / 256 * 256)));
 }
 
-// Resident planning-kernel waves per CU, or 0.
+// Resident planning-kernel workgroups (kPruneGroup waves each) per CU, or 0.
 int PruneBlocksPerCu(int blockDocs)
 {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (blockDocs) {
     case 2048:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<2048>, kWave, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<2048>, kPruneGroup * kWave, 0);
         break;
     case 4096:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<4096>, kWave, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<4096>, kPruneGroup * kWave, 0);
         break;
     case 8192:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<8192>, kWave, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, KPrune<8192>, kPruneGroup * kWave, 0);
         break;
     default:
         break;
