@@ -64,6 +64,13 @@ __host__ __device__ constexpr int32_t BlockKeepCap(int32_t k, int32_t blockDocs,
     return c < blockDocs ? c : blockDocs;
 }
 
+// Row stride of the per-task candidate counts: blocks rounded up to 8, so that each row starts on
+// 32 bytes and the final kernel loads 8 counts as two 16-byte words.
+__host__ __device__ constexpr int64_t BlockCountStride(int32_t blocks)
+{
+    return (static_cast<int64_t>(blocks) + 7) & ~int64_t(7);
+}
+
 // Stats slots (unsigned long long).
 enum {
     kStatPostings = 0, // postings streamed (including masked window postings of short lists)
@@ -143,7 +150,10 @@ struct ModelView {
 // Per-Search device scratch and results.
 struct WorkView {
     unsigned long long *pool; // composite keys (score bits << 32 | ~doc)
-    uint32_t *count;          // [maxQ] pool fill
+    uint32_t *count;          // [maxQ] pool fill of threshold queries (live path)
+    // Per plain task (q * BlockCountStride(blocks) + blk): the candidates it appended to its pool
+    // region, entries [blk * BlockKeepCap, ...) of the query's pool. Every plain task writes it.
+    uint32_t *blockCount;     // [maxQ * BlockCountStride(blocks)]
     uint32_t *hist;           // [maxQ * kHistBins] histogram of kept candidates' scores
     uint32_t *tau;            // [maxQ] running lower bound of the K-th score (bits)
     unsigned long long *stats;// [grid][kStats]: one counter row per scoring wave

@@ -412,11 +412,43 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // l*kHistPerLane ...): k of them at or above a bucket's edge bound the K-th score from below
     // as well. The bound is taken each time the count has doubled (a few times per query) and
     // published through tau, so that every wave of the query starts its selections from it.
-    uint32_t ownHist[kHistPerLane];
+    // pendHist holds the appends not yet added to the query's histogram in global memory: they
+    // are added at every append until the wave holds 2K of the query's candidates, then when the
+    // bound is taken and when the wave leaves the query, not per task (global atomics per task
+    // stall the wave's next posting loads; measured on the 10M-doc Q8 x 3-term rotation, 102 ->
+    // 85 us without them).
+    uint32_t ownHist[kHistPerLane], pendHist[kHistPerLane];
 #pragma unroll
     for (int i = 0; i < kHistPerLane; ++i)
-        ownHist[i] = 0;
+        ownHist[i] = pendHist[i] = 0;
     uint32_t ownCount = 0, ownAt = 0;
+    auto flushHist = [&](int qq) {
+        uint32_t *histw = wk.hist + static_cast<int64_t>(qq) * kHistBins;
+#pragma unroll
+        for (int r = 0; r < kHistPerLane; ++r) {
+            if (pendHist[r])
+                atomicAdd(histw + lane * kHistPerLane + r, pendHist[r]);
+            pendHist[r] = 0;
+        }
+    };
+    // Candidates of a plain query go to this wave's own part of the query's pool, with no
+    // atomic: the regions of the wave's blocks of the query, entries [segBlock * cap, (last block
+    // + 1) * cap) (cap = BlockKeepCap(K) per block), filled contiguously from the first (ownCount
+    // of them). The final kernel reads per-block counts: every plain task writes 0 for its block,
+    // and when the wave leaves the query it writes the counts that cover its entries, region by
+    // region (cap for each full region, then the rest), so that the final kernel's loads of a
+    // wave's entries are contiguous.
+    int32_t segBlock = 0;
+    uint32_t capQ = 1;
+    auto setCount = [&](int qq, int bb, uint32_t v) {
+        if (lane == 0)
+            wk.blockCount[static_cast<int64_t>(qq) * BlockCountStride(blocks) + bb] = v;
+    };
+    auto finishSegment = [&](int qq) {
+        if (lane == 0)
+            for (uint32_t done = 0, bb = static_cast<uint32_t>(segBlock); done < ownCount; done += capQ, ++bb)
+                wk.blockCount[static_cast<int64_t>(qq) * BlockCountStride(blocks) + bb] = min(capQ, ownCount - done);
+    };
     for (uint32_t t = tBegin; t < tEnd; ++t) {
         BM25MI_PHASE(0);
         const int q = static_cast<int>(t / static_cast<uint32_t>(blocks));
@@ -424,7 +456,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         const int32_t base = blk * B;
         const int32_t end = min(base + B, ix.ndocs);
         if (q != qCur) {
+            if (qCur >= 0) {
+                flushHist(qCur);
+                finishSegment(qCur);
+            }
             qCur = q;
+            segBlock = blk;
             const int64_t w0 = bt.qoff[q];
             n = static_cast<int>(bt.qoff[q + 1] - w0);
             thr = bt.thr[q];
@@ -442,6 +479,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             for (int i = 0; i < kHistPerLane; ++i)
                 ownHist[i] = 0;
             ownCount = ownAt = 0;
+            capQ = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B, keepCap));
         }
         if (thr > 0.0F) {
             // A threshold query's live blocks are scored by KLive.
@@ -705,6 +743,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         }
         if (WaveSum(len) == 0) {
             ++skipped; // no query word has postings here: every score is 0
+            setCount(q, blk, 0u);
             continue;
         }
         // 2. Stream and add, word by word in query order.
@@ -721,8 +760,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         if (readHist)
             histBound = max(histBound, WaveHistBoundRegs(hregs, k));
         const uint32_t floor = max(max(tauLoaded, tau0q), histBound);
-        if (top < floor)
+        if (top < floor) {
+            setCount(q, blk, 0u);
             continue;
+        }
         const uint4 *bits4 = reinterpret_cast<const uint4 *>(acc);
         uint32_t mine = 0, mx = 0;
 #pragma unroll 2
@@ -733,12 +774,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             mx = max(mx, max(max(v.x, v.y), max(v.z, v.w)));
         }
         uint32_t kept = WaveSum(mine);
-        if (kept == 0)
+        if (kept == 0) {
+            setCount(q, blk, 0u);
             continue;
+        }
         BM25MI_PHASE(3);
-        // Cut (rare), then append the kept docs to the pool as composite keys (score bits << 32 |
-        // ~doc) and count them in the histogram: one global atomic reserves the pool slots,
-        // one per non-empty bucket counts them, and the cut raises tau.
+        // Cut (rare), then append the kept docs to the task's region of the pool as composite keys
+        // (score bits << 32 | ~doc) and count them in this wave's histogram; the cut raises tau.
         LdsU32 *bitsL = (LdsU32 *)bits4; // acc is in LDS
         uint32_t tb = floor, te = 0, tauNew = 0;
         const uint32_t cap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B, keepCap));
@@ -752,16 +794,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             histBound = max(histBound, tauNew); // the block holds k docs at or above it
             ++cuts;
         }
-        uint32_t slot0 = 0;
-        if (lane == 0)
-            slot0 = atomicAdd(wk.count + q, kept);
+        // The wave's entries fit its regions (at most cap per task) and the pool: they end at most at
+        // (last block) * cap + (its docs) <= ndocs, or (last block + 1) * cap <= blocks * cap.
+        const uint32_t slot0 = static_cast<uint32_t>(segBlock) * cap + ownCount;
+        setCount(q, blk, 0u);
         uint32_t before = mine;
         before = WaveInclusiveSum(before);
         before -= mine;
 #pragma unroll
         for (int r = 0; r < kHistPerLane; ++r)
             sHist[lane * kHistPerLane + r] = 0;
-        slot0 = ReadLane(slot0, 0);
         unsigned long long *pool = wk.pool + poolBase;
         const int64_t capq = poolCap;
         if (mine) {
@@ -786,18 +828,18 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 }
             }
         }
-        uint32_t *histw = wk.hist + static_cast<int64_t>(q) * kHistBins;
 #pragma unroll
         for (int r = 0; r < kHistPerLane; ++r) {
-            const uint32_t bin = lane * kHistPerLane + r;
-            const uint32_t c = sHist[bin];
+            const uint32_t c = sHist[lane * kHistPerLane + r];
             ownHist[r] += c;
-            if (c)
-                atomicAdd(histw + bin, c);
+            pendHist[r] += c;
         }
         ownCount += kept;
+        if (ownCount < 2 * k)
+            flushHist(q); // early appends: they raise the other waves' bounds the most
         if (ownCount >= k && ownCount >= 2 * ownAt) {
             ownAt = ownCount;
+            flushHist(q);
             const uint32_t ob = WaveHistBoundRegs(ownHist, k);
             if (ob > histBound && ob > tauLoaded)
                 tauNew = max(tauNew, ob);
@@ -807,6 +849,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             atomicMax(wk.tau + q, tauNew);
         emitted += kept;
         ++emitting;
+    }
+    if (qCur >= 0) {
+        flushHist(qCur);
+        finishSegment(qCur);
     }
     BM25MI_PHASE(0);
 #undef BM25MI_PHASE
@@ -1843,6 +1889,49 @@ __global__ void KReset(WorkView wk, int32_t maxQ)
         wk.hist[i] = 0;
 }
 
+__device__ __forceinline__ unsigned long long Shfl64(unsigned long long v, uint32_t src)
+{
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v)), static_cast<int>(src)));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(v >> 32)), static_cast<int>(src)));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long Shfl64Xor(unsigned long long v, uint32_t mask)
+{
+    return Shfl64(v, __lane_id() ^ mask);
+}
+
+// One key per lane sorted across the wave, descending (lane 0 the largest): a bitonic sort
+// through lane exchanges. Every lane active.
+__device__ __forceinline__ unsigned long long WaveSortDesc(unsigned long long v)
+{
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t size = 2; size <= kWave; size <<= 1)
+#pragma unroll
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            const unsigned long long o = Shfl64Xor(v, stride);
+            const bool lower = (lane & stride) == 0, desc = (lane & size) == 0;
+            v = (lower == desc) == (o > v) ? o : v; // the larger where lower == desc, else the smaller
+        }
+    return v;
+}
+
+// The 64 largest of two descending wave-sorted lists, descending: max(a[l], b[63 - l]) is a
+// bitonic sequence holding them, then a bitonic merge.
+__device__ __forceinline__ unsigned long long WaveMergeTop(unsigned long long a, unsigned long long b)
+{
+    const uint32_t lane = __lane_id();
+    const unsigned long long r = Shfl64(b, kWave - 1 - lane);
+    unsigned long long v = a > r ? a : r;
+#pragma unroll
+    for (uint32_t stride = kWave / 2; stride > 0; stride >>= 1) {
+        const unsigned long long o = Shfl64Xor(v, stride);
+        v = ((lane & stride) == 0) == (o > v) ? o : v;
+    }
+    return v;
+}
+
 // The k-th largest of p unique keys in global memory (p > k), by a 64-step bisection
 // with block-wide counts. All threads call; every thread gets the result.
 __device__ unsigned long long KthFromGlobal(const unsigned long long *keys, uint32_t p, uint32_t k, uint32_t *red)
@@ -1859,17 +1948,99 @@ __device__ unsigned long long KthFromGlobal(const unsigned long long *keys, uint
     return kappa;
 }
 
+__device__ __forceinline__ uint32_t ShflU32(uint32_t v, uint32_t src)
+{
+    return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(src)));
+}
+
+// Visits the candidates of a plain query, gathered from the per-task regions of its pool: block
+// b's region holds counts[b] keys from entry b * cap. Wave w of the workgroup takes the groups of
+// 64 blocks w, w + 16, ..., kGatherGroups at a time: their counts are loaded together, then each
+// lane finds the block of its entry by a binary search over the group's count prefix (six lane
+// reads) and every group's first two rounds of 64 entries are loaded before any is visited
+// (further rounds of a group with more than 128 entries follow). visit(key, valid) is called by
+// every lane of the wave, also for lanes without an entry (valid false).
+constexpr int kGatherGroups = 5;
+// The final kernel's gather of a plain query: blocks per window (8 per thread; their counts and
+// the threads' prefix in LDS) and entries per thread per pass.
+constexpr int kGatherWin = 8 * kFinalThreads;
+constexpr int kGatherPerThread = 8;
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu; // rounds of 64 entries per group the final kernel loads at once
+template <class Visit>
+__device__ __forceinline__ void GatherBlocks(const unsigned long long *pool, const uint32_t *counts, int32_t blocks,
+                                             uint32_t cap, Visit &&visit)
+{
+    constexpr int kWaves = kFinalThreads / kWave;
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const int groups = (blocks + kWave - 1) / kWave;
+    // The key of entry i of a group (incl: the group's inclusive count prefix, lane l holding
+    // block g * 64 + l), loaded when i < total. The block is found by a binary search over the
+    // prefix with the whole wave active (i clamped into the group), so that every lane read
+    // reads an active lane.
+    auto entry = [&](uint32_t incl, uint32_t c, int g, uint32_t total, uint32_t i) -> unsigned long long {
+        const uint32_t ic = min(i, total - 1u);
+        uint32_t pos = 0; // lanes whose inclusive count prefix is <= ic
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1)
+            pos += ShflU32(incl, pos + step - 1) <= ic ? step : 0u;
+        const uint32_t start = ShflU32(incl - c, pos);
+        const int64_t b = static_cast<int64_t>(g) * kWave + pos;
+        return i < total ? pool[b * cap + (ic - start)] : 0ULL;
+    };
+    for (int g0 = wave; g0 < groups; g0 += kWaves * kGatherGroups) {
+        uint32_t c[kGatherGroups], incl[kGatherGroups], total[kGatherGroups];
+#pragma unroll
+        for (int j = 0; j < kGatherGroups; ++j) {
+            const int b = (g0 + j * kWaves) * kWave + static_cast<int>(lane);
+            c[j] = b < blocks ? counts[b] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kGatherGroups; ++j) {
+            incl[j] = WaveInclusiveSum(c[j]);
+            total[j] = ReadLane(incl[j], kWave - 1);
+        }
+        unsigned long long key[kGatherGroups][2];
+#pragma unroll
+        for (int j = 0; j < kGatherGroups; ++j)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                key[j][r] = 0ULL;
+                if (static_cast<uint32_t>(r * kWave) < total[j]) // uniform
+                    key[j][r] = entry(incl[j], c[j], g0 + j * kWaves, total[j], r * kWave + lane);
+            }
+#pragma unroll
+        for (int j = 0; j < kGatherGroups; ++j)
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                visit(key[j][r], r * kWave + lane < total[j]);
+#pragma unroll
+        for (int j = 0; j < kGatherGroups; ++j)
+            for (uint32_t r0 = 2 * kWave; r0 < total[j]; r0 += kWave) {
+                const unsigned long long v = entry(incl[j], c[j], g0 + j * kWaves, total[j], r0 + lane);
+                visit(v, r0 + lane < total[j]);
+            }
+    }
+}
+
 // Per query: the exact top-K of the candidate pool. Entries scoring below the query's
 // final lower bound tau cannot be in the top-K and are dropped; the rest are sorted in
 // LDS (score desc, doc asc) when they fit, else reduced by a 64-step bisection first.
-// Fewer than K positive docs are padded with the lowest-ID zero-score docs.
+// Fewer than K positive docs are padded with the lowest-ID zero-score docs. A plain query's
+// candidates are gathered from its tasks' pool regions (blocks, blockDocs and keepCap give
+// their size, BlockKeepCap); a threshold query's pool is filled from entry 0.
 __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView wk, unsigned long long *hostStats,
-                                                         int32_t scoreGrid)
+                                                         int32_t scoreGrid, int32_t blocks, int32_t blockDocs,
+                                                         int32_t keepCap)
 {
     __shared__ unsigned long long sKeys[kSortCap];
     __shared__ unsigned long long sIds[1024];
     __shared__ uint32_t sRed[kFinalThreads / 64];
     __shared__ uint32_t sN;
+    // Plain query, per window of kGatherWin blocks: the prefix of each thread's 8 block counts,
+    // and the counts (16-bit, 8 per thread).
+    __shared__ uint32_t sGrp[kFinalThreads];
+    __shared__ __align__(16) uint16_t sCnt[kGatherWin];
     const int q = blockIdx.x;
     const int tid = threadIdx.x;
     // The query's state, loaded with the query count and before the counter rows so that all
@@ -1886,36 +2057,85 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     for (int i = 0; i < kHistPerLane; ++i)
         hregs[i] = tid < kWave ? wk.hist[static_cast<int64_t>(q) * kHistBins + tid * kHistPerLane + i] : 0u;
     const bool active = q < nq;
+    // A plain query's block counts of the gather's first window (thread t: blocks 8t .. 8t+7 of
+    // the window), loaded early. The count row is padded to a multiple of 8 blocks, so that the
+    // two 16-byte loads stay within it; padding entries count 0 (blocks past the query's).
+    const uint32_t *counts = wk.blockCount + static_cast<int64_t>(q) * BlockCountStride(blocks);
+    const bool plainActive = active && !(thrQ > 0.0F);
+    auto loadCounts = [&](int w0, uint4 (&c8)[2]) {
+        const int b = w0 + 8 * tid;
+        const uint4 *p = reinterpret_cast<const uint4 *>(counts + b);
+        c8[0] = c8[1] = make_uint4(0u, 0u, 0u, 0u);
+        if (plainActive && b < blocks) {
+            c8[0] = p[0];
+            c8[1] = p[1];
+            // Entries past the last block (row padding) are not written by the scoring kernel.
+            const int n = blocks - b; // >= 1
+            c8[0].y = n > 1 ? c8[0].y : 0u;
+            c8[0].z = n > 2 ? c8[0].z : 0u;
+            c8[0].w = n > 3 ? c8[0].w : 0u;
+            c8[1].x = n > 4 ? c8[1].x : 0u;
+            c8[1].y = n > 5 ? c8[1].y : 0u;
+            c8[1].z = n > 6 ? c8[1].z : 0u;
+            c8[1].w = n > 7 ? c8[1].w : 0u;
+        }
+    };
+    uint4 cg0[2];
+    bool prefetched = false;
+    auto prefetchCounts = [&]() {
+        prefetched = true;
+        loadCounts(0, cg0);
+    };
     {
-        // The scoring kernel has finished: this workgroup sums its share of the per-wave
-        // counter rows (64 threads per counter; counters 12..15 take the maximum) into its
-        // row of hostStats, which the host adds up.
-        static_assert(kFinalThreads == 64 * kStats, "one wave per counter");
-        const int stat = tid >> 6;
+        // The scoring kernel has finished: this workgroup sums its share of the per-wave counter
+        // rows (counters 12..15 take the maximum) into its row of hostStats, which the host adds
+        // up. The rows are read as contiguous words (thread t: words t, t + 1024, ... of the
+        // share, all of counter t % 16), eight per thread in flight; a plain query's first block
+        // counts are loaded in the same round trip.
+        static_assert(kFinalThreads % kStats == 0 && kStats == 16, "a thread sums one counter");
+        const int stat = tid & (kStats - 1);
         const int per = (scoreGrid + gridDim.x - 1) / gridDim.x;
         const int r0 = q * per, r1 = min(scoreGrid, r0 + per);
+        const int64_t w0 = static_cast<int64_t>(r0) * kStats, w1 = static_cast<int64_t>(max(r0, r1)) * kStats;
         unsigned long long acc = 0;
-        // Eight rows per lane in flight at a time.
-        for (int r8 = r0 + (tid & 63); r8 < r1; r8 += 64 * 8) {
+        for (int64_t wb = w0 + tid; wb < w1 || wb == w0 + tid; wb += 8 * kFinalThreads) {
             unsigned long long v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int r = r8 + 64 * u;
-                v[u] = r < r1 ? wk.stats[static_cast<int64_t>(r) * kStats + stat] : 0ULL;
+                const int64_t w = wb + u * kFinalThreads;
+                v[u] = w < w1 ? wk.stats[w] : 0ULL;
             }
+            if (wb == w0 + tid)
+                prefetchCounts();
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                acc = stat >= kStatMax ? max(acc, v[u]) : acc + v[u];
+                acc = stat >= kStatMax ? (v[u] > acc ? v[u] : acc) : acc + v[u];
         }
-        acc = stat >= kStatMax ? __ockl_wfred_max_u64(acc) : __ockl_wfred_add_u64(acc); // stat: one per wave
-        if ((tid & 63) == 0)
-            hostStats[static_cast<int64_t>(q) * kStats + stat] = acc;
+        // Lanes l, l^16, l^32, l^48 hold counter l % 16; then the 16 waves through LDS.
+#pragma unroll
+        for (uint32_t m = 16; m < kWave; m <<= 1) {
+            const unsigned long long o = Shfl64Xor(acc, m);
+            acc = stat >= kStatMax ? (o > acc ? o : acc) : acc + o;
+        }
+        if ((tid & 63) < kStats)
+            sIds[(tid >> 6) * kStats + stat] = acc;
+        __syncthreads();
+        if (tid < kStats) {
+            unsigned long long t = 0;
+            for (int w = 0; w < kFinalThreads / kWave; ++w) {
+                const unsigned long long o = sIds[w * kStats + tid];
+                t = stat >= kStatMax ? (o > t ? o : t) : t + o;
+            }
+            hostStats[static_cast<int64_t>(q) * kStats + tid] = t;
+        }
+        __syncthreads(); // sIds is used again below
     }
     if (!active)
         return;
     const int64_t cap = poolHi - poolLo;
     const uint32_t p = static_cast<uint32_t>(min(static_cast<int64_t>(countQ), cap));
     const bool overflowed = static_cast<int64_t>(countQ) > cap; // only threshold queries (rerun)
+    const uint32_t blockCap = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), blockDocs, keepCap));
     uint32_t *hist = wk.hist + static_cast<int64_t>(q) * kHistBins;
     if (tid < 64) {
         const uint32_t hb = WaveHistBoundRegs(hregs, k);
@@ -1938,12 +2158,99 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
     for (int i = tid; i < kHistBins; i += kFinalThreads)
         hist[i] = 0;
     // Keep entries with score bits >= tau.
-    for (uint32_t i0 = 0; i0 < p; i0 += kFinalThreads) {
-        const uint32_t i = i0 + tid;
-        const unsigned long long key = i < p ? pool[i] : 0ULL;
-        const uint32_t at = WaveAppend(i < p && static_cast<uint32_t>(key >> 32) >= tau, &sN);
-        if (at < static_cast<uint32_t>(kSortCap))
-            sKeys[at] = key;
+    if (pruned) {
+        for (uint32_t i0 = 0; i0 < p; i0 += kFinalThreads) {
+            const uint32_t i = i0 + tid;
+            const unsigned long long key = i < p ? pool[i] : 0ULL;
+            const uint32_t at = WaveAppend(i < p && static_cast<uint32_t>(key >> 32) >= tau, &sN);
+            if (at < static_cast<uint32_t>(kSortCap))
+                sKeys[at] = key;
+        }
+    } else {
+        // Plain query, in windows of kGatherWin blocks: thread t holds the counts of the window's
+        // blocks 8t .. 8t+7 (two 16-byte loads, made with the counter rows' loads for the first
+        // window); one workgroup-wide prefix over the threads' totals goes to LDS with the counts
+        // (16-bit, 8 per thread); then thread t takes the window's entries t, t + 1024, ...
+        // (kGatherPerThread per pass, all loads in flight together), finds each entry's thread
+        // by a binary search over the prefix and its block among that thread's 8 counts, and the
+        // survivors (score bits >= tau) are placed with one wave-wide prefix.
+        if (!prefetched)
+            prefetchCounts();
+        for (int w0 = 0; w0 < blocks; w0 += kGatherWin) {
+            uint4 c8[2];
+            if (w0 == 0) {
+                c8[0] = cg0[0];
+                c8[1] = cg0[1];
+            } else {
+                loadCounts(w0, c8);
+            }
+            const uint32_t mineCount = c8[0].x + c8[0].y + c8[0].z + c8[0].w + c8[1].x + c8[1].y + c8[1].z + c8[1].w;
+            uint32_t total = 0;
+            const uint32_t ex = BlockExclusive<kFinalThreads>(mineCount, sRed, &total);
+            sGrp[tid] = ex;
+            reinterpret_cast<uint4 *>(sCnt)[tid] =
+                make_uint4(c8[0].x | c8[0].y << 16, c8[0].z | c8[0].w << 16, c8[1].x | c8[1].y << 16,
+                           c8[1].z | c8[1].w << 16);
+            __syncthreads();
+            for (uint32_t e0 = 0; e0 < total; e0 += kFinalThreads * kGatherPerThread) {
+                // The last thread whose prefix is <= i holds entry i (threads without entries
+                // share their successor's prefix), then its block among that thread's 8. The
+                // searches of a thread's entries advance together, step by step (index clamped
+                // into the window's entries), so that their LDS reads are in flight together.
+                uint32_t g[kGatherPerThread], ic[kGatherPerThread];
+#pragma unroll
+                for (int m = 0; m < kGatherPerThread; ++m) {
+                    g[m] = 0;
+                    ic[m] = min(e0 + m * kFinalThreads + tid, total - 1u);
+                }
+#pragma unroll
+                for (uint32_t step = kFinalThreads / 2; step > 0; step >>= 1)
+#pragma unroll
+                    for (int m = 0; m < kGatherPerThread; ++m)
+                        g[m] += sGrp[g[m] + step] <= ic[m] ? step : 0u;
+                uint32_t at[kGatherPerThread];
+#pragma unroll
+                for (int m = 0; m < kGatherPerThread; ++m) {
+                    uint32_t r = ic[m] - sGrp[g[m]];
+                    const uint4 cc = reinterpret_cast<const uint4 *>(sCnt)[g[m]];
+                    const uint32_t cnt[8] = {cc.x & 0xFFFFu, cc.x >> 16, cc.y & 0xFFFFu, cc.y >> 16,
+                                             cc.z & 0xFFFFu, cc.z >> 16, cc.w & 0xFFFFu, cc.w >> 16};
+                    uint32_t blk = 0;
+                    bool found = false;
+#pragma unroll
+                    for (int jb = 0; jb < 8; ++jb) {
+                        const bool here = !found && r < cnt[jb];
+                        blk = here ? static_cast<uint32_t>(jb) : blk;
+                        r = !found && !here ? r - cnt[jb] : r;
+                        found = found || here;
+                    }
+                    at[m] = e0 + m * kFinalThreads + tid < total
+                                ? (static_cast<uint32_t>(w0) + 8 * g[m] + blk) * blockCap + r
+                                : kNoEntry;
+                }
+                unsigned long long key[kGatherPerThread];
+#pragma unroll
+                for (int m = 0; m < kGatherPerThread; ++m)
+                    key[m] = at[m] != kNoEntry ? pool[at[m]] : 0ULL;
+                uint32_t mine = 0;
+#pragma unroll
+                for (int m = 0; m < kGatherPerThread; ++m)
+                    mine += at[m] != kNoEntry && static_cast<uint32_t>(key[m] >> 32) >= tau ? 1u : 0u;
+                const uint32_t inclMine = WaveInclusiveSum(mine);
+                uint32_t slot = 0;
+                if ((tid & 63) == kWave - 1 && inclMine)
+                    slot = atomicAdd(&sN, inclMine);
+                slot = ReadLane(slot, kWave - 1) + inclMine - mine;
+#pragma unroll
+                for (int m = 0; m < kGatherPerThread; ++m)
+                    if (at[m] != kNoEntry && static_cast<uint32_t>(key[m] >> 32) >= tau) {
+                        if (slot < static_cast<uint32_t>(kSortCap))
+                            sKeys[slot] = key[m];
+                        ++slot;
+                    }
+            }
+            __syncthreads(); // sGrp and sCnt are rewritten by the next window
+        }
     }
     __syncthreads();
     const uint32_t c = sN;
@@ -1971,6 +2278,32 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
                 sKeys[rank] = v;
         }
         __syncthreads();
+    } else if (c <= static_cast<uint32_t>(kSortCap) && k <= static_cast<uint32_t>(kWave)) {
+        // More than 64 survivors, K <= 64: only the top K are needed. Each wave keeps the 64
+        // largest keys of its share (every 1024th chunk of 64) sorted in its registers, merging
+        // each chunk with a register sort (no workgroup barriers), writes its top K, and wave 0
+        // merges the 16 lists the same way.
+        const int wave = tid >> 6;
+        unsigned long long best = 0;
+        for (uint32_t i0 = static_cast<uint32_t>(wave) * kWave; i0 < c; i0 += kFinalThreads) {
+            const uint32_t i = i0 + (tid & 63);
+            best = WaveMergeTop(best, WaveSortDesc(i < c ? sKeys[i] : 0ULL));
+        }
+        __syncthreads(); // every wave has read its share of sKeys
+        if ((tid & 63) < static_cast<int>(k))
+            sIds[wave * k + (tid & 63)] = best;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t n = (kFinalThreads / kWave) * k;
+            best = 0;
+            for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+                const uint32_t i = i0 + tid;
+                best = WaveMergeTop(best, WaveSortDesc(i < n ? sIds[i] : 0ULL));
+            }
+            if (tid < static_cast<int>(k))
+                sKeys[tid] = best;
+        }
+        __syncthreads();
     } else if (c <= static_cast<uint32_t>(kSortCap)) {
         int n2 = 1;
         while (n2 < static_cast<int>(c))
@@ -1981,16 +2314,40 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
         BitonicDesc<kFinalThreads>(sKeys, n2);
     } else {
         // Too many to sort: the K-th key over the whole pool, then the K keys at or above it.
-        const unsigned long long kappa = KthFromGlobal(pool, p, k, sRed);
+        unsigned long long kappa = 0;
+        if (pruned) {
+            kappa = KthFromGlobal(pool, p, k, sRed);
+        } else {
+            // The same 64-step bisection over the gathered regions (rare: more than kSortCap
+            // candidates at or above the final bound, i.e. large K with massive score ties).
+            for (int bit = 63; bit >= 0; --bit) {
+                const unsigned long long cand = kappa | (1ULL << bit);
+                uint32_t n = 0;
+                GatherBlocks(pool, counts, blocks, blockCap,
+                             [&](unsigned long long key, bool valid) { n += valid && key >= cand ? 1u : 0u; });
+                if (BlockSum<kFinalThreads>(n, sRed) >= k)
+                    kappa = cand;
+            }
+        }
         if (tid == 0)
             sN = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < p; i += kFinalThreads)
-            if (pool[i] >= kappa) {
-                const uint32_t at = atomicAdd(&sN, 1u);
-                if (at < k)
-                    sIds[at] = pool[i];
-            }
+        if (pruned) {
+            for (uint32_t i = tid; i < p; i += kFinalThreads)
+                if (pool[i] >= kappa) {
+                    const uint32_t at = atomicAdd(&sN, 1u);
+                    if (at < k)
+                        sIds[at] = pool[i];
+                }
+        } else {
+            GatherBlocks(pool, counts, blocks, blockCap, [&](unsigned long long key, bool valid) {
+                if (valid && key >= kappa) {
+                    const uint32_t at = atomicAdd(&sN, 1u);
+                    if (at < k)
+                        sIds[at] = key;
+                }
+            });
+        }
         __syncthreads();
         int n2 = 1;
         while (n2 < static_cast<int>(k))
@@ -2176,9 +2533,10 @@ int ScoreBlocksPerCu(int blockDocs)
 }
 
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t workgroups, unsigned long long *hostStats,
-                 int32_t scoreGrid, hipStream_t stream)
+                 int32_t scoreGrid, int32_t blocks, int32_t blockDocs, int32_t keepCap, hipStream_t stream)
 {
-    hipLaunchKernelGGL(KFinal, dim3(workgroups), dim3(kFinalThreads), 0, stream, bt, wk, hostStats, scoreGrid);
+    hipLaunchKernelGGL(KFinal, dim3(workgroups), dim3(kFinalThreads), 0, stream, bt, wk, hostStats, scoreGrid, blocks,
+                       blockDocs, keepCap);
 }
 
 } // namespace bm25mi

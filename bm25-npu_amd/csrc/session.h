@@ -51,6 +51,7 @@ struct Query {
 // counters, result block, stream and captured graph.
 struct Slot {
     void *dPacket = nullptr, *dPool = nullptr, *dCount = nullptr, *dTau = nullptr, *dStats = nullptr;
+    void *dBlockCount = nullptr;
     void *dHist = nullptr, *dItems = nullptr, *dDead = nullptr, *dCursor = nullptr, *dLive = nullptr;
     void *hPacket = nullptr, *hPacketDev = nullptr, *hResult = nullptr, *hResultDev = nullptr;
     StageRanges ranges{}; // the packet ranges the staging kernel copies for the packed batch
@@ -215,7 +216,7 @@ int32_t PruneWaves(int64_t items);
 int ScoreBlocksPerCu(int blockDocs);
 // One workgroup per query slot (workgroups beyond the batch's queries only sum counter rows).
 void LaunchFinal(const BatchView &bt, const WorkView &wk, int32_t workgroups, unsigned long long *hostStats,
-                 int32_t scoreGrid, hipStream_t stream);
+                 int32_t scoreGrid, int32_t blocks, int32_t blockDocs, int32_t keepCap, hipStream_t stream);
 
 } // namespace bm25mi
 

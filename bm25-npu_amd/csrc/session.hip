@@ -206,6 +206,8 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     Require(blockDocs == 2048 || blockDocs == 4096 || blockDocs == 8192, "block_docs must be 2048, 4096 or 8192");
     ValidateIndex(offsets, v, docs, tf, m, norm, n, idf, 0);
     Require(m >= 1, "Index needs at least one posting");
+    // Posting indices are 32-bit on the device.
+    Require(m < 0xFFFFFFFFLL, "Index has too many postings (at most 2^32 - 2)");
     Require(cfg.max_k <= n, "max_k exceeds the document count");
     device = cfg.device_id;
 
@@ -269,7 +271,7 @@ Session::Session(const int64_t *offsets, int64_t v, const int32_t *docs, const f
     const uint64_t known = static_cast<uint64_t>(m) * 8 + static_cast<uint64_t>(m) / 4 + 4 +
                            layout.dir.size() * 4 + layout.blockMax.size() * 4 +
                            static_cast<uint64_t>(nslots) * static_cast<uint64_t>(poolEntries) * 8 +
-                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks +
+                           static_cast<uint64_t>(nslots) * static_cast<uint64_t>(cfg.max_queries) * blocks * 5 +
                            static_cast<uint64_t>(nslots) * static_cast<uint64_t>(std::max<int64_t>(1, cfg.max_words)) * blocks * 4;
     // MemoryGate of large_search.cpp:87-93: known bytes must fit 80% of free HBM.
     Require(known <= static_cast<uint64_t>(freeBytes) / 5 * 4, "Index, pools and arenas exceed 80% of free HBM");
@@ -370,6 +372,7 @@ void Session::InitSlot(Slot &slot)
     BM25MI_HIP(hipHostGetDevicePointer(&slot.hResultDev, slot.hResult, 0), "map pinned result");
     slot.dPool = Alloc<unsigned long long>(static_cast<size_t>(poolEntries), "alloc candidate pools");
     slot.dCount = Alloc<uint32_t>(q, "alloc pool counts");
+    slot.dBlockCount = Alloc<uint32_t>(q * static_cast<size_t>(BlockCountStride(blocks)), "alloc task candidate counts");
     slot.dHist = Alloc<uint32_t>(q * kHistBins, "alloc score histograms");
     slot.dTau = Alloc<uint32_t>(q, "alloc thresholds");
     slot.dStats = Alloc<unsigned long long>(2 * static_cast<size_t>(gridCapacity) * kStats, "alloc stats");
@@ -383,6 +386,7 @@ void Session::InitSlot(Slot &slot)
     auto *rs = static_cast<unsigned char *>(slot.hResultDev);
     slot.wk = WorkView{static_cast<unsigned long long *>(slot.dPool),
                        static_cast<uint32_t *>(slot.dCount),
+                       static_cast<uint32_t *>(slot.dBlockCount),
                        static_cast<uint32_t *>(slot.dHist),
                        static_cast<uint32_t *>(slot.dTau),
                        static_cast<unsigned long long *>(slot.dStats),
@@ -488,8 +492,8 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
     // to share the counter rows; a captured graph has one per query of the session's capacity.
     // Workgroups beyond the batch's queries only sum counter rows.
     slot.finalGrid = capture ? cfg.max_queries : FinalGrid(slot.nq);
-    LaunchFinal(slot.bt, slot.wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), rows,
-                slot.stream);
+    LaunchFinal(slot.bt, slot.wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), rows, blocks,
+                blockDocs, cfg.keep_cap, slot.stream);
 }
 
 void Session::Capture(Slot &slot)
@@ -794,11 +798,14 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
 {
     Require(outScores && outIds && status, "Null output array");
     Require(!query.thr, "The no-allocation Search takes no thresholds");
-    // Scratch layout of this Search: packet | count | tau | histograms | counter rows | pools.
+    // Scratch layout of this Search: packet | count | task counts | tau | histograms | counter rows
+    // | pools. Every task of a plain query writes its count, so the task counts need no clearing.
     const size_t q = static_cast<size_t>(cfg.max_queries);
     size_t at = Align(packetBytes, 256);
     const size_t offCount = at;
     at = Align(at + q * 4, 256);
+    const size_t offBlockCount = at;
+    at = Align(at + q * static_cast<size_t>(BlockCountStride(blocks)) * 4, 256);
     const size_t offTau = at;
     at = Align(at + q * 4, 256);
     const size_t offHist = at;
@@ -826,6 +833,7 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
     WorkView wk = slot.wk; // status, scores and IDs: the slot's mapped result block
     wk.pool = reinterpret_cast<unsigned long long *>(base + at);
     wk.count = reinterpret_cast<uint32_t *>(base + offCount);
+    wk.blockCount = reinterpret_cast<uint32_t *>(base + offBlockCount);
     wk.tau = reinterpret_cast<uint32_t *>(base + offTau);
     wk.hist = reinterpret_cast<uint32_t *>(base + offHist);
     wk.stats = reinterpret_cast<unsigned long long *>(base + offRows);
@@ -842,7 +850,8 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
         Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream, nullptr, nullptr, 0, cfg.keep_cap),
                 "Unsupported block size");
         slot.finalGrid = FinalGrid(slot.nq);
-        LaunchFinal(bt, wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), grid, slot.stream);
+        LaunchFinal(bt, wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), grid, blocks, blockDocs,
+                    cfg.keep_cap, slot.stream);
         BM25MI_HIP(hipGetLastError(), "launch Search kernels (scratch)");
         Complete(slot, outScores, outIds, status);
         last = slot.last;
@@ -942,8 +951,8 @@ void Session::Release() noexcept
             (void)hipStreamDestroy(slot.stream);
             slot.stream = nullptr;
         }
-        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dTau, &slot.dStats, &slot.dHist, &slot.dItems,
-                         &slot.dDead, &slot.dCursor, &slot.dLive})
+        for (void **p : {&slot.dPacket, &slot.dPool, &slot.dCount, &slot.dBlockCount, &slot.dTau, &slot.dStats,
+                         &slot.dHist, &slot.dItems, &slot.dDead, &slot.dCursor, &slot.dLive})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
