@@ -27,12 +27,11 @@ constexpr float kLiveMinRatio = 0.3F;
 // live granule) measured 0.92-1.0x of the plain path on the 10M-doc corpus.
 constexpr int32_t kLiveMaxK = 64;
 #ifndef BM25MI_COLUMNS
-#define BM25MI_COLUMNS 16
+#define BM25MI_COLUMNS 14
 #endif
-// 64-posting columns a scoring wave keeps in flight per round (16, with the wave reductions on
-// DPP: 0.111 / 0.153 ms end to end on the 10M-doc Q8 x 3-term K10 A/B batches / 8-batch
-// rotation, against 0.114 / 0.154 for 14, 0.114 / 0.158 for 12 and 0.117 / 0.153 for 18), and the
-// live kernel's.
+// 64-posting columns a scoring wave keeps in flight per round (14: 0.1175 ms end to end on the
+// 10M-doc Q8 x 3-term K10 8-batch rotation against 0.1205 for 16, which needs one register more
+// than the 128 of four waves per SIMD and spills it), and the live kernel's.
 constexpr int kColumns = BM25MI_COLUMNS;
 #ifndef BM25MI_LIVE_COLUMNS
 #define BM25MI_LIVE_COLUMNS 12

@@ -440,14 +440,40 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // wave's entries are contiguous.
     int32_t segBlock = 0;
     uint32_t capQ = 1;
+    uint32_t floorSeen = 1; // the largest floor the wave used for the query
     auto setCount = [&](int qq, int bb, uint32_t v) {
         if (lane == 0)
             wk.blockCount[static_cast<int64_t>(qq) * BlockCountStride(blocks) + bb] = v;
     };
+    // Before writing them, the wave drops the entries below its latest floor (kept while its
+    // bounds were still low, e.g. at its first tasks): such a doc scores below a lower bound of the
+    // query's K-th score, so it cannot be in the top K. The survivors are compacted in place
+    // (entry n + j is written after entry n + j was read: n never exceeds the entries read).
     auto finishSegment = [&](int qq) {
+        uint32_t n = 0;
+        if (ownCount > 0) {
+            // The floor: the largest the wave used, including the histogram bound its last task of
+            // the query read.
+            const uint32_t fl = max(floorSeen, histBound);
+            unsigned long long *seg = wk.pool + poolBase + static_cast<int64_t>(segBlock) * capQ;
+            // The entries written (the pool's capacity bounds them; it always holds them all).
+            const uint32_t have = static_cast<uint32_t>(
+                min(static_cast<int64_t>(ownCount), max(poolCap - static_cast<int64_t>(segBlock) * capQ, int64_t(0))));
+            // The wave's lanes read each other's appends.
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            for (uint32_t i0 = 0; i0 < have; i0 += kWave) {
+                const uint32_t i = i0 + lane;
+                const unsigned long long key = i < have ? seg[i] : 0ULL;
+                const bool keep = i < have && static_cast<uint32_t>(key >> 32) >= fl;
+                const unsigned long long m = __ballot(keep);
+                if (keep)
+                    seg[n + static_cast<uint32_t>(__popcll(m & ((1ULL << lane) - 1ULL)))] = key;
+                n += static_cast<uint32_t>(__popcll(m));
+            }
+        }
         if (lane == 0)
-            for (uint32_t done = 0, bb = static_cast<uint32_t>(segBlock); done < ownCount; done += capQ, ++bb)
-                wk.blockCount[static_cast<int64_t>(qq) * BlockCountStride(blocks) + bb] = min(capQ, ownCount - done);
+            for (uint32_t done = 0, bb = static_cast<uint32_t>(segBlock); done < n; done += capQ, ++bb)
+                wk.blockCount[static_cast<int64_t>(qq) * BlockCountStride(blocks) + bb] = min(capQ, n - done);
     };
     for (uint32_t t = tBegin; t < tEnd; ++t) {
         BM25MI_PHASE(0);
@@ -480,6 +506,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 ownHist[i] = 0;
             ownCount = ownAt = 0;
             capQ = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B, keepCap));
+            floorSeen = 1;
         }
         if (thr > 0.0F) {
             // A threshold query's live blocks are scored by KLive.
@@ -497,6 +524,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // append, 84 us every 16th task; reading it after the first round of posting loads at
         // each wave's 4th task halved the candidates but doubled the streaming time).
         const bool readHist = histAge >= (k > kHistLargeK ? kHistEveryLargeK : kHistEvery);
+        // The query's last task in this wave's range reads the histogram again, with device scope,
+        // after its last round of posting loads (so that the read stalls no posting load): the
+        // bound it gives is the floor the wave's kept entries are filtered with when it leaves the
+        // query.
+        const bool lastTask = t + 1 == tEnd || blk + 1 == blocks;
         uint32_t hregs[kHistPerLane];
 #pragma unroll
         for (int i = 0; i < kHistPerLane; ++i)
@@ -510,7 +542,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         }
         ++histAge;
         // The query's tau: every task, loaded (device scope: the raises the other waves
-        // published, not a cached copy) after the task's first round of posting loads, so that a
+        // published, not a cached copy) after the task's last round of posting loads, so that a
         // slow read of this contended line delays only the selection, not the postings.
         uint32_t tauLoaded = 1;
 
@@ -660,9 +692,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                             e[u] = ix.post[ReadLane(segLo, sg) + off + lane];
                     }
                 }
-                if (r0 == 0) {
+                if (r0 + U >= columns) { // the task's last round: nothing waits for these loads before
+                                         // its selection
                     __builtin_amdgcn_sched_barrier(0);
                     tauLoaded = __hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (lastTask) {
+#pragma unroll
+                        for (int i = 0; i < kHistPerLane; ++i)
+                            hregs[i] = __hip_atomic_load(wk.hist + static_cast<int64_t>(q) * kHistBins +
+                                                             lane * kHistPerLane + i,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if (r0 == 0) {
@@ -757,9 +797,10 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // the accumulator row is scanned.
         BM25MI_PHASE(2);
         const uint32_t top = WaveMax(__float_as_uint(runMax));
-        if (readHist)
+        if (readHist || lastTask)
             histBound = max(histBound, WaveHistBoundRegs(hregs, k));
         const uint32_t floor = max(max(tauLoaded, tau0q), histBound);
+        floorSeen = max(floorSeen, floor);
         if (top < floor) {
             setCount(q, blk, 0u);
             continue;
@@ -2293,15 +2334,16 @@ __global__ __launch_bounds__(kFinalThreads) void KFinal(BatchView bt, WorkView w
         if ((tid & 63) < static_cast<int>(k))
             sIds[wave * k + (tid & 63)] = best;
         __syncthreads();
-        if (wave == 0) {
-            const uint32_t n = (kFinalThreads / kWave) * k;
-            best = 0;
-            for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
-                const uint32_t i = i0 + tid;
-                best = WaveMergeTop(best, WaveSortDesc(i < n ? sIds[i] : 0ULL));
-            }
-            if (tid < static_cast<int>(k))
-                sKeys[tid] = best;
+        // The 16 lists (16K keys, zeros past a wave's share): thread i ranks key i by the keys above
+        // it (every thread reads the same key at a time: LDS broadcasts); the top K go to sKeys.
+        const uint32_t n = (kFinalThreads / kWave) * k;
+        if (static_cast<uint32_t>(tid) < n) {
+            const unsigned long long v = sIds[tid];
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < n; ++j)
+                rank += sIds[j] > v ? 1u : 0u;
+            if (rank < k && v != 0ULL)
+                sKeys[rank] = v;
         }
         __syncthreads();
     } else if (c <= static_cast<uint32_t>(kSortCap)) {
