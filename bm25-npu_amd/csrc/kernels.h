@@ -53,6 +53,12 @@ constexpr uint32_t kHistEvery = 16;
 constexpr uint32_t kHistEveryLargeK = 4;
 constexpr uint32_t kHistLargeK = 64;
 constexpr uint32_t kHistBase = (127u - 4u) << 4;
+// The scoring kernel adds a wave's appends to the query's histogram at once when the batch has at
+// most this many scoring waves per query (measured on the 10M-doc corpus, Q64 x 2-12 terms: fewer
+// candidates and a faster kernel than adding them when the wave's own count doubles); with more
+// (Q8: 512 per query), the atomics on a query's histogram lines contend, and a wave adds its
+// appends when its own count has doubled and when it leaves the query.
+constexpr int kEagerWaves = 128;
 
 // Candidates a block may append: all of them when at most this many score at or above the
 // query's floor, else a cut keeping between K and this many. It sizes the candidate pools.

@@ -412,12 +412,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     // l*kHistPerLane ...): k of them at or above a bucket's edge bound the K-th score from below
     // as well. The bound is taken each time the count has doubled (a few times per query) and
     // published through tau, so that every wave of the query starts its selections from it.
-    // pendHist holds the appends not yet added to the query's histogram in global memory: they
-    // are added at every append until the wave holds 2K of the query's candidates, then when the
-    // bound is taken and when the wave leaves the query, not per task (global atomics per task
+    // pendHist holds the appends not yet added to the query's histogram in global memory: they are
+    // added at the query's next task with postings, after that task's first round of posting
+    // loads, and when the wave leaves the query, not right after the append (global atomics there
     // stall the wave's next posting loads; measured on the 10M-doc Q8 x 3-term rotation, 102 ->
     // 85 us without them).
     uint32_t ownHist[kHistPerLane], pendHist[kHistPerLane];
+    bool flushDue = false;
+    // With at most kEagerWaves waves per query of the batch, the atomics on a query's histogram
+    // contend little, and each append is added at once.
+    const bool eagerFlush = static_cast<int64_t>(gridDim.x) <= static_cast<int64_t>(kEagerWaves) * bt.header[0];
 #pragma unroll
     for (int i = 0; i < kHistPerLane; ++i)
         ownHist[i] = pendHist[i] = 0;
@@ -430,6 +434,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 atomicAdd(histw + lane * kHistPerLane + r, pendHist[r]);
             pendHist[r] = 0;
         }
+        flushDue = false;
     };
     // Candidates of a plain query go to this wave's own part of the query's pool, with no
     // atomic: the regions of the wave's blocks of the query, entries [segBlock * cap, (last block
@@ -452,8 +457,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     auto finishSegment = [&](int qq) {
         uint32_t n = 0;
         if (ownCount > 0) {
-            // The floor: the largest the wave used, including the histogram bound its last task of
-            // the query read.
+            // The floor: the largest the wave used, and its own histogram's bound.
             const uint32_t fl = max(floorSeen, histBound);
             unsigned long long *seg = wk.pool + poolBase + static_cast<int64_t>(segBlock) * capQ;
             // The entries written (the pool's capacity bounds them; it always holds them all).
@@ -514,36 +518,21 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             continue;
         }
 
-        // Lower bound of the query's K-th score, needed only by the selection: the score
+        // Lower bounds of the query's K-th score, needed only by the selection: the score
         // histogram, read at the query's first task here and then every kHistEvery-th task, and
-        // tau (below). These bounds only grow, so a value read early is still a lower bound. The
-        // histogram is not read every task: the appends' atomics keep its lines out of the
-        // caches, and a load's completion is counted in order with the posting loads issued
-        // after it, so every read stalls the task's first column round (measured on the 10M-doc
-        // Q8 x 3-term K10 Search: 97 us reading every task, 94 us every 8th task and after each
-        // append, 84 us every 16th task; reading it after the first round of posting loads at
-        // each wave's 4th task halved the candidates but doubled the streaming time).
+        // tau, every task; both loaded after the task's last round of posting loads, so that no
+        // posting load waits for them (a load's completion is counted in order with the loads
+        // issued before it, and the appends' atomics make these lines slow to read). These bounds
+        // only grow, so a value read early is still a lower bound.
         const bool readHist = histAge >= (k > kHistLargeK ? kHistEveryLargeK : kHistEvery);
-        // The query's last task in this wave's range reads the histogram again, with device scope,
-        // after its last round of posting loads (so that the read stalls no posting load): the
-        // bound it gives is the floor the wave's kept entries are filtered with when it leaves the
-        // query.
-        const bool lastTask = t + 1 == tEnd || blk + 1 == blocks;
         uint32_t hregs[kHistPerLane];
 #pragma unroll
         for (int i = 0; i < kHistPerLane; ++i)
             hregs[i] = 0;
-        if (readHist) {
+        if (readHist)
             histAge = 0;
-#pragma unroll
-            for (int i = 0; i < kHistPerLane; ++i)
-                hregs[i] = __hip_atomic_load(wk.hist + static_cast<int64_t>(q) * kHistBins + lane * kHistPerLane + i,
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
         ++histAge;
-        // The query's tau: every task, loaded (device scope: the raises the other waves
-        // published, not a cached copy) after the task's last round of posting loads, so that a
-        // slow read of this contended line delays only the selection, not the postings.
+        // The query's tau (device scope: the raises the other waves published, not a cached copy).
         uint32_t tauLoaded = 1;
 
         const int64_t g0 = static_cast<int64_t>(blk) * kGpb;
@@ -692,16 +681,23 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                             e[u] = ix.post[ReadLane(segLo, sg) + off + lane];
                     }
                 }
+                if (r0 == 0 && flushDue) {
+                    // The previous tasks' appends, added to the query's histogram after this
+                    // round's posting loads: waits for those loads do not wait for these atomics.
+                    __builtin_amdgcn_sched_barrier(0);
+                    flushHist(q);
+                    flushDue = false;
+                }
                 if (r0 + U >= columns) { // the task's last round: nothing waits for these loads before
                                          // its selection
                     __builtin_amdgcn_sched_barrier(0);
                     tauLoaded = __hip_atomic_load(wk.tau + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (lastTask) {
+                    if (readHist) {
 #pragma unroll
                         for (int i = 0; i < kHistPerLane; ++i)
                             hregs[i] = __hip_atomic_load(wk.hist + static_cast<int64_t>(q) * kHistBins +
                                                              lane * kHistPerLane + i,
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
@@ -797,7 +793,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         // the accumulator row is scanned.
         BM25MI_PHASE(2);
         const uint32_t top = WaveMax(__float_as_uint(runMax));
-        if (readHist || lastTask)
+        if (readHist)
             histBound = max(histBound, WaveHistBoundRegs(hregs, k));
         const uint32_t floor = max(max(tauLoaded, tau0q), histBound);
         floorSeen = max(floorSeen, floor);
@@ -842,9 +838,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         uint32_t before = mine;
         before = WaveInclusiveSum(before);
         before -= mine;
+        // Lanes count into each other's bins, so the bins are written and read as volatile: the
+        // compiler must not forward a lane's own zero to its read below, which would drop the
+        // other lanes' counts (LDS operations of a wave are performed in order).
+        volatile __attribute__((address_space(3))) uint32_t *bins =
+            (volatile __attribute__((address_space(3))) uint32_t *)sHist;
 #pragma unroll
         for (int r = 0; r < kHistPerLane; ++r)
-            sHist[lane * kHistPerLane + r] = 0;
+            bins[lane * kHistPerLane + r] = 0;
         unsigned long long *pool = wk.pool + poolBase;
         const int64_t capq = poolCap;
         if (mine) {
@@ -871,16 +872,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
         }
 #pragma unroll
         for (int r = 0; r < kHistPerLane; ++r) {
-            const uint32_t c = sHist[lane * kHistPerLane + r];
+            const uint32_t c = bins[lane * kHistPerLane + r];
             ownHist[r] += c;
             pendHist[r] += c;
         }
         ownCount += kept;
-        if (ownCount < 2 * k)
-            flushHist(q); // early appends: they raise the other waves' bounds the most
+        if (eagerFlush)
+            flushHist(q);
         if (ownCount >= k && ownCount >= 2 * ownAt) {
             ownAt = ownCount;
-            flushHist(q);
+            flushDue = true; // added at the query's next task with postings (or when the wave leaves it)
             const uint32_t ob = WaveHistBoundRegs(ownHist, k);
             if (ob > histBound && ob > tauLoaded)
                 tauNew = max(tauNew, ob);
@@ -1788,6 +1789,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 #pragma unroll
             for (int i = 0; i < kHistPerLane; ++i)
                 sHist[lane * kHistPerLane + i] = 0;
+            // Lanes count into each other's bins (see the scoring kernel): no forwarding of a
+            // lane's own zero to its read below.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
             for (int c = 0; c < kVpl; ++c)
                 if (vals[c] > tb ||
@@ -1798,6 +1802,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                     ++slot;
                     atomicAdd(sHist + HistBucket(vals[c]), 1u);
                 }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             // The query's score histogram, from which the final kernel bounds its K-th score.
             uint32_t *histw = wk.hist + static_cast<int64_t>(q) * kHistBins;
 #pragma unroll
