@@ -329,9 +329,8 @@ __device__ __forceinline__ uint32_t PlanTask(const int64_t *prefix, int32_t nq, 
 
 // This wave's contiguous range [t0, t1) of the task order: an equal share of the plan's total
 // cost, found by comparing the cost prefix with the share's ends, 64 queries per load.
-__device__ __forceinline__ void WaveRange(const BatchView &bt, int32_t blocks, uint32_t &t0, uint32_t &t1)
+__device__ __forceinline__ void WaveRange(const BatchView &bt, int32_t nq, int32_t blocks, uint32_t &t0, uint32_t &t1)
 {
-    const int32_t nq = bt.header[0];
     const uint64_t total = static_cast<uint64_t>(bt.costPrefix[nq]);
     const uint64_t lo = total * blockIdx.x / gridDim.x, hi = total * (blockIdx.x + 1) / gridDim.x;
     uint32_t c0 = 0, c1 = 0;
@@ -348,10 +347,12 @@ __device__ __forceinline__ void WaveRange(const BatchView &bt, int32_t blocks, u
 }
 
 template <int B>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t addRows, int32_t keepCap)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void KScore(IndexView ix, BatchView bt, WorkView wk, int32_t blocks, int32_t addRows, int32_t keepCap, int32_t nqKnown)
 {
     // addRows: the live kernel ran before this one in the same Search and wrote counter rows
     // [0, addRows); a wave below that adds its counters to its row. keepCap: BlockKeepCap's.
+    // nqKnown: the batch's query count when the launch knows it (-1: read from the packet, as a
+    // captured graph does), which saves a dependent load.
     constexpr int kVec = B / (4 * kWave);   // float4 accumulators per lane
     constexpr int kGpb = B / kGranule;      // granules per block
     constexpr int U = kColumns;
@@ -389,9 +390,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
 #endif
 
     // This wave's tasks: a contiguous range of the query-major task order (task = q * blocks
-    // + blk), cut by the host so that every wave gets the same estimated work.
+    // + blk), cut by the host so that every wave gets the same estimated work. In batches of fewer
+    // than 64 queries, lane i first loads query i's state, in flight with the plan's loads: the
+    // wave's first query then needs no further round trip but its words.
+    const int32_t nq = nqKnown >= 0 ? nqKnown : bt.header[0];
+    const bool fewQueries = nq < kWave;
+    int64_t preOff = 0, prePool = 0;
+    float preThr = 0.0F;
+    int32_t preK = 1;
+    uint32_t preTau0 = 0;
+    if (fewQueries && static_cast<int32_t>(lane) <= nq) {
+        preOff = bt.qoff[lane];
+        prePool = bt.poolOff[lane];
+        if (static_cast<int32_t>(lane) < nq) {
+            preThr = bt.thr[lane];
+            preK = bt.k[lane];
+            preTau0 = bt.tau0[lane];
+        }
+    }
     uint32_t tBegin, tEnd;
-    WaveRange(bt, blocks, tBegin, tEnd);
+    WaveRange(bt, nq, blocks, tBegin, tEnd);
     // Per-query state, reloaded only when the task's query changes.
     int qCur = -1, n = 0;
     float thr = 0.0F;
@@ -421,7 +439,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
     bool flushDue = false;
     // With at most kEagerWaves waves per query of the batch, the atomics on a query's histogram
     // contend little, and each append is added at once.
-    const bool eagerFlush = static_cast<int64_t>(gridDim.x) <= static_cast<int64_t>(kEagerWaves) * bt.header[0];
+    const bool eagerFlush = static_cast<int64_t>(gridDim.x) <= static_cast<int64_t>(kEagerWaves) * nq;
 #pragma unroll
     for (int i = 0; i < kHistPerLane; ++i)
         ownHist[i] = pendHist[i] = 0;
@@ -479,6 +497,42 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
             for (uint32_t done = 0, bb = static_cast<uint32_t>(segBlock); done < n; done += capQ, ++bb)
                 wk.blockCount[static_cast<int64_t>(qq) * BlockCountStride(blocks) + bb] = min(capQ, n - done);
     };
+    // Enters query q at block blk (its state as loaded).
+    auto enterQuery = [&](int q, int blk, int64_t w0, int64_t w1, float thrQ, int32_t kQ, uint32_t tau0Q, int64_t p0,
+                          int64_t p1) {
+        qCur = q;
+        segBlock = blk;
+        n = static_cast<int>(w1 - w0);
+        thr = thrQ;
+        k = static_cast<uint32_t>(kQ);
+        tau0q = tau0Q;
+        info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
+        if (static_cast<int>(lane) < n)
+            info = bt.words[w0 + lane];
+        chunksValid = false;
+        poolBase = p0;
+        poolCap = p1 - p0;
+        histBound = 1;
+        histAge = kHistEvery;
+#pragma unroll
+        for (int i = 0; i < kHistPerLane; ++i)
+            ownHist[i] = 0;
+        ownCount = ownAt = 0;
+        capQ = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B, keepCap));
+        floorSeen = 1;
+    };
+    if (fewQueries && tBegin < tEnd) {
+        // The first query's state from the lanes that loaded it (q + 1 <= nq < 64).
+        const int q = static_cast<int>(tBegin / static_cast<uint32_t>(blocks));
+        const auto lane64 = [](int64_t v, int i) {
+            return static_cast<int64_t>((static_cast<uint64_t>(ReadLane(static_cast<uint32_t>(v >> 32), i)) << 32) |
+                                        ReadLane(static_cast<uint32_t>(v), i));
+        };
+        enterQuery(q, static_cast<int>(tBegin - static_cast<uint32_t>(q) * static_cast<uint32_t>(blocks)),
+                   lane64(preOff, q), lane64(preOff, q + 1), __uint_as_float(ReadLane(__float_as_uint(preThr), q)),
+                   static_cast<int32_t>(ReadLane(static_cast<uint32_t>(preK), q)), ReadLane(preTau0, q),
+                   lane64(prePool, q), lane64(prePool, q + 1));
+    }
     for (uint32_t t = tBegin; t < tEnd; ++t) {
         BM25MI_PHASE(0);
         const int q = static_cast<int>(t / static_cast<uint32_t>(blocks));
@@ -490,27 +544,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
                 flushHist(qCur);
                 finishSegment(qCur);
             }
-            qCur = q;
-            segBlock = blk;
-            const int64_t w0 = bt.qoff[q];
-            n = static_cast<int>(bt.qoff[q + 1] - w0);
-            thr = bt.thr[q];
-            k = static_cast<uint32_t>(bt.k[q]);
-            tau0q = bt.tau0[q];
-            info = make_uint4(0u, 0u, 0xFFFFFFFFu, 0u);
-            if (static_cast<int>(lane) < n)
-                info = bt.words[w0 + lane];
-            chunksValid = false;
-            poolBase = bt.poolOff[q];
-            poolCap = bt.poolOff[q + 1] - poolBase;
-            histBound = 1;
-            histAge = kHistEvery;
-#pragma unroll
-            for (int i = 0; i < kHistPerLane; ++i)
-                ownHist[i] = 0;
-            ownCount = ownAt = 0;
-            capQ = static_cast<uint32_t>(BlockKeepCap(static_cast<int32_t>(k), B, keepCap));
-            floorSeen = 1;
+            enterQuery(q, blk, bt.qoff[q], bt.qoff[q + 1], bt.thr[q], bt.k[q], bt.tau0[q], bt.poolOff[q],
+                       bt.poolOff[q + 1]);
         }
         if (thr > 0.0F) {
             // A threshold query's live blocks are scored by KLive.
@@ -2449,22 +2484,23 @@ void LaunchInit(const WorkView &wk, int32_t maxQ, hipStream_t stream)
 }
 
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
-                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, int32_t addRows, int32_t keepCap)
+                 int32_t grid, hipStream_t stream, hipEvent_t start, hipEvent_t stop, int32_t addRows, int32_t keepCap,
+                 int32_t nqKnown)
 {
     // With events, hipExtLaunchKernelGGL stamps them at the kernel's own start and end (the
     // interval the profiler reports), not around the launch.
     switch (blockDocs) {
     case 2048:
         hipExtLaunchKernelGGL(KScore<2048>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows,
-                              keepCap);
+                              keepCap, nqKnown);
         return true;
     case 4096:
         hipExtLaunchKernelGGL(KScore<4096>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows,
-                              keepCap);
+                              keepCap, nqKnown);
         return true;
     case 8192:
         hipExtLaunchKernelGGL(KScore<8192>, dim3(grid), dim3(kWave), 0, stream, start, stop, 0, ix, bt, wk, blocks, addRows,
-                              keepCap);
+                              keepCap, nqKnown);
         return true;
     default:
         return false;

@@ -196,10 +196,11 @@ void LaunchStage(const uint4 *src, uint4 *dst, const StageRanges &ranges, const 
                  const ModelView &mv, int32_t predictQueries, hipStream_t stream);
 // start/stop (optional): events stamped at the kernel's own start and end.
 // addRows: add the counters to the rows of the live kernel that ran before it in the same Search.
-// keepCap: the session's keep_cap (BlockKeepCap).
+// keepCap: the session's keep_cap (BlockKeepCap). nqKnown: the batch's query count, or -1 for
+// the kernel to read it from the packet (a captured graph).
 bool LaunchScore(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,
                  int32_t grid, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
-                 int32_t addRows = 0, int32_t keepCap = 0);
+                 int32_t addRows = 0, int32_t keepCap = 0, int32_t nqKnown = -1);
 // Live granules of threshold queries, after LaunchPrune with pruneWaves item waves; counter rows
 // from rowBase on.
 bool LaunchLive(int blockDocs, const IndexView &ix, const BatchView &bt, const WorkView &wk, int32_t blocks,

@@ -466,7 +466,7 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
         BM25MI_HIP(hipEventRecord(slot.ev[1], slot.stream), "record score start");
     if (mode == 0) {
         Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, timed ? slot.ev[1] : nullptr,
-                            timed ? slot.ev[2] : nullptr, 0, cfg.keep_cap),
+                            timed ? slot.ev[2] : nullptr, 0, cfg.keep_cap, capture ? -1 : slot.nq),
                 "Unsupported block size");
     } else {
         // One stream: planning (with the plain queries' wave plan of learned batches), live
@@ -480,7 +480,7 @@ void Session::Enqueue(Slot &slot, bool timed, int mode, bool capture)
                 "Unsupported block size");
         if (plainToo)
             Require(LaunchScore(blockDocs, ix, slot.bt, slot.wk, blocks, grid, slot.stream, nullptr, nullptr, liveGrid,
-                                cfg.keep_cap),
+                                cfg.keep_cap, capture ? -1 : slot.nq),
                     "Unsupported block size");
     }
     if (timed && mode != 0)
@@ -847,7 +847,7 @@ void Session::SearchScratch(const Query &query, float *outScores, int32_t *outId
         // The scratch keeps the last call's contents, possibly another session's: the counters
         // are set by a fill kernel before scoring.
         LaunchInit(wk, cfg.max_queries, slot.stream);
-        Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream, nullptr, nullptr, 0, cfg.keep_cap),
+        Require(LaunchScore(blockDocs, ix, bt, wk, blocks, grid, slot.stream, nullptr, nullptr, 0, cfg.keep_cap, slot.nq),
                 "Unsupported block size");
         slot.finalGrid = FinalGrid(slot.nq);
         LaunchFinal(bt, wk, slot.finalGrid, static_cast<unsigned long long *>(slot.hResultDev), grid, blocks, blockDocs,
