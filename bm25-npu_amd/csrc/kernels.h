@@ -26,6 +26,11 @@ constexpr float kLiveMinRatio = 0.3F;
 // ... and only when its K is at most kLiveMaxK: at K = 100 the live path (a cut of up to K docs per
 // live granule) measured 0.92-1.0x of the plain path on the 10M-doc corpus.
 constexpr int32_t kLiveMaxK = 64;
+// Live granules whose appends a threshold query's candidate pool reserves room for beyond the
+// plain path's capacity (which covers many more on large indexes: 312K entries at K = 10 on the
+// 10M-doc corpus, against 78K granules x 10 for every granule). The learned thresholds keep a
+// few hundred docs per batch there; a live pool that overflows reruns its query on the plain path.
+constexpr int64_t kLiveReserveGranules = 4096;
 #ifndef BM25MI_COLUMNS
 #define BM25MI_COLUMNS 14
 #endif

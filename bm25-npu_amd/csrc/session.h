@@ -129,8 +129,9 @@ public:
 
     void Release() noexcept;
     // Candidate pool entries one query may need: every block of a plain query appends at most
-    // BlockKeepCap(K) docs, every live granule of a threshold query at most min(K, 256), and no
-    // doc is appended twice. A threshold query's pool also fits the plain path.
+    // BlockKeepCap(K) docs and no doc is appended twice. A threshold query's pool also holds the
+    // appends of up to kLiveReserveGranules live granules (min(K, kGranule) each); an overflow
+    // reruns it on the plain path.
     int64_t PoolCapacity(int32_t k, bool threshold = false) const;
     // Score bits of the largest K-th term impact among the query's terms: every doc scores at
     // least each of its impacts, so the query's K-th score is at least this (0 if unknown).

@@ -93,8 +93,14 @@ int64_t Session::PoolCapacity(int32_t k, bool threshold) const
 {
     const int64_t plain =
         std::min<int64_t>(static_cast<int64_t>(blocks) * BlockKeepCap(k, blockDocs, cfg.keep_cap), ndocs);
-    // A threshold query may be scored on the plain path (Pack), so its pool covers both.
-    return threshold ? std::max(plain, std::min<int64_t>(granules * std::min<int64_t>(k, kGranule), ndocs)) : plain;
+    // A threshold query may be scored on the plain path (Pack, or the staging kernel's routing),
+    // so its pool covers the plain path; on the live path each live granule appends at most
+    // min(K, kGranule) docs, reserved for up to kLiveReserveGranules granules (a live pool that
+    // still overflows reruns the query on the plain path: the final kernel flags it).
+    if (!threshold)
+        return plain;
+    const int64_t live = std::min<int64_t>(granules, kLiveReserveGranules) * std::min<int64_t>(k, kGranule);
+    return std::max(plain, std::min<int64_t>(live, ndocs));
 }
 
 uint32_t Session::StartBound(const int32_t *terms, int64_t n, int32_t k) const
