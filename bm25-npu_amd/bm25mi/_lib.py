@@ -165,10 +165,28 @@ def check(rc, handle=None):
         raise Bm25Error(rc, message, device_code)
 
 
+# Addresses of small arrays passed again (request arrays and output buffers a caller reuses):
+# numpy's ctypes.data costs about 2 us per array, a few percent of a Search. The cache holds each
+# array itself, so its id cannot be reused by another object and numpy refuses to resize it in
+# place (refcheck) while it is cached: the address stays valid. At most _ADDR_MAX entries of at
+# most _ADDR_BYTES each; the cache is emptied when full.
+_ADDR_MAX = 64
+_ADDR_BYTES = 1 << 16
+_addr = {}
+
+
 def ptr(array):
     """Raw data address of a C-contiguous numpy array (or None), passed as a void* argument."""
     if array is None:
         return None
+    hit = _addr.get(id(array))
+    if hit is not None and hit[0] is array:
+        return hit[1]
     if not array.flags.c_contiguous:
         raise ValueError('array must be C-contiguous')
-    return array.ctypes.data
+    address = array.ctypes.data
+    if array.nbytes <= _ADDR_BYTES:
+        if len(_addr) >= _ADDR_MAX:
+            _addr.clear()
+        _addr[id(array)] = (array, address)
+    return address

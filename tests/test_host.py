@@ -263,3 +263,23 @@ def test_search_session_request_and_response_checks():
     for s, i in broken:
         with pytest.raises(RuntimeError):
             check_response(s, i, [[1], [2]], 3, 10)
+
+
+def test_array_address_cache():
+    """_lib.ptr caches the addresses of small arrays it has seen (request arrays and output
+    buffers a caller reuses): the address equals numpy's, the array cannot be resized in place
+    while cached, non-contiguous arrays are rejected, large arrays are not cached and the cache
+    stays bounded."""
+    a = np.arange(64, dtype=np.int64)
+    assert _lib.ptr(a) == a.ctypes.data and _lib.ptr(a) == a.ctypes.data
+    with pytest.raises(ValueError):
+        a.resize(128)  # the cached address would go stale
+    with pytest.raises(ValueError):
+        _lib.ptr(np.arange(64, dtype=np.int32)[::2])
+    assert _lib.ptr(None) is None
+    big = np.zeros(_lib._ADDR_BYTES // 4 + 1, np.int32)
+    assert _lib.ptr(big) == big.ctypes.data and id(big) not in _lib._addr
+    keep = [np.zeros(4, np.int32) for _ in range(3 * _lib._ADDR_MAX)]
+    for x in keep:
+        assert _lib.ptr(x) == x.ctypes.data
+    assert len(_lib._addr) <= _lib._ADDR_MAX
