@@ -3,12 +3,14 @@
 //
 // Parameter layout (flat): W1[64][in] b1[64] W2[64][64] b2[64] W3[64] b3[1].
 //   forward   one thread per sample, weights staged in LDS (broadcast reads).
-//   backward  each workgroup walks 64-sample tiles: tile forward pass into LDS,
-//             mean-squared-error gradient, and per-thread register accumulators
-//             for its share of every parameter gradient; partials go to a
+//   backward  each workgroup walks 64-sample tiles: the tile's forward pass into LDS,
+//             the mean-squared-error gradient, and its products on the matrix cores
+//             (MFMA 16x16x4 FP32), with the weight gradients accumulated in the
+//             waves' MFMA accumulators across tiles; partials go to a
 //             per-workgroup buffer and a second kernel sums them in a fixed
 //             order, so gradients are deterministic run to run.
-//   adam      elementwise Adam update with bias correction.
+//   adam      elementwise Adam update with bias correction; a training step sums the
+//             partials and applies the update in one launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,158 +71,217 @@ __global__ __launch_bounds__(kMlpThreads) void KMlpForward(const float *__restri
     y[s] = out;
 }
 
+typedef float MfmaF4 __attribute__((ext_vector_type(4)));
+
+// LDS row strides of the backward kernel's tiles: odd, so that the 16 rows one operand fetch
+// spans fall in distinct banks.
+constexpr int kXS = kMaxIn + 1;
+constexpr int kHS = kH + 1;
+
+// acc[t] += A[m0, m0+16) x [0, kK) * B[0, kK) x [16t, 16t+16) on the matrix cores
+// (v_mfma_f32_16x16x4f32: lane l gives A[l % 16][l / 16] and B[l / 16][l % 16] per step of 4,
+// and holds C[4 (l / 16) + r][l % 16] in element r). a points at A[m0][0], b at B[0][0];
+// A[m][k] = a[m * kAS + k * kAK], B[k][n] = b[k * kBS + n * kBN].
+template <int kTiles, int kK, int kAS, int kAK, int kBS, int kBN>
+__device__ inline void MfmaRows(MfmaF4 (&acc)[kTiles], const float *a, const float *b, int lane)
+{
+    const int r = lane & 15, c = lane >> 4;
+    a += r * kAS + c * kAK;
+    b += c * kBS + r * kBN;
+#pragma unroll
+    for (int k0 = 0; k0 < kK; k0 += 4) {
+        const float av = a[k0 * kAK];
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b[k0 * kBS + 16 * t * kBN], acc[t], 0, 0, 0);
+    }
+}
+
 // Partial gradients of one workgroup: partial[blockIdx.x][0, P) and lossPartial[blockIdx.x].
+// Each 64-sample tile runs its five 64-wide products on the matrix cores, wave w taking the
+// output rows 16w .. 16w+15: h1 = relu(x W1^T + b1), h2 = relu(h1 W2^T + b2), then
+// dW2 += dz2^T h1, dz1 = (dz2 W2) * (h1 > 0) and dW1 += dz1^T x, with dz2 = dy w3 * (h2 > 0).
+// The weight gradients stay in the waves' accumulators across tiles; the 64-long sums (y, the
+// biases' and W3's gradients) are per-lane loops of wave 0.
 __global__ __launch_bounds__(kMlpThreads) void KMlpBackward(const float *__restrict__ p, const float *__restrict__ x,
                                                             const float *__restrict__ target, int n, int in,
                                                             float *__restrict__ partial,
                                                             float *__restrict__ lossPartial)
 {
-    __shared__ float sX[kTile * kMaxIn];
-    __shared__ float sH1[kTile * kH]; // h1, later dz1
-    __shared__ float sH2[kTile * kH]; // h2, later dz2
-    __shared__ float sW2[kH * kH];
+    __shared__ float sX[kTile * kXS];  // x tile; columns in .. kMaxIn-1 and rows past the batch are 0
+    __shared__ float sW1[kH * kXS];    // W1; columns in .. kMaxIn-1 are 0
+    __shared__ float sW2[kH * kHS];
+    __shared__ float sH1[kTile * kHS]; // h1, later dz1
+    __shared__ float sH2[kTile * kHS]; // h2, later dz2
     __shared__ float sDy[kTile];
     __shared__ float sRed[kMlpThreads / 64];
     const float *W1 = p, *B1 = W1 + kH * in, *W2 = B1 + kH, *B2 = W2 + kH * kH, *W3 = B2 + kH, *B3 = W3 + kH;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < kH * kMaxIn; i += kMlpThreads) {
+        const int j = i / kMaxIn, c = i - j * kMaxIn;
+        sW1[j * kXS + c] = c < in ? W1[j * in + c] : 0.0F;
+    }
     for (int i = tid; i < kH * kH; i += kMlpThreads)
-        sW2[i] = W2[i];
-    // Ownership: unit j = tid % 64 and rows r = tid / 64 + 4 * m (m < 16) of every 64x64 tile.
-    const int unit = tid & 63, quarter = tid >> 6;
-    float gW2[16];   // dW2[unit][quarter*16 + m]
-    float gW1[kMaxIn / 4 + 1]; // dW1[unit][quarter + 4*m], in/4 rounded up
-    float gB1 = 0, gB2 = 0, gW3 = 0, gB3 = 0, loss = 0;
+        sW2[(i / kH) * kHS + i % kH] = W2[i];
+    // Lane's place in a 16 x 16 output tile: rows 4 * quad + r, column col.
+    const int col = lane & 15, quad = lane >> 4, m0 = 16 * wave;
+    float b1v[4], b2v[4];
 #pragma unroll
-    for (int m = 0; m < 16; ++m)
-        gW2[m] = 0;
+    for (int t = 0; t < 4; ++t) {
+        b1v[t] = B1[16 * t + col];
+        b2v[t] = B2[16 * t + col];
+    }
+    MfmaF4 gW2[4], gW1[kMaxIn / 16]; // dW2[m0 + 4 quad + r][16t + col], dW1[m0 + 4 quad + r][16t + col]
 #pragma unroll
-    for (int m = 0; m < kMaxIn / 4 + 1; ++m)
-        gW1[m] = 0;
+    for (int t = 0; t < 4; ++t)
+        gW2[t] = MfmaF4{0.0F, 0.0F, 0.0F, 0.0F};
+#pragma unroll
+    for (int t = 0; t < kMaxIn / 16; ++t)
+        gW1[t] = MfmaF4{0.0F, 0.0F, 0.0F, 0.0F};
+    float gB1 = 0, gB2 = 0, gW3 = 0, gB3 = 0, loss = 0; // wave 0: unit = lane
+    const float w3 = W3[lane];
     const float scale = 2.0F / static_cast<float>(n);
     __syncthreads();
     for (int64_t t0 = int64_t(blockIdx.x) * kTile; t0 < n; t0 += int64_t(gridDim.x) * kTile) {
         const int rows = static_cast<int>(min<int64_t>(kTile, n - t0));
-        for (int i = tid; i < kTile * in; i += kMlpThreads) {
-            const int s = i / in;
-            sX[i] = s < rows ? x[(t0 + s) * in + (i - s * in)] : 0.0F;
+        for (int i = tid; i < kTile * kMaxIn; i += kMlpThreads) {
+            const int s = i / kMaxIn, c = i - s * kMaxIn;
+            sX[s * kXS + c] = s < rows && c < in ? x[(t0 + s) * in + c] : 0.0F;
         }
         __syncthreads();
-        // h1 = relu(W1 x + b1)
-        for (int s = quarter; s < kTile; s += 4) {
-            float z = B1[unit];
-            for (int i = 0; i < in; ++i)
-                z = fmaf(W1[unit * in + i], sX[s * in + i], z);
-            sH1[s * kH + unit] = fmaxf(z, 0.0F);
+        {
+            MfmaF4 acc[4] = {};
+            MfmaRows<4, kMaxIn, kXS, 1, 1, kXS>(acc, sX + m0 * kXS, sW1, lane);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    sH1[(m0 + 4 * quad + r) * kHS + 16 * t + col] = fmaxf(acc[t][r] + b1v[t], 0.0F);
         }
         __syncthreads();
-        // h2 = relu(W2 h1 + b2)
-        for (int s = quarter; s < kTile; s += 4) {
-            float z = B2[unit];
-            for (int j = 0; j < kH; ++j)
-                z = fmaf(sW2[unit * kH + j], sH1[s * kH + j], z);
-            sH2[s * kH + unit] = fmaxf(z, 0.0F);
+        {
+            MfmaF4 acc[4] = {};
+            MfmaRows<4, kH, kHS, 1, 1, kHS>(acc, sH1 + m0 * kHS, sW2, lane);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    sH2[(m0 + 4 * quad + r) * kHS + 16 * t + col] = fmaxf(acc[t][r] + b2v[t], 0.0F);
         }
         __syncthreads();
-        // y, dy = 2 (y - target) / n
-        if (tid < kTile) {
+        // y, dy = 2 (y - target) / n, then dW3, db3 (wave 0, sample / unit = lane).
+        if (wave == 0) {
             float yv = B3[0];
             for (int k = 0; k < kH; ++k)
-                yv = fmaf(W3[k], sH2[tid * kH + k], yv);
-            const float d = tid < rows ? yv - target[t0 + tid] : 0.0F;
+                yv = fmaf(W3[k], sH2[lane * kHS + k], yv);
+            const float d = lane < rows ? yv - target[t0 + lane] : 0.0F;
             loss = fmaf(d, d, loss);
-            sDy[tid] = d * scale;
+            sDy[lane] = d * scale;
         }
         __syncthreads();
-        // dW3[unit], db3, then dz2 = dy * W3 * (h2 > 0) in place of h2.
-        if (quarter == 0) {
+        if (wave == 0) {
             for (int s = 0; s < kTile; ++s)
-                gW3 = fmaf(sDy[s], sH2[s * kH + unit], gW3);
-            if (unit == 0)
+                gW3 = fmaf(sDy[s], sH2[s * kHS + lane], gW3);
+            if (lane == 0)
                 for (int s = 0; s < kTile; ++s)
                     gB3 += sDy[s];
         }
         __syncthreads();
-        const float w3 = W3[unit];
-        for (int s = quarter; s < kTile; s += 4)
-            sH2[s * kH + unit] = sH2[s * kH + unit] > 0.0F ? sDy[s] * w3 : 0.0F;
+        // dz2 = dy * W3 * (h2 > 0) in place of h2.
+        for (int s = wave; s < kTile; s += 4)
+            sH2[s * kHS + lane] = sH2[s * kHS + lane] > 0.0F ? sDy[s] * w3 : 0.0F;
         __syncthreads();
-        // dW2[k][j] += sum_s dz2[s][k] h1[s][j]; owner k = unit, j = quarter*16 + m.
+        // dW2[k][j] += sum_s dz2[s][k] h1[s][j] (rows k), and dz2 W2 (rows s) before h1 is overwritten.
+        MfmaRows<4, kTile, 1, kHS, kHS, 1>(gW2, sH2 + m0, sH1, lane);
+        MfmaF4 dz[4] = {};
+        MfmaRows<4, kH, kHS, 1, kHS, 1>(dz, sH2 + m0 * kHS, sW2, lane);
+        if (wave == 0)
+            for (int s = 0; s < kTile; ++s)
+                gB2 += sH2[s * kHS + lane];
+        __syncthreads();
+        // dz1 = (dz2 W2) * (h1 > 0) in place of h1: each element read and written by its own lane.
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int j = quarter * 16 + m;
-            float acc = gW2[m];
-            for (int s = 0; s < kTile; ++s)
-                acc = fmaf(sH2[s * kH + unit], sH1[s * kH + j], acc);
-            gW2[m] = acc;
-        }
-        if (quarter == 0)
-            for (int s = 0; s < kTile; ++s)
-                gB2 += sH2[s * kH + unit];
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float &h = sH1[(m0 + 4 * quad + r) * kHS + 16 * t + col];
+                h = h > 0.0F ? dz[t][r] : 0.0F;
+            }
         __syncthreads();
-        // dz1[s][j] = (h1 > 0) * sum_k dz2[s][k] W2[k][j], in place of h1 (same owner reads then writes).
-        for (int s = quarter; s < kTile; s += 4) {
-            float acc = 0.0F;
-            for (int k = 0; k < kH; ++k)
-                acc = fmaf(sH2[s * kH + k], sW2[k * kH + unit], acc);
-            sH1[s * kH + unit] = sH1[s * kH + unit] > 0.0F ? acc : 0.0F;
-        }
-        __syncthreads();
-        // dW1[j][i] += sum_s dz1[s][j] x[s][i]; owner j = unit, i = quarter + 4*m.
-        for (int m = 0; quarter + 4 * m < in; ++m) {
-            const int i = quarter + 4 * m;
-            float acc = gW1[m];
+        // dW1[j][i] += sum_s dz1[s][j] x[s][i] (rows j, columns i < kMaxIn).
+        MfmaRows<kMaxIn / 16, kTile, 1, kHS, kXS, 1>(gW1, sH1 + m0, sX, lane);
+        if (wave == 0)
             for (int s = 0; s < kTile; ++s)
-                acc = fmaf(sH1[s * kH + unit], sX[s * in + i], acc);
-            gW1[m] = acc;
-        }
-        if (quarter == 0)
-            for (int s = 0; s < kTile; ++s)
-                gB1 += sH1[s * kH + unit];
+                gB1 += sH1[s * kHS + lane];
         __syncthreads();
     }
     float *out = partial + int64_t(blockIdx.x) * MlpParams(in);
     float *oW1 = out, *oB1 = oW1 + kH * in, *oW2 = oB1 + kH, *oB2 = oW2 + kH * kH, *oW3 = oB2 + kH, *oB3 = oW3 + kH;
 #pragma unroll
-    for (int m = 0; m < 16; ++m)
-        oW2[unit * kH + quarter * 16 + m] = gW2[m];
-    for (int m = 0; quarter + 4 * m < in; ++m)
-        oW1[unit * in + quarter + 4 * m] = gW1[m];
-    if (quarter == 0) {
-        oB1[unit] = gB1;
-        oB2[unit] = gB2;
-        oW3[unit] = gW3;
-        if (unit == 0)
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            oW2[(m0 + 4 * quad + r) * kH + 16 * t + col] = gW2[t][r];
+#pragma unroll
+    for (int t = 0; t < kMaxIn / 16; ++t)
+        if (16 * t + col < in)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                oW1[(m0 + 4 * quad + r) * in + 16 * t + col] = gW1[t][r];
+    if (wave == 0) {
+        oB1[lane] = gB1;
+        oB2[lane] = gB2;
+        oW3[lane] = gW3;
+        if (lane == 0)
             oB3[0] = gB3;
     }
     // Loss partial: fixed-order block sum.
     for (int o = 32; o > 0; o >>= 1)
         loss += __shfl_xor(loss, o);
-    if ((tid & 63) == 0)
-        sRed[tid >> 6] = loss;
+    if (lane == 0)
+        sRed[wave] = loss;
     __syncthreads();
     if (tid == 0)
         lossPartial[blockIdx.x] = sRed[0] + sRed[1] + sRed[2] + sRed[3];
 }
 
-// Sums the workgroup partials in a fixed order. train = 0: grad += sum, lossOut[0] = mean loss;
-// train = 1 (training steps): grad = sum, lossOut[0] += mean loss (the epoch's running sum).
+// Fixed-order sum of the workgroup partials of parameter i.
+__device__ inline float PartialSum(const float *__restrict__ partial, int blocks, int64_t count, int64_t i)
+{
+    float s = 0.0F;
+    for (int b = 0; b < blocks; ++b)
+        s += partial[int64_t(b) * count + i];
+    return s;
+}
+
+__device__ inline float MeanLoss(const float *__restrict__ lossPartial, int blocks, int n)
+{
+    float s = 0.0F;
+    for (int b = 0; b < blocks; ++b)
+        s += lossPartial[b];
+    return s / static_cast<float>(n);
+}
+
+// Adam update of parameter i with gradient gi; c1, c2 are the bias corrections 1 - beta^step.
+__device__ inline void AdamAt(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v, int64_t i, float gi,
+                              float lr, float beta1, float beta2, float eps, float c1, float c2)
+{
+    const float mi = beta1 * m[i] + (1.0F - beta1) * gi;
+    const float vi = beta2 * v[i] + (1.0F - beta2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// grad += the summed partials, lossOut[0] = mean loss.
 __global__ void KMlpReduce(const float *__restrict__ partial, const float *__restrict__ lossPartial, int blocks,
-                           int64_t count, int n, float *__restrict__ grad, float *__restrict__ lossOut, int train)
+                           int64_t count, int n, float *__restrict__ grad, float *__restrict__ lossOut)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < count) {
-        float s = 0.0F;
-        for (int b = 0; b < blocks; ++b)
-            s += partial[int64_t(b) * count + i];
-        grad[i] = train ? s : grad[i] + s;
-    }
-    if (i == 0 && lossOut) {
-        float s = 0.0F;
-        for (int b = 0; b < blocks; ++b)
-            s += lossPartial[b];
-        const float mean = s / static_cast<float>(n);
-        lossOut[0] = train ? lossOut[0] + mean : mean;
-    }
+    if (i < count)
+        grad[i] += PartialSum(partial, blocks, count, i);
+    if (i == 0 && lossOut)
+        lossOut[0] = MeanLoss(lossPartial, blocks, n);
 }
 
 __global__ void KMlpAdam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
@@ -228,14 +289,23 @@ __global__ void KMlpAdam(float *__restrict__ p, const float *__restrict__ g, flo
                          float c1, float c2)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= count)
-        return;
-    const float gi = g[i];
-    const float mi = beta1 * m[i] + (1.0F - beta1) * gi;
-    const float vi = beta2 * v[i] + (1.0F - beta2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    if (i < count)
+        AdamAt(p, m, v, i, g[i], lr, beta1, beta2, eps, c1, c2);
+}
+
+// One training step after the backward kernel, in one launch: each parameter's gradient is the
+// fixed-order sum of the workgroup partials, followed by its Adam update; lossOut[0] += the
+// minibatch's mean loss (the epoch's running sum).
+__global__ void KMlpReduceAdam(const float *__restrict__ partial, const float *__restrict__ lossPartial, int blocks,
+                               int64_t count, int n, float *__restrict__ p, float *__restrict__ m,
+                               float *__restrict__ v, float *__restrict__ lossOut, float lr, float beta1, float beta2,
+                               float eps, float c1, float c2)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count)
+        AdamAt(p, m, v, i, PartialSum(partial, blocks, count, i), lr, beta1, beta2, eps, c1, c2);
+    if (i == 0)
+        lossOut[0] += MeanLoss(lossPartial, blocks, n);
 }
 
 namespace {
@@ -296,7 +366,7 @@ extern "C" int bm25mi_mlp_backward(const float *params, const float *x, const fl
         BM25MI_MLP_HIP(hipGetLastError(), "MLP backward launch");
         const int rgrid = static_cast<int>((count + 255) / 256);
         hipLaunchKernelGGL(KMlpReduce, dim3(rgrid), dim3(256), 0, s, partial, lossPartial, blocks, count, n, grad,
-                           loss_out, 0);
+                           loss_out);
         BM25MI_MLP_HIP(hipGetLastError(), "MLP reduce launch");
         // The scratch buffer is reused by the next call: finish this one first.
         BM25MI_MLP_HIP(hipStreamSynchronize(s), "MLP backward sync");
@@ -457,7 +527,7 @@ extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *targ
         const int64_t count = MlpParams(in);
         const int blocks = static_cast<int>(std::min<int64_t>(kBackwardGrid, (int64_t(batch) + kTile - 1) / kTile));
         Stream st;
-        DeviceBuffer dp(count), dg(count), dm(count), dv(count), dx(static_cast<size_t>(n) * in), dt(n), dl(1);
+        DeviceBuffer dp(count), dm(count), dv(count), dx(static_cast<size_t>(n) * in), dt(n), dl(1);
         DeviceBuffer partial(static_cast<size_t>(blocks) * count), lossPartial(static_cast<size_t>(blocks));
         dp.Upload(params, count, st.s);
         BM25MI_MLP_HIP(hipMemsetAsync(dm.p, 0, count * sizeof(float), st.s), "Adam m clear");
@@ -469,7 +539,6 @@ extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *targ
         uint64_t rng = seed;
         int32_t step = 0;
         const int rgrid = static_cast<int>((count + 255) / 256);
-        const int agrid = rgrid;
         for (int e = 0; e < epochs; ++e) {
             for (int32_t i = n - 1; i > 0; --i) // Fisher-Yates
                 std::swap(order[i], order[static_cast<int32_t>(SplitMix64(rng) % static_cast<uint64_t>(i + 1))]);
@@ -485,13 +554,11 @@ extern "C" int bm25mi_mlp_train(float *params, const float *x, const float *targ
             for (int32_t b0 = 0; b0 + batch <= n; b0 += batch) {
                 hipLaunchKernelGGL(KMlpBackward, dim3(blocks), dim3(kMlpThreads), 0, st.s, dp.p,
                                    dx.p + static_cast<size_t>(b0) * in, dt.p + b0, batch, in, partial.p, lossPartial.p);
-                hipLaunchKernelGGL(KMlpReduce, dim3(rgrid), dim3(256), 0, st.s, partial.p, lossPartial.p, blocks,
-                                   count, batch, dg.p, dl.p, 1);
                 ++step;
                 const float c1 = 1.0F - std::pow(0.9F, static_cast<float>(step));
                 const float c2 = 1.0F - std::pow(0.999F, static_cast<float>(step));
-                hipLaunchKernelGGL(KMlpAdam, dim3(agrid), dim3(256), 0, st.s, dp.p, dg.p, dm.p, dv.p, count, lr, 0.9F,
-                                   0.999F, 1e-8F, c1, c2);
+                hipLaunchKernelGGL(KMlpReduceAdam, dim3(rgrid), dim3(256), 0, st.s, partial.p, lossPartial.p, blocks,
+                                   count, batch, dp.p, dm.p, dv.p, dl.p, lr, 0.9F, 0.999F, 1e-8F, c1, c2);
                 ++batches;
             }
             BM25MI_MLP_HIP(hipGetLastError(), "MLP training launches");

@@ -1,4 +1,4 @@
-import os
+import ctypes
 import sys
 from pathlib import Path
 
@@ -16,11 +16,16 @@ def pytest_configure(config):
 
 
 def gpu_available():
+    """Whether HIP sees a GPU, asked of the ROCm HIP runtime the library links (libamdhip64.so.7)
+    rather than of torch: torch's wheel bundles a HIP runtime of its own, and a process that
+    initialises that one first and loads the library's afterwards gets 'no ROCm-capable device'
+    from every call of the second."""
     try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:  # noqa: BLE001
+        hip = ctypes.CDLL('libamdhip64.so.7')
+    except OSError:
         return False
+    count = ctypes.c_int(0)
+    return hip.hipGetDeviceCount(ctypes.byref(count)) == 0 and count.value > 0
 
 
 @pytest.fixture(scope='session')

@@ -50,8 +50,10 @@ def test_param_layout_and_init_are_seeded():
 
 @pytest.mark.gpu
 def test_gpu_forward_and_gradient_match_oracle(need_gpu):
-    for seed in (1, 2):
-        params, x, target = random_case(seed)
+    # Partial last tiles, feature counts other than 16 (up to the kernels' 32), and more tiles
+    # than the backward kernel's 512 workgroups (each then accumulates several tiles).
+    for seed, n, n_in in ((1, 300, 16), (2, 300, 16), (3, 64, 32), (4, 77, 5), (5, 40_000, 16)):
+        params, x, target = random_case(seed, n, n_in)
         y = learned.forward(params, x)
         assert np.allclose(y, mo.forward(params, x), rtol=1e-4, atol=1e-5)
         g, loss = learned.gradient(params, x, target)
