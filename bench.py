@@ -341,17 +341,23 @@ def learned_leg(args, index, offsets):
     data = learned.collect(s, qs, ks, batch=256)
     del qs
     t_collect = time.time() - t0
-    t1 = time.time()
-    model = learned.ThresholdModel(seed=0).fit(data, epochs=args.train_epochs, batch=args.train_batch, lr=2e-3,
-                                               seed=0)
-    t_fit = time.time() - t1
+    # Selection queries (to pick among the seeds' models) and calibration queries (for the kept
+    # model's safety factor): both held out from training and from the timed batches.
+    sq, sk = learned.make_queries(offsets, 4096, seed=300)
+    sdata = learned.collect(s, sq, sk, batch=256)
     vq, vk = learned.make_queries(offsets, 4096, seed=200)
     vdata = learned.collect(s, vq, vk, batch=256)
+    t1 = time.time()
+    model, candidates = learned.fit_best(data, sdata, seeds=tuple(range(args.train_models)),
+                                         fallback_rate=args.fallback_target, epochs=args.train_epochs,
+                                         batch=args.train_batch, lr=2e-3)
+    t_fit = time.time() - t1
     model.calibrate(vdata, fallback_rate=args.fallback_target)
     s.close()
     out = {'train_pairs': int(len(data.kth)), 'train_terms': '2-12 df-weighted', 'train_k': [10, 100],
            'draw_s': round(t_draw, 2), 'collect_s': round(t_collect, 2), 'fit_s': round(t_fit, 2),
            'epochs': args.train_epochs, 'batch': args.train_batch,
+           'candidates': [{'seed': c[0], 'selection_safety': c[1], 'loss_last': c[2]} for c in candidates],
            'loss_first_last': [float(model.losses[0]), float(model.losses[-1])], 'safety': model.safety,
            'inference': 'on the GPU inside every timed learned Search (threshold kernel before scoring)',
            'cases': []}
@@ -508,6 +514,8 @@ def main():
     ap.add_argument('--batched', action='store_true', help='add the 8.84M-passage uneven-batch runs')
     ap.add_argument('--train-queries', type=int, default=1_000_000, help='exact (query, K-th score) training pairs')
     ap.add_argument('--train-epochs', type=int, default=20)
+    ap.add_argument('--train-models', type=int, default=4,
+                    help='models trained from seeds 0..N-1; the tightest on held-out selection queries is kept')
     ap.add_argument('--train-batch', type=int, default=1024)
     ap.add_argument('--live-max-k', type=int, default=0, help='largest K scored on the live path (0 = library default)')
     ap.add_argument('--fallback-target', type=float, default=0.005,

@@ -153,6 +153,22 @@ class ThresholdModel:
         session.set_threshold_model(self.params, self.mean, self.scale, self.y_mean, self.y_scale, self.safety)
 
 
+def fit_best(data, select, seeds=(0, 1, 2, 3), fallback_rate=0.005, **fit_args):
+    """Trains one model per seed and keeps the one whose thresholds are tightest on held-out
+    selection queries: the largest safety factor at the target rerun rate (the thresholds'
+    average distance below the K-th scores, at the same share of reruns). The selection queries
+    are not the ones the kept model is calibrated on afterwards. Returns the model and one
+    (seed, selection safety, last epoch loss) row per candidate."""
+    rows, best = [], None
+    for seed in seeds:
+        model = ThresholdModel(seed=seed).fit(data, seed=seed, **fit_args)
+        safety = model.calibrate(select, fallback_rate)
+        rows.append((int(seed), float(safety), float(model.losses[-1])))
+        if best is None or safety > best[0]:
+            best = (safety, model)
+    return best[1], rows
+
+
 def compare(session, queries, k, thresholds):
     """Plain and pruned Search of the same queries: (mismatches, fallback flags, plain stats,
     pruned stats). Mismatches count differing doc IDs or FP32 score bits."""

@@ -308,10 +308,14 @@ __global__ void KMlpReduceAdam(const float *__restrict__ partial, const float *_
         lossOut[0] += MeanLoss(lossPartial, blocks, n);
 }
 
+// Workgroup partials of bm25mi_mlp_backward, shared by its calls: each call records scratchDone
+// after its last kernel, and the next call's stream waits on it (or the host does, before the
+// buffer grows), so calls on any streams take the buffer in turn without a host wait per call.
 namespace {
 std::mutex scratchMutex;
 float *scratch = nullptr;
 size_t scratchFloats = 0;
+hipEvent_t scratchDone = nullptr;
 } // namespace
 
 } // namespace bm25mi
@@ -348,19 +352,22 @@ extern "C" int bm25mi_mlp_backward(const float *params, const float *x, const fl
         const int64_t count = MlpParams(in_features);
         const int blocks = static_cast<int>(std::min<int64_t>(kBackwardGrid, (int64_t(n) + kTile - 1) / kTile));
         std::lock_guard<std::mutex> guard(scratchMutex);
+        auto s = static_cast<hipStream_t>(stream);
+        if (!scratchDone)
+            BM25MI_MLP_HIP(hipEventCreateWithFlags(&scratchDone, hipEventDisableTiming), "MLP scratch event");
         const size_t need = static_cast<size_t>(blocks) * (count + 1);
         if (need > scratchFloats) {
-            auto s = static_cast<hipStream_t>(stream);
-            BM25MI_MLP_HIP(hipStreamSynchronize(s), "MLP scratch sync");
+            BM25MI_MLP_HIP(hipEventSynchronize(scratchDone), "MLP scratch sync");
             if (scratch)
                 BM25MI_MLP_HIP(hipFree(scratch), "MLP scratch free");
             scratch = nullptr;
             scratchFloats = 0;
             BM25MI_MLP_HIP(hipMalloc(&scratch, need * sizeof(float)), "MLP scratch alloc");
             scratchFloats = need;
+        } else {
+            BM25MI_MLP_HIP(hipStreamWaitEvent(s, scratchDone, 0), "MLP scratch wait");
         }
         float *partial = scratch, *lossPartial = scratch + static_cast<size_t>(blocks) * count;
-        auto s = static_cast<hipStream_t>(stream);
         hipLaunchKernelGGL(KMlpBackward, dim3(blocks), dim3(kMlpThreads), 0, s, params, x, target, n, in_features,
                            partial, lossPartial);
         BM25MI_MLP_HIP(hipGetLastError(), "MLP backward launch");
@@ -368,8 +375,7 @@ extern "C" int bm25mi_mlp_backward(const float *params, const float *x, const fl
         hipLaunchKernelGGL(KMlpReduce, dim3(rgrid), dim3(256), 0, s, partial, lossPartial, blocks, count, n, grad,
                            loss_out);
         BM25MI_MLP_HIP(hipGetLastError(), "MLP reduce launch");
-        // The scratch buffer is reused by the next call: finish this one first.
-        BM25MI_MLP_HIP(hipStreamSynchronize(s), "MLP backward sync");
+        BM25MI_MLP_HIP(hipEventRecord(scratchDone, s), "MLP scratch record");
     });
 }
 

@@ -119,6 +119,26 @@ def test_learned_threshold_prunes_and_keeps_results(small_corpus, need_gpu):
         assert scanned['learned'] < min(scanned['lower'], scanned['plain']), scanned
 
 
+@pytest.mark.gpu
+def test_fit_best_keeps_the_tightest_model_on_selection_queries(small_corpus, need_gpu):
+    """Model selection over training seeds: the kept model is the candidate with the largest
+    safety factor on the selection queries, identical to a lone fit from that seed."""
+    corpus, index = small_corpus
+    with bm25mi.Session(index, max_queries=64, max_words=64 * 12, max_k=100) as s:
+        qs, ks = learned.make_queries(corpus.offsets, 1024, seed=100, k_choices=(10, 50))
+        data = learned.collect(s, qs, ks, batch=64)
+        sq, sk = learned.make_queries(corpus.offsets, 512, seed=300, k_choices=(10, 50))
+        select = learned.collect(s, sq, sk, batch=64)
+    model, rows = learned.fit_best(data, select, seeds=(0, 1, 2), fallback_rate=0.02, epochs=20, batch=128)
+    assert [r[0] for r in rows] == [0, 1, 2]
+    assert len({r[1] for r in rows}) > 1  # the seeds give different models
+    best = max(rows, key=lambda r: r[1])
+    assert model.safety == best[1]
+    lone = learned.ThresholdModel(seed=best[0]).fit(data, epochs=20, batch=128, seed=best[0])
+    assert np.array_equal(lone.params, model.params)
+    assert lone.calibrate(select, 0.02) == best[1]
+
+
 @pytest.fixture(scope='module')
 def trained_small(small_corpus):
     """A model trained on exact K-th scores of the 300K-doc corpus, calibrated on held-out
