@@ -342,7 +342,9 @@ int64_t bm25mi_mlp_param_count(int32_t in_features);
 int bm25mi_mlp_forward(const float *params, const float *x, int32_t n, int32_t in_features, float *y,
                        void *stream);
 /* Mean-squared-error loss and gradient: grad (same layout as params) +=
- * d/dparams of (1/n) * sum (y - target)^2 ; loss_out[0] = that mean. */
+ * d/dparams of (1/n) * sum (y - target)^2 ; loss_out[0] = that mean. Asynchronous on stream:
+ * calls share one buffer of workgroup partials, and each call's stream waits on the previous
+ * call's work (from any stream) before reusing it; the host waits only when the buffer grows. */
 int bm25mi_mlp_backward(const float *params, const float *x, const float *target, int32_t n, int32_t in_features,
                         float *grad, float *loss_out, void *stream);
 /* Adam step over count parameters; state m, v are device arrays of count floats. */
