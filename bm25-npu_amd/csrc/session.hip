@@ -689,9 +689,8 @@ void Session::Launch(Slot &slot, bool timed)
     // Device-time events only for timed Searches: the two records cost about 2 us per call.
     if (timed)
         BM25MI_HIP(hipEventRecord(slot.ev[0], slot.stream), "record start");
-    // Mixed batches launch directly: replaying the captured fork/join graph measured slower than
-    // the same launches issued on the two streams (Q64 x 2-12 terms: +0.35 ms end to end).
-    // Learned batches (mode 3) launch directly as well.
+    // Mixed batches (mode 1) launch directly; the others replay their graph when the session
+    // captured one (use_graph) and the Search is not timed.
     if (slot.mode != 1 && slot.exec[slot.mode] && !timed) {
         BM25MI_HIP(hipGraphLaunch(slot.exec[slot.mode], slot.stream), "replay Search graph");
         slot.finalGrid = cfg.max_queries;
